@@ -1,0 +1,77 @@
+/*
+ * heat_oracle.h -- CPU oracle for the matrix-free Q1 hexahedral heat solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * solver src/heat_solver_het.cu (single rank: the MPI slab decomposition
+ * collapses to one rank that owns every node).  Only tests/, the smoke check
+ * and the CPU-baseline leg of bench.py may load it, and only as the checker.
+ * The product (mnhack8_amd/) never links, imports or runs anything here.
+ *
+ * Every function cites the reference lines it restates.
+ */
+#ifndef HEAT_ORACLE_H
+#define HEAT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element kernel variants */
+#define HO_KERNEL_GAUSS  0   /* 2x2x2 Gauss on the fly, src/heat_solver_het.cu:310-338 */
+#define HO_KERNEL_FAST   1   /* 8x8 reference matrices, src/heat_solver_het.cu:340-349 */
+#define HO_KERNEL_STRUCT 2   /* closed-form Hamming-distance rows (the GPU's fast form) */
+
+/* source models */
+#define HO_SOURCE_GAUSSIAN 0 /* 1000 exp(-r^2/(2*0.1^2)), src/heat_solver_het.cu:229-237 */
+#define HO_SOURCE_CONST    1 /* Q = const (new: f = 1 workloads) */
+
+typedef struct ho_op ho_op;
+
+/* setup: src/heat_solver_het.cu:756-816 (load_fields replaced by caller arrays) */
+ho_op* ho_create(int n, double dt, const double* k, const double* rhoc,
+                 int kernel, int colored, int verify, int source_mode,
+                 double source_value, int detect);
+void   ho_destroy(ho_op* op);
+const char* ho_last_error(void);
+void   ho_set_threads(ho_op* op, int nthreads);
+void   ho_set_no_source(ho_op* op, int no_source);
+
+long   ho_nn(const ho_op* op);
+long   ho_ne(const ho_op* op);
+/* copies of internal vectors: 0 diag, 1 F, 2 wM, 3 wA, 4 ones_int */
+int    ho_get_vector(const ho_op* op, int which, double* out);
+void   ho_get_ref_matrices(const ho_op* op, double* Mh, double* Kh);
+void   ho_get_stats(const ho_op* op, double* out4); /* kmin kmax rhocmin rhocmax */
+
+/* HeatOperator methods, src/heat_solver_het.cu:885-988 */
+void   ho_apply_operator(ho_op* op, const double* x, double* y, double dt_eff,
+                         int allow_fi, int project);
+void   ho_build_rhs(ho_op* op, const double* Tn, double* b);
+double ho_dot(ho_op* op, const double* x, const double* y);
+void   ho_minmax(ho_op* op, const double* x, double* vmin, double* vmax);
+void   ho_apply_preconditioner(ho_op* op, const double* r, double* z);
+
+/* FP32 restatement of the structured operator (FP32 inputs, FP32 accumulation) */
+void   ho_apply_operator_f32(ho_op* op, const float* x, float* y, double dt_eff,
+                             int project);
+
+/* fault injection: src/heat_solver_het.cu:181-186, 369-374 */
+void   ho_arm_fault(ho_op* op, int level, long target, int bit);
+void   ho_flip_bit(double* v, long idx, int bit);
+void   ho_abft_take(ho_op* op, double* step_max, long* calls);
+
+/* pcg_solve, src/heat_solver_het.cu:994-1039.  hist (optional) receives
+ * (max_iter+1) rows of {rel_residual, alpha, beta}. */
+int    ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol,
+                    double* hist);
+
+/* initial conditions, src/heat_solver_het.cu:556-584.  mode 1 gaussian, 2 sine mode at t */
+void   ho_initial_condition(ho_op* op, double* v, int mode, double t, double alpha);
+
+/* the whole command-line program (src/heat_solver_het.cu:1355-1629) */
+int    ho_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
