@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the matrix-free Q1 heat solve on one MI355X.
+
+Workload (the N=128 heterogeneous configuration): unit cube, 128^3 Q1 hex
+elements (2,097,152 elements, 2,146,689 nodes), per-element conductivity
+kappa = exp(clip(g, -3, 3)) with g ~ N(0,1) i.i.d. from a counter-based
+generator keyed by (seed, element), rho*c = 1, f = 1, dt = 1e-4, FP64,
+colour-ordered scatter.
+
+A "step" is one operator apply y = P (M + dt kappa K) x (the PCG matvec),
+timed on the device with HIP events over K back-to-back launches.  value =
+nodes / time per apply (GDOF/s).  Also reported: the same with the caches
+flushed before every apply, time-to-1e-8 of the first time step's solve for
+plain FP64 PCG and for the FP32-refined solve, the roofline position of the
+matvec kernel, and the CPU oracle's matvec rate on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD specification
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "gather_matvec_n128_traffic.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the matvec kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    try:
+        with open(PROFILE_TRAFFIC) as f:
+            for line in f:
+                m = re.match(r"\s*traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n, k, rhoc, dt, reps=20, warm=3):
+    """The CPU oracle's colour-sweep matvec (same arithmetic), OpenMP threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import OracleOp
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="colored", source="const",
+                   source_value=1.0, threads=threads)
+    x = np.random.default_rng(0).standard_normal(orc.nn)
+    for _ in range(warm):
+        orc.apply(x, dt)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        orc.apply(x, dt)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": orc.nn / med / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} colour-sweep matvecs of the same N={n} mesh after {warm} warm-ups, median "
+                      f"{med * 1e3:.1f} ms per apply"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="run this many device copies of one node vector first (PMC byte-counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mnhack8_amd import fields
+    from mnhack8_amd.heat import HeatOperator
+
+    n, dt = a.n, 1e-4
+    log(f"generating fields N={n}")
+    k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=a.seed)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    log("setup")
+    op = HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="colored", source="const", source_value=1.0)
+    nn, ne = op.nn, op.ne
+    x = np.random.default_rng(a.seed).standard_normal(nn)
+    op.upload("p", x)
+
+    for _ in range(a.probe_copy):
+        op.copy("p", "r")
+    op.synchronize()
+    log(f"warmup {a.warmup}")
+    op.time_matvec("p", "Ap", max(1, a.warmup))
+    torch.cuda.synchronize()
+    log(f"timing {a.steps} applies")
+    t_host0 = time.perf_counter()
+    avg_ms, _ = op.time_matvec("p", "Ap", a.steps)
+    torch.cuda.synchronize()
+    host_s = time.perf_counter() - t_host0
+    gdofs = nn / (avg_ms * 1e-3) / 1e9
+    log(f"warm: {avg_ms * 1e3:.2f} us/apply  {gdofs:.1f} GDOF/s  (host wall {host_s * 1e3:.1f} ms)")
+    cold_ms, cold_min = op.time_matvec("p", "Ap", 20, flush_bytes=1 << 30)
+    log(f"cold (1 GiB flush before each): {cold_ms * 1e3:.2f} us/apply")
+
+    alg_bytes = 8 * (2 * nn + 2 * ne)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    achieved_cold = alg_bytes / (cold_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+
+    result = {
+        "metric": "PCG matvec GDOF/s at N=128 hex; time-to-1e-8, FP32-refined vs FP64",
+        "value": round(gdofs, 3),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": avg_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"N={n} Q1 hex unit cube, iid log-normal kappa clip 3, rho*c=1, f=1, dt=1e-4, "
+                               f"FP64 colour-ordered matvec y = P(M + dt kappa K)x",
+                   "n": n, "nodes": nn, "elements": ne, "scatter": "colored (node-centric, colour order)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
+                       "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
+        "algorithmic_bytes_per_apply": alg_bytes,
+    }
+
+    if not a.no_solve:
+        # time to relative residual 1e-8 for the first time step (T^0 = 0, b = dt F)
+        op.upload("T", np.zeros(nn))
+        op.build_rhs("T", "b")
+        op.synchronize()
+        # reference solution to a tight tolerance for the error figures
+        op.pcg_solve("b", "T", 5000, 1e-13)
+        xref = op.download("T")
+        res = {}
+        for label in ("fp64", "fp32_refined"):
+            best = None
+            for rep in range(3):
+                op.upload("T", np.zeros(nn))
+                op.synchronize()
+                t0 = time.perf_counter()
+                if label == "fp64":
+                    its = op.pcg_solve("b", "T", 5000, 1e-8)
+                    extra = {"iterations": its}
+                else:
+                    outer, inner = op.pcg_solve_mixed("b", "T", 50, 5000, 1e-8, 1e-4)
+                    extra = {"outer": outer, "inner_iterations": inner}
+                op.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3
+                if best is None or ms < best:
+                    best = ms
+            xs = op.download("T")
+            err = float(np.linalg.norm(xs - xref) / np.linalg.norm(xref))
+            res[label] = {"ms": round(best, 3), "rel_err_vs_fp64_1e-13": err, **extra}
+            log(f"time-to-1e-8 {label}: {best:.2f} ms {extra} err {err:.2e}")
+        res["speedup_fp32_refined"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
+        result["time_to_1e-8"] = res
+
+    if not a.no_cpu:
+        try:
+            log("CPU baseline")
+            result["cpu_baseline"] = cpu_baseline(n, k, rhoc, dt)
+        except Exception as ex:  # the oracle is optional on a user's machine
+            result["cpu_baseline"] = {"value": None, "error": str(ex)}
+    op.cleanup()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

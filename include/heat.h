@@ -1,0 +1,177 @@
+/*
+ * heat.h -- C ABI of the MI355X matrix-free Q1 hexahedral heat solver.
+ *
+ * One handle owns one structured mesh ([0,1]^3, N cells per axis), its
+ * per-element coefficients, the Jacobi diagonal, the source vector, the PCG
+ * work vectors and the ABFT checksum vectors, all in device memory of the
+ * current HIP device, plus one HIP stream on which every call is ordered.
+ * A handle is not thread-safe.  Every entry point returns 0 on success and a
+ * non-zero HEAT_E* code on failure; heat_last_error() then describes it.  No
+ * entry point aborts the host process.
+ *
+ * Vector arguments are DEVICE pointers of length heat_nn() (nodes, x slowest:
+ * id = ix*(N+1)^2 + iy*(N+1) + iz) holding FP64 values, unless noted.  They
+ * may be handle-owned (heat_vector) or caller-owned (hipMalloc, a torch
+ * tensor's data_ptr()).  Calls are asynchronous on the handle's stream except
+ * those that return a scalar to the host, which synchronise it.
+ *
+ * Each declaration cites the reference interface it replaces
+ * (src/heat_solver_het.cu of mariagarciapenalva/mnhack8).
+ */
+#ifndef MNHACK8_AMD_HEAT_H
+#define MNHACK8_AMD_HEAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEAT_OK          0
+#define HEAT_E_ARG       1   /* invalid argument */
+#define HEAT_E_HIP       2   /* HIP runtime error */
+#define HEAT_E_FIELD     3   /* non-positive or NaN coefficient */
+#define HEAT_E_SELFCHECK 4   /* reference matrices failed their closed-form check */
+#define HEAT_E_STATE     5   /* call not valid in the handle's state */
+
+/* --kernel */
+#define HEAT_KERNEL_GAUSS 0  /* 2x2x2 Gauss quadrature on the fly (elem_apply_gauss) */
+#define HEAT_KERNEL_FAST  1  /* element-congruent closed-form rows (elem_apply_fast) */
+/* --scatter */
+#define HEAT_SCATTER_ATOMIC  0
+#define HEAT_SCATTER_COLORED 1
+/* source models */
+#define HEAT_SOURCE_GAUSSIAN 0  /* 1000 exp(-|x-c|^2 / (2*0.1^2)) (source_at) */
+#define HEAT_SOURCE_CONST    1  /* Q = source_value everywhere */
+/* colored-scatter implementations (both give the same bits) */
+#define HEAT_COLORED_GATHER 0   /* one node-centric launch, contributions summed in colour order */
+#define HEAT_COLORED_SWEEP  1   /* eight element launches, one per colour */
+
+/* handle-owned vectors for heat_vector() */
+#define HEAT_VEC_T    0
+#define HEAT_VEC_B    1
+#define HEAT_VEC_R    2
+#define HEAT_VEC_Z    3
+#define HEAT_VEC_P    4
+#define HEAT_VEC_AP   5
+#define HEAT_VEC_DIAG 6
+#define HEAT_VEC_F    7
+#define HEAT_VEC_WM   8   /* M 1_int, unprojected (needs detect) */
+#define HEAT_VEC_WA   9   /* (M + dt K) 1_int, unprojected (needs detect) */
+#define HEAT_VEC_ONES 10  /* 1 on interior nodes, 0 on the boundary (needs detect) */
+
+typedef struct heat_op heat_op;
+
+typedef struct heat_config {
+    int    n;              /* cells per axis (cube [0,1]^3, h = 1/n) */
+    double dt;
+    int    kernel;         /* HEAT_KERNEL_* */
+    int    scatter;        /* HEAT_SCATTER_* */
+    int    verify;         /* 1: source off (Grid::verify) */
+    int    no_source;      /* 1: F is built but not added by build_rhs (Grid::no_source) */
+    int    source_mode;    /* HEAT_SOURCE_* */
+    double source_value;   /* Q for HEAT_SOURCE_CONST */
+    int    detect;         /* 1: build ABFT vectors and check every projected apply */
+} heat_config;
+
+/* Last error message of the calling thread. */
+const char* heat_last_error(void);
+
+/* HeatOperator::setup + load_fields + setup_detectors
+ * (src/heat_solver_het.cu:756-816, 711-753, 940-953).  k_cell and rhoc_cell
+ * are HOST arrays of N^3 doubles, element e = ix*N*N + iy*N + iz. */
+int heat_setup(const heat_config* cfg, const double* k_cell, const double* rhoc_cell,
+               heat_op** out);
+/* HeatOperator::cleanup (src/heat_solver_het.cu:818-824). */
+int heat_cleanup(heat_op* op);
+
+int heat_sizes(const heat_op* op, int64_t* nn, int64_t* ne);
+/* kmin, kmax, rhocmin, rhocmax (HeatOperator::kmin... src/heat_solver_het.cu:658) */
+int heat_field_stats(const heat_op* op, double out4[4]);
+/* Device pointer of a handle-owned vector (HEAT_VEC_*). */
+int heat_vector(heat_op* op, int which, double** dev_ptr);
+/* hipStream_t of the handle, as void*. */
+int heat_stream(heat_op* op, void** stream);
+int heat_synchronize(heat_op* op);
+int heat_upload(heat_op* op, double* dev_dst, const double* host_src, int64_t count);
+int heat_download(heat_op* op, double* host_dst, const double* dev_src, int64_t count);
+/* Select how --scatter colored is executed (HEAT_COLORED_*). */
+int heat_set_colored_impl(heat_op* op, int impl);
+
+/* HeatOperator::apply_operator (src/heat_solver_het.cu:885-921):
+ * y = P (rhoc M + dt_eff k K) x, P zeroing Dirichlet rows when project != 0,
+ * a pending level-R fault fires when allow_fi != 0, D1 accumulates when the
+ * handle detects and project != 0.  dt_eff must be 0 or the handle's dt. */
+int heat_apply_operator(heat_op* op, const double* x, double* y, double dt_eff,
+                        int allow_fi, int project);
+/* FP32 operator of the refined solve (new): x and y are FP32 device vectors,
+ * FP32 coefficients and FP32 accumulation, summed in colour order. */
+int heat_apply_operator_f32(heat_op* op, const float* x, float* y, double dt_eff, int project);
+/* HeatOperator::apply_matvec (src/heat_solver_het.cu:955). */
+int heat_apply_matvec(heat_op* op, const double* x, double* y);
+/* HeatOperator::apply_preconditioner (src/heat_solver_het.cu:957-960): z = r / diag. */
+int heat_apply_preconditioner(heat_op* op, const double* r, double* z);
+/* HeatOperator::build_rhs (src/heat_solver_het.cu:963-971): b = P M Tn + dt F. */
+int heat_build_rhs(heat_op* op, const double* Tn, double* b);
+/* HeatOperator::dot / norm2 (src/heat_solver_het.cu:973-988): fixed-order
+ * two-stage sum (256-node blocks, then a 1024-way strided sum and tree). */
+int heat_dot(heat_op* op, const double* x, const double* y, double* out);
+int heat_norm2(heat_op* op, const double* x, double* out);
+/* HeatOperator::minmax (src/heat_solver_het.cu:924-936). */
+int heat_minmax(heat_op* op, const double* x, double* vmin, double* vmax);
+/* apply_dirichlet_value_kernel (src/heat_solver_het.cu:427-434). */
+int heat_set_dirichlet(heat_op* op, double* v, double value);
+/* y = x  (copy_kernel, src/heat_solver_het.cu:444-447). */
+int heat_copy(heat_op* op, const double* x, double* y);
+/* y += a x  (axpy_kernel, src/heat_solver_het.cu:436-439). */
+int heat_axpy(heat_op* op, double a, const double* x, double* y);
+/* initial conditions (src/heat_solver_het.cu:554-584, 1115-1116):
+ * mode 0 zero, 1 Gaussian bump, 2 sine eigenmode exp(-3 pi^2 alpha t) sin sin sin. */
+int heat_initial_condition(heat_op* op, double* v, int mode, double t, double alpha);
+
+/* pcg_solve (src/heat_solver_het.cu:994-1039): Jacobi PCG on the handle's
+ * operator, x0 = x, stop when ||r||/||b|| < tol or after max_iter iterations
+ * or when |p.Ap| < 1e-30.  alpha/beta and every inner product stay on the
+ * device; the host polls a convergence flag every few iterations.  *iters
+ * receives the reference's return value.  hist (HOST, optional) receives
+ * 3*(max_iter+1) doubles: row 0 {||r0||/||b||, NaN, NaN}, row i
+ * {||r_i||/||b||, alpha_i, beta_i} for i = 1..*iters (the --diag-cg log). */
+int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double tol,
+                   int* iters, double* hist);
+
+/* FP32-refined solve (new; no reference counterpart): FP64 defect correction
+ * x += e, r = b - A x (FP64 operator) around an FP32 Jacobi-PCG on A e = r
+ * (FP32 operator and vectors, FP64 alpha/beta inner products) to the relative
+ * inner tolerance tol_inner, until ||b - A x|| / ||b|| < tol.  *outer and
+ * *inner_iters receive the outer and the total inner iteration counts. */
+int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
+                         int max_inner, double tol, double tol_inner,
+                         int* outer, int* inner_iters);
+
+/* Level R: the next apply with allow_fi != 0 flips bit `bit` of element
+ * `target`'s accumulator ye[0] before it is scattered, once
+ * (src/heat_solver_het.cu:369-374, 902, 1137). */
+int heat_arm_fault(heat_op* op, int level, int64_t target, int bit);
+int heat_disarm_fault(heat_op* op);
+/* Level G: XOR bit `bit` of v[idx] (flip_bit_global_kernel, src/heat_solver_het.cu:181-186). */
+int heat_flip_bit(heat_op* op, double* v, int64_t idx, int bit);
+/* D1 state accumulated since the last call (abft_step_max, abft_step_calls,
+ * src/heat_solver_het.cu:642-643, 1204, 1233); reading resets it. */
+int heat_abft_take(heat_op* op, double* step_max, int64_t* calls);
+/* Launch tuning (new): PCG iterations per convergence poll (1..256) and
+ * whether a chunk is replayed as a captured hipGraph. */
+int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);
+/* Benchmark helper (new): time `reps` back-to-back apply_matvec launches
+ * (y = P A x, dt_eff = dt, no fault, no D1) with HIP events on the handle's
+ * stream.  With flush_bytes > 0 a buffer of that size is overwritten before
+ * every rep, outside the timed interval, to evict the caches; each rep is then
+ * timed alone.  *avg_ms and *min_ms receive the mean and minimum per apply. */
+int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t flush_bytes,
+                     double* avg_ms, double* min_ms);
+/* The D1 threshold (chk_tol, src/heat_solver_het.cu:637) used by the driver. */
+int heat_abft_threshold(heat_op* op, double* chk_tol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

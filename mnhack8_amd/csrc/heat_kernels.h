@@ -1,0 +1,871 @@
+// heat_kernels.h -- gfx950 device code of the matrix-free Q1 heat solver.
+//
+// Mesh: N^3 hexahedral Q1 elements on [0,1]^3, nodes id = ix*(N+1)^2 + iy*(N+1)
+// + iz (z fastest), elements e = ix*N^2 + iy*N + iz (src/heat_solver_het.cu:
+// 202-205, 289-291).  All node kernels use 256-thread blocks over consecutive
+// node ids, which is also the first stage of every inner product, so fused
+// inner products give the same bits as the stand-alone two-stage dot
+// (src/heat_solver_het.cu:458-492).
+//
+// Arithmetic is compiled with -ffp-contract=off: every fused multiply-add is
+// an explicit fma(), so the CPU restatement reproduces each rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace heat {
+
+constexpr int NT = 256;   // node-kernel block (threads_node, src/heat_solver_het.cu:653)
+constexpr int ET = 128;   // element-kernel block (threads_elem, src/heat_solver_het.cu:654)
+constexpr int MAXSUM = 4; // inner products fused into one pass
+
+struct Grid {
+    int n;        // cells per axis
+    int nn1;      // nodes per axis
+    int sx, sy;   // node strides
+    int nn, ne;   // node and element counts (N <= 1000)
+    double h;
+};
+
+// Level-R arming passed by value (FIConfig, src/heat_solver_het.cu:172-179).
+struct FiArm {
+    int armed;
+    int bit;
+    long long target;   // element id; compared without a modulo (:370)
+};
+
+// Device-resident solver scalars: alpha/beta and the convergence state never
+// travel to the host inside a solve.
+struct DevState {
+    double rho, bnorm, alpha, beta, tol;
+    double abft_max, nwM, nwA;
+    double res[8];                 // results of stand-alone reductions
+    long long abft_calls;
+    int iter, max_iter, done, broke;
+    // mixed precision
+    double bnorm_outer, rnorm_outer, tol_outer, inner_bnorm;
+    int outer, outer_done, inner_total, pad2;
+};
+
+
+__device__ __forceinline__ double flip_bit(double v, int bit) {
+    unsigned long long w = (unsigned long long)__double_as_longlong(v);
+    w ^= (1ULL << bit);
+    return __longlong_as_double((long long)w);
+}
+
+// One row of the cube's element matrix (rhoc M + dt k K) applied to x_e.
+// S_d sums x over the element's nodes at Hamming distance d from the row
+// node.  M_ij = m3*{8,4,2,1}[d] and K_ij = k2*{-4,0,1,1}[d] are the closed
+// forms the reference checks its quadrature against
+// (src/heat_solver_het.cu:677-705); cr = rhoc*m3, ck = (dt*k)*k2.
+__device__ __forceinline__ double struct_row(double S0, double S1, double S2, double S3,
+                                             double cr, double ck) {
+    double t1 = fma(2.0, S2, S3);
+    double t2 = fma(4.0, S1, t1);
+    double P = fma(8.0, S0, t2);
+    double Q = fma(-4.0, S0, S2 + S3);
+    return fma(cr, P, ck * Q);
+}
+__device__ __forceinline__ float struct_row(float S0, float S1, float S2, float S3,
+                                            float cr, float ck) {
+    float t1 = fmaf(2.0f, S2, S3);
+    float t2 = fmaf(4.0f, S1, t1);
+    float P = fmaf(8.0f, S0, t2);
+    float Q = fmaf(-4.0f, S0, S2 + S3);
+    return fmaf(cr, P, ck * Q);
+}
+
+__device__ __forceinline__ bool node_is_boundary(const Grid& g, int ix, int iy, int iz) {
+    return ix == 0 || ix == g.n || iy == 0 || iy == g.n || iz == 0 || iz == g.n;
+}
+__device__ __forceinline__ void node_coords(const Grid& g, int nid, int& ix, int& iy, int& iz) {
+    ix = nid / g.sx;
+    int rem = nid - ix * g.sx;
+    iy = rem / g.nn1;
+    iz = rem - iy * g.nn1;
+}
+
+// ---------------------------------------------------------------------------
+// Block and grid reductions in the reference's order.
+// ---------------------------------------------------------------------------
+// 256-wide tree: s[t] += s[t+st] for st = 128..1 (src/heat_solver_het.cu:470-476).
+template <int K>
+__device__ __forceinline__ void block_tree(double (*sd)[NT], double v[K], double* partials,
+                                           int nb) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) sd[k][t] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if (t < st) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) sd[k][t] += sd[k][t + st];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) partials[(size_t)k * nb + blockIdx.x] = sd[k][0];
+    }
+}
+
+// Second stage (reduce_partials_kernel, src/heat_solver_het.cu:479-492): one
+// block of FT = 1024 threads, launched after the pass that wrote the
+// partials; thread t sums partials t, t+1024, ... in order, then a 1024-wide
+// tree.  (A last-block-arrives variant inside the pass would need a
+// device-scope release fence per block, which costs an L2 write-back per
+// block on this multi-XCD part.)
+constexpr int FT = 1024;
+template <int K>
+__device__ __forceinline__ void final_tree(const double* __restrict__ partials, int nb,
+                                           double out[K]) {
+    __shared__ double sf[K][FT];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double* pk = partials + (size_t)k * nb;
+        double acc = 0.0;
+        for (int i = t; i < nb; i += FT) acc += pk[i];
+        sf[k][t] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int st = FT / 2; st > 0; st >>= 1) {
+        if (t < st) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) sf[k][t] += sf[k][t + st];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = sf[k][0];
+}
+
+// D1 update from s = 1_int^T y, c = w^T x, xx = x^T x
+// (src/heat_solver_het.cu:908-919).
+__device__ __forceinline__ void abft_record(DevState* st, double s, double c, double xx,
+                                            double nw) {
+    double rel = fabs(s - c) / (nw * sqrt(xx) + 1e-300);
+    if (!isfinite(rel)) rel = 1e300;
+    st->abft_max = fmax(st->abft_max, rel);
+    st->abft_calls += 1;
+}
+
+// ---------------------------------------------------------------------------
+// Node-centric colour-ordered apply (the colored scatter in one launch).
+//
+// For node n, the element of colour c = (cx,cy,cz) that touches n is the one
+// whose lower corner has parities (cx,cy,cz); elements of one colour share no
+// node (src/heat_solver_het.cu:43-47), so the 8-launch colour sweep adds
+// exactly one contribution per colour to y[n], in colour order 0..7, starting
+// from 0.  This kernel forms the same eight contributions with the same
+// arithmetic and adds them in the same order: the result equals the sweep's
+// bit for bit, with no scatter, no zero pass and no atomics.
+//
+// The 27-node neighbourhood is read through a mirror: the stride sign along
+// an axis is flipped when the node's coordinate is odd, so that in mirrored
+// coordinates the colour-c element always lies on the side given by c's bits
+// and every register index is a compile-time constant.
+//
+// Boundary nodes get 0 (the Dirichlet projection); interior nodes always have
+// all eight elements.  Fused epilogue: inner products of the pass (p.Ap and
+// the three D1 sums) reduced in the reference's order; the last block turns
+// them into alpha or the D1 record.
+// ---------------------------------------------------------------------------
+enum EpiMode { EPI_NONE = 0, EPI_D1 = 1, EPI_PCG = 2, EPI_PCG_D1 = 3 };
+
+template <typename T, bool MASS, int EPI>
+__global__ __launch_bounds__(NT) void k_apply_gather(
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cr,
+    const T* __restrict__ ck, Grid g, FiArm fi, const double* __restrict__ w, double nw,
+    double* __restrict__ partials, DevState* st, int check_done) {
+    if (check_done && st->done) return;
+    __shared__ double sred[MAXSUM][NT];
+    const int nid = blockIdx.x * NT + threadIdx.x;
+    T acc = T(0);
+    T xv = T(0);
+    bool interior = false;
+    if (nid < g.nn) {
+        int ix, iy, iz;
+        node_coords(g, nid, ix, iy, iz);
+        xv = x[nid];
+        interior = !node_is_boundary(g, ix, iy, iz);
+        if (interior) {
+            const int px = ix & 1, py = iy & 1, pz = iz & 1;
+            const int DX = px ? -g.sx : g.sx;
+            const int DY = py ? -g.sy : g.sy;
+            const int DZ = pz ? -1 : 1;
+            T m[3][3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        m[a][b][c] = x[nid + (a - 1) * DX + (b - 1) * DY + (c - 1) * DZ];
+            const int par = (px << 2) | (py << 1) | pz;
+            const int nsq = g.n * g.n;
+#pragma unroll
+            for (int col = 0; col < 8; ++col) {
+                const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
+                // mirrored index of the element's other node along each axis
+                const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
+                const int sa = cx ^ px, sb = cy ^ py, sc = cz ^ pz;   // element below n?
+                const int e = (ix - sa) * nsq + (iy - sb) * g.n + (iz - sc);
+                const T S0 = m[1][1][1];
+                const T S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
+                const T S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
+                const T S3 = m[ox][oy][oz];
+                // mass-only apply (dt_eff = 0): the stiffness coefficient is 0
+                T v = struct_row(S0, S1, S2, S3, cr[e], MASS ? T(0) : ck[e]);
+                if (fi.armed && col == par && (long long)e == fi.target)
+                    v = (T)flip_bit((double)v, fi.bit);
+                acc += v;
+            }
+        }
+        y[nid] = acc;   // boundary rows: 0 (projection)
+    }
+    if (EPI == EPI_NONE) return;
+    constexpr int K = (EPI == EPI_PCG) ? 1 : (EPI == EPI_D1 ? 3 : 4);
+    double v[K];
+    const bool valid = nid < g.nn;
+    const double yd = valid ? (double)acc : 0.0;
+    const double xd = valid ? (double)xv : 0.0;
+    int k = 0;
+    if (EPI == EPI_PCG || EPI == EPI_PCG_D1) v[k++] = xd * yd;                  // p.Ap
+    if (EPI == EPI_D1 || EPI == EPI_PCG_D1) {
+        v[k++] = valid ? yd * (interior ? 1.0 : 0.0) : 0.0;                   // 1_int.y
+        v[k++] = valid ? w[nid] * xd : 0.0;                                    // w.x
+        v[k++] = xd * xd;                                                      // x.x
+    }
+    const int nb = gridDim.x;
+    block_tree<K>(sred, v, partials, nb);
+}
+
+// ---------------------------------------------------------------------------
+// Element-centric apply (operator_kernel, src/heat_solver_het.cu:351-378):
+// thread per element of one colour (color >= 0, plain +=) or of the whole
+// mesh (color < 0, atomic add).  KIND 0: closed-form rows, KIND 1: Gauss.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool resolve_element(const Grid& g, int tid, int color, int& e,
+                                                int& ix, int& iy, int& iz) {
+    if (color < 0) {
+        if (tid >= g.ne) return false;
+        e = tid;
+        ix = tid / (g.n * g.n);
+        iy = (tid / g.n) % g.n;
+        iz = tid % g.n;
+        return true;
+    }
+    const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
+    const int nxa = (g.n - cx + 1) / 2, nyb = (g.n - cy + 1) / 2, nzc = (g.n - cz + 1) / 2;
+    if (tid >= nxa * nyb * nzc) return false;
+    const int a = tid / (nyb * nzc), b = (tid / nzc) % nyb, c = tid % nzc;
+    ix = 2 * a + cx; iy = 2 * b + cy; iz = 2 * c + cz;
+    e = ix * g.n * g.n + iy * g.n + iz;
+    return true;
+}
+
+__constant__ int c_off[8][3] = {{0,0,0},{1,0,0},{1,1,0},{0,1,0},{0,0,1},{1,0,1},{1,1,1},{0,1,1}};
+
+__device__ __forceinline__ void shape_func_grad(double xi, double eta, double zeta, double N[8],
+                                                double dN[8][3]) {
+    const int sg[8][3] = {{-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},
+                          {-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1}};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        double sx = sg[i][0], sy = sg[i][1], sz = sg[i][2];
+        double a = 1 + sx * xi, b = 1 + sy * eta, c = 1 + sz * zeta;
+        N[i] = 0.125 * a * b * c;
+        dN[i][0] = 0.125 * sx * b * c;
+        dN[i][1] = 0.125 * a * sy * c;
+        dN[i][2] = 0.125 * a * b * sz;
+    }
+}
+
+// elem_apply_gauss, src/heat_solver_het.cu:310-338
+__device__ __forceinline__ void elem_gauss(const double xe[8], double ye[8], double k_e,
+                                           double rhoc_e, double dt, double h) {
+    const double det_J = h * h * h / 8.0;
+    const double inv_J = 2.0 / h;
+    const double gp = 1.0 / 1.7320508075688772;
+    const double pts[2] = {-gp, gp};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ye[i] = 0.0;
+    for (int gi = 0; gi < 2; ++gi)
+    for (int gj = 0; gj < 2; ++gj)
+    for (int gk = 0; gk < 2; ++gk) {
+        double N[8], dNref[8][3];
+        shape_func_grad(pts[gi], pts[gj], pts[gk], N, dNref);
+        double dNx[8], dNy[8], dNz[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            dNx[i] = inv_J * dNref[i][0];
+            dNy[i] = inv_J * dNref[i][1];
+            dNz[i] = inv_J * dNref[i][2];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            double Mx = 0.0, Kx = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                Mx += rhoc_e * N[i] * N[j] * xe[j];
+                Kx += k_e * (dNx[i] * dNx[j] + dNy[i] * dNy[j] + dNz[i] * dNz[j]) * xe[j];
+            }
+            ye[i] += det_J * (Mx + dt * Kx);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) { unsafeAtomicAdd(p, v); }
+
+template <typename T, int KIND, bool ATOMIC>
+__global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* __restrict__ y,
+                                                   const double* __restrict__ kcell,
+                                                   const double* __restrict__ rhoc, Grid g,
+                                                   double dt_eff, double m3, double k2, FiArm fi,
+                                                   int color) {
+    const int tid = blockIdx.x * ET + threadIdx.x;
+    int e, ix, iy, iz;
+    if (!resolve_element(g, tid, color, e, ix, iy, iz)) return;
+    int gid[8];
+    T xe[8], ye[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        gid[i] = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
+        xe[i] = x[gid[i]];
+    }
+    if (KIND == 1) {
+        double xd[8], yd[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) xd[i] = (double)xe[i];
+        elem_gauss(xd, yd, kcell[e], rhoc[e], dt_eff, g.h);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ye[i] = (T)yd[i];
+    } else {
+        T crv, ckv;
+        if (sizeof(T) == 8) {
+            crv = (T)(rhoc[e] * m3);
+            ckv = (T)((dt_eff * kcell[e]) * k2);
+        } else {
+            crv = (T)((float)rhoc[e] * (float)m3);
+            ckv = (T)(((float)dt_eff * (float)kcell[e]) * (float)k2);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int ox = c_off[i][0], oy = c_off[i][1], oz = c_off[i][2];
+            // local index of the node with the given offsets (inverse of c_off)
+            auto L = [](int a, int b, int c) { return (c << 2) | (b ? (a ? 2 : 3) : (a ? 1 : 0)); };
+            const T S0 = xe[i];
+            const T S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
+            const T S2 = (xe[L(1 - ox, 1 - oy, oz)] + xe[L(1 - ox, oy, 1 - oz)]) + xe[L(ox, 1 - oy, 1 - oz)];
+            const T S3 = xe[L(1 - ox, 1 - oy, 1 - oz)];
+            ye[i] = struct_row(S0, S1, S2, S3, crv, ckv);
+        }
+    }
+    if (fi.armed && (long long)e == fi.target) ye[0] = (T)flip_bit((double)ye[0], fi.bit);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (ATOMIC) atomic_add(&y[gid[i]], ye[i]);
+        else y[gid[i]] += ye[i];
+    }
+}
+
+// Projection and the inner products of an element-centric apply
+// (the tail of apply_operator, src/heat_solver_het.cu:904-919).
+template <typename T, int EPI>
+__global__ __launch_bounds__(NT) void k_apply_post(const T* __restrict__ x, T* __restrict__ y,
+                                                   Grid g, int project,
+                                                   const double* __restrict__ w, double nw,
+                                                   double* __restrict__ partials, DevState* st,
+                                                   int check_done) {
+    if (check_done && st->done) return;
+    __shared__ double sred[MAXSUM][NT];
+    const int nid = blockIdx.x * NT + threadIdx.x;
+    const bool valid = nid < g.nn;
+    double yd = 0.0, xd = 0.0;
+    bool interior = false;
+    if (valid) {
+        int ix, iy, iz;
+        node_coords(g, nid, ix, iy, iz);
+        interior = !node_is_boundary(g, ix, iy, iz);
+        T yv = y[nid];
+        if (project && !interior) { yv = T(0); y[nid] = yv; }
+        yd = (double)yv;
+        xd = (double)x[nid];
+    }
+    if (EPI == EPI_NONE) return;
+    constexpr int K = (EPI == EPI_PCG) ? 1 : (EPI == EPI_D1 ? 3 : 4);
+    double v[K];
+    int k = 0;
+    if (EPI == EPI_PCG || EPI == EPI_PCG_D1) v[k++] = xd * yd;
+    if (EPI == EPI_D1 || EPI == EPI_PCG_D1) {
+        v[k++] = valid ? yd * (interior ? 1.0 : 0.0) : 0.0;
+        v[k++] = valid ? w[nid] * xd : 0.0;
+        v[k++] = xd * xd;
+    }
+    const int nb = gridDim.x;
+    block_tree<K>(sred, v, partials, nb);
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector passes (src/heat_solver_het.cu:999-1036), alpha/beta on device.
+// ---------------------------------------------------------------------------
+// r = b - Ap; z = r/d; p = z; bnorm = ||b||; rho = r.z; hist row 0.
+__global__ __launch_bounds__(NT) void k_pcg_init(const double* __restrict__ b,
+                                                 const double* __restrict__ Ap,
+                                                 const double* __restrict__ d,
+                                                 double* __restrict__ r, double* __restrict__ z,
+                                                 double* __restrict__ p, int nn,
+                                                 double* __restrict__ partials) {
+    __shared__ double sred[MAXSUM][NT];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (i < nn) {
+        const double bi = b[i];
+        const double ri = fma(-1.0, Ap[i], bi);
+        const double zi = ri / d[i];
+        r[i] = ri; z[i] = zi; p[i] = zi;
+        v[0] = bi * bi; v[1] = ri * ri; v[2] = ri * zi;
+    }
+    const int nb = gridDim.x;
+    block_tree<3>(sred, v, partials, nb);
+}
+
+// x += alpha p; r -= alpha Ap; z = r/d; then ||r||, r.z -> convergence, beta.
+__global__ __launch_bounds__(NT) void k_pcg_update(double* __restrict__ x,
+                                                   const double* __restrict__ p,
+                                                   double* __restrict__ r,
+                                                   const double* __restrict__ Ap,
+                                                   const double* __restrict__ d,
+                                                   double* __restrict__ z, int nn,
+                                                   double* __restrict__ partials,
+                                                   const DevState* st) {
+    if (st->done) return;
+    __shared__ double sred[MAXSUM][NT];
+    const double alpha = st->alpha;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < nn) {
+        x[i] = fma(alpha, p[i], x[i]);
+        const double ri = fma(-alpha, Ap[i], r[i]);
+        const double zi = ri / d[i];
+        r[i] = ri; z[i] = zi;
+        v[0] = ri * ri; v[1] = ri * zi;
+    }
+    const int nb = gridDim.x;
+    block_tree<2>(sred, v, partials, nb);
+}
+
+// p = z + beta p (aypx_kernel as a fused multiply-add)
+__global__ __launch_bounds__(NT) void k_pcg_dir(double* __restrict__ p,
+                                                const double* __restrict__ z, int nn,
+                                                const DevState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < nn) p[i] = fma(beta, p[i], z[i]);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-refined solve: FP64 outer residual, FP32 inner PCG with FP64 dots.
+// ---------------------------------------------------------------------------
+// r64 = b - A64 x (gather), fused: ||b||^2, ||r64||^2, and the inner start
+// e = 0, r32 = (float) r64, z32 = r32/d32, p32 = z32, ||r32||^2, r32.z32.
+__global__ __launch_bounds__(NT) void k_mixed_residual(
+    const double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ cr,
+    const double* __restrict__ ck, Grid g, float* __restrict__ e32, float* __restrict__ r32,
+    float* __restrict__ z32, float* __restrict__ p32, const float* __restrict__ d32,
+    double* __restrict__ partials, DevState* st) {
+    __shared__ double sred[MAXSUM][NT];
+    const int nid = blockIdx.x * NT + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (nid < g.nn) {
+        int ix, iy, iz;
+        node_coords(g, nid, ix, iy, iz);
+        double acc = 0.0;
+        if (!node_is_boundary(g, ix, iy, iz)) {
+            const int px = ix & 1, py = iy & 1, pz = iz & 1;
+            const int DX = px ? -g.sx : g.sx, DY = py ? -g.sy : g.sy, DZ = pz ? -1 : 1;
+            double m[3][3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        m[a][bb][c] = x[nid + (a - 1) * DX + (bb - 1) * DY + (c - 1) * DZ];
+            const int nsq = g.n * g.n;
+#pragma unroll
+            for (int col = 0; col < 8; ++col) {
+                const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
+                const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
+                const int e = (ix - (cx ^ px)) * nsq + (iy - (cy ^ py)) * g.n + (iz - (cz ^ pz));
+                const double S0 = m[1][1][1];
+                const double S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
+                const double S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
+                const double S3 = m[ox][oy][oz];
+                acc += struct_row(S0, S1, S2, S3, cr[e], ck[e]);
+            }
+        }
+        const double bi = b[nid];
+        const double ri = fma(-1.0, acc, bi);
+        const float rf = (float)ri;
+        const float zf = rf / d32[nid];
+        e32[nid] = 0.0f; r32[nid] = rf; z32[nid] = zf; p32[nid] = zf;
+        v[0] = bi * bi; v[1] = ri * ri;
+        v[2] = (double)rf * (double)rf; v[3] = (double)rf * (double)zf;
+    }
+    const int nb = gridDim.x;
+    block_tree<4>(sred, v, partials, nb);
+}
+
+__global__ __launch_bounds__(NT) void k_mixed_update(float* __restrict__ e,
+                                                     const float* __restrict__ p,
+                                                     float* __restrict__ r,
+                                                     const float* __restrict__ Ap,
+                                                     const float* __restrict__ d,
+                                                     float* __restrict__ z, int nn,
+                                                     double* __restrict__ partials,
+                                                     DevState* st) {
+    if (st->done) return;
+    __shared__ double sred[MAXSUM][NT];
+    const double alpha = st->alpha;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < nn) {
+        e[i] = (float)fma(alpha, (double)p[i], (double)e[i]);
+        const float ri = (float)fma(-alpha, (double)Ap[i], (double)r[i]);
+        const float zi = ri / d[i];
+        r[i] = ri; z[i] = zi;
+        v[0] = (double)ri * (double)ri; v[1] = (double)ri * (double)zi;
+    }
+    const int nb = gridDim.x;
+    block_tree<2>(sred, v, partials, nb);
+}
+
+__global__ __launch_bounds__(NT) void k_mixed_dir(float* __restrict__ p,
+                                                  const float* __restrict__ z, int nn,
+                                                  const DevState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < nn) p[i] = (float)fma(beta, (double)p[i], (double)z[i]);
+}
+
+// x += e (FP64 correction)
+__global__ __launch_bounds__(NT) void k_mixed_correct(double* __restrict__ x,
+                                                      const float* __restrict__ e, int nn) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < nn) x[i] += (double)e[i];
+}
+
+// ---------------------------------------------------------------------------
+// Setup and utility kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* __restrict__ v, int n, T val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = val;
+}
+
+template <typename T>
+__global__ void k_dirichlet(T* __restrict__ v, Grid g, T val) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= g.nn) return;
+    int ix, iy, iz;
+    node_coords(g, i, ix, iy, iz);
+    if (node_is_boundary(g, ix, iy, iz)) v[i] = val;
+}
+
+__global__ void k_ones_interior(double* __restrict__ v, Grid g) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= g.nn) return;
+    int ix, iy, iz;
+    node_coords(g, i, ix, iy, iz);
+    v[i] = node_is_boundary(g, ix, iy, iz) ? 0.0 : 1.0;
+}
+
+// coefficients of the closed-form rows: cr = rhoc*m3, ck = (dt*k)*k2
+__global__ void k_coef64(const double* __restrict__ kcell, const double* __restrict__ rhoc,
+                         double dt, double m3, double k2, double* __restrict__ cr,
+                         double* __restrict__ ck, int ne) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    cr[e] = rhoc[e] * m3;
+    ck[e] = (dt * kcell[e]) * k2;
+}
+__global__ void k_coef32(const double* __restrict__ kcell, const double* __restrict__ rhoc,
+                         double dt, double m3, double k2, float* __restrict__ cr,
+                         float* __restrict__ ck, int ne) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    cr[e] = (float)rhoc[e] * (float)m3;
+    ck[e] = ((float)dt * (float)kcell[e]) * (float)k2;
+}
+
+template <typename TO, typename TI>
+__global__ void k_convert(TO* __restrict__ o, const TI* __restrict__ in, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (TO)in[i];
+}
+
+// diag_kernel, src/heat_solver_het.cu:409-422 (mdiag/kdiag: the diagonal
+// entries of M and K used by the selected element kernel)
+template <bool ATOMIC>
+__global__ __launch_bounds__(ET) void k_diag(double* __restrict__ d,
+                                             const double* __restrict__ kcell,
+                                             const double* __restrict__ rhoc, Grid g, double dt,
+                                             double mdiag, double kdiag, int color) {
+    const int tid = blockIdx.x * ET + threadIdx.x;
+    int e, ix, iy, iz;
+    if (!resolve_element(g, tid, color, e, ix, iy, iz)) return;
+    const double de = rhoc[e] * mdiag + (dt * kcell[e]) * kdiag;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int nid = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
+        if (ATOMIC) unsafeAtomicAdd(&d[nid], de); else d[nid] += de;
+    }
+}
+
+// source_kernel, src/heat_solver_het.cu:380-407 (mode 0 Gaussian, 1 constant)
+template <bool ATOMIC>
+__global__ __launch_bounds__(ET) void k_source(double* __restrict__ F, Grid g, int mode,
+                                               double qconst, int color) {
+    const int tid = blockIdx.x * ET + threadIdx.x;
+    int e, ix, iy, iz;
+    if (!resolve_element(g, tid, color, e, ix, iy, iz)) return;
+    const double x0 = ix * g.h, y0 = iy * g.h, z0 = iz * g.h;
+    const double det_J = g.h * g.h * g.h / 8.0;
+    const double gp = 1.0 / 1.7320508075688772;
+    const double pts[2] = {-gp, gp};
+    double Fe[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int gi = 0; gi < 2; ++gi)
+    for (int gj = 0; gj < 2; ++gj)
+    for (int gk = 0; gk < 2; ++gk) {
+        const double xg = x0 + (pts[gi] + 1.0) * 0.5 * g.h;
+        const double yg = y0 + (pts[gj] + 1.0) * 0.5 * g.h;
+        const double zg = z0 + (pts[gk] + 1.0) * 0.5 * g.h;
+        double Q;
+        if (mode == 1) Q = qconst;
+        else {
+            const double r2 = (xg - 0.5) * (xg - 0.5) + (yg - 0.5) * (yg - 0.5) + (zg - 0.5) * (zg - 0.5);
+            Q = 1000.0 * exp(-r2 / (2.0 * 0.1 * 0.1));
+        }
+        double N[8], dN[8][3];
+        shape_func_grad(pts[gi], pts[gj], pts[gk], N, dN);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Fe[i] += det_J * Q * N[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int nid = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
+        if (ATOMIC) unsafeAtomicAdd(&F[nid], Fe[i]); else F[nid] += Fe[i];
+    }
+}
+
+// gaussian_ic_kernel / exact_mode_kernel, src/heat_solver_het.cu:556-584
+__global__ void k_initial(double* __restrict__ v, Grid g, int mode, double t, double alpha) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= g.nn) return;
+    int ix, iy, iz;
+    node_coords(g, i, ix, iy, iz);
+    const double x = ix * g.h, y = iy * g.h, z = iz * g.h;
+    double val = 0.0;
+    if (mode == 1) {
+        const double r2 = (x - 0.5) * (x - 0.5) + (y - 0.5) * (y - 0.5) + (z - 0.5) * (z - 0.5);
+        val = exp(-r2 / (2.0 * 0.1 * 0.1));
+    } else if (mode == 2) {
+        const double pi = 3.14159265358979323846;
+        const double amp = exp(-3.0 * pi * pi * alpha * t);
+        val = amp * sin(pi * x) * sin(pi * y) * sin(pi * z);
+    }
+    if (node_is_boundary(g, ix, iy, iz)) val = 0.0;
+    v[i] = val;
+}
+
+__global__ void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, int n) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n) y[i] = fma(a, x[i], y[i]);
+}
+__global__ void k_jacobi(const double* __restrict__ r, const double* __restrict__ d,
+                         double* __restrict__ z, int n) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n) z[i] = r[i] / d[i];
+}
+
+// flip_bit_global_kernel, src/heat_solver_het.cu:181-186
+__global__ void k_flip(double* v, long long idx, int bit) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) v[idx] = flip_bit(v[idx], bit);
+}
+
+// Stand-alone dot (owned_dot_partial_kernel + reduce_partials_kernel) with
+// the result left in st->res[slot].
+__global__ __launch_bounds__(NT) void k_dot(const double* __restrict__ x,
+                                            const double* __restrict__ y, int n,
+                                            double* __restrict__ partials, DevState* st,
+                                            int slot) {
+    __shared__ double sred[MAXSUM][NT];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    double v[1] = {i < n ? x[i] * y[i] : 0.0};
+    const int nb = gridDim.x;
+    block_tree<1>(sred, v, partials, nb);
+}
+
+// owned_minmax_partial_kernel + reduce_minmax_kernel: results in res[slot], res[slot+1]
+__global__ __launch_bounds__(NT) void k_minmax(const double* __restrict__ x, int n,
+                                               double* __restrict__ partials, DevState* st,
+                                               int slot) {
+    __shared__ double smin[NT], smax[NT];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * NT + t;
+    smin[t] = (i < n) ? x[i] : 1e300;
+    smax[t] = (i < n) ? x[i] : -1e300;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (t < s) { smin[t] = fmin(smin[t], smin[t + s]); smax[t] = fmax(smax[t], smax[t + s]); }
+        __syncthreads();
+    }
+    const int nb = gridDim.x;
+    if (t == 0) { partials[blockIdx.x] = smin[0]; partials[nb + blockIdx.x] = smax[0]; }
+}
+
+// ---------------------------------------------------------------------------
+// Finalize kernels: one block of FT threads after each pass with partials.
+// ---------------------------------------------------------------------------
+// after an apply pass: alpha = rho / p.Ap (EPI_PCG*) and the D1 record (EPI_*D1)
+template <int EPI>
+__global__ __launch_bounds__(FT) void k_fin_apply(const double* __restrict__ partials, int nb,
+                                                  DevState* st, double nw, int check_done) {
+    if (check_done && st->done) return;
+    constexpr int K = (EPI == EPI_PCG) ? 1 : (EPI == EPI_D1 ? 3 : 4);
+    double out[K];
+    final_tree<K>(partials, nb, out);
+    if (threadIdx.x == 0) {
+        int o = 0;
+        if (EPI == EPI_PCG || EPI == EPI_PCG_D1) {
+            const double pAp = out[o++];
+            if (fabs(pAp) < 1e-30) { st->done = 1; st->broke = 1; }   // :1013
+            else st->alpha = st->rho / pAp;
+        }
+        if (EPI == EPI_D1 || EPI == EPI_PCG_D1) abft_record(st, out[o], out[o + 1], out[o + 2], nw);
+    }
+}
+
+// after k_pcg_init: bnorm (clamped), rho, history row 0 (:1002-1007)
+__global__ __launch_bounds__(FT) void k_fin_init(const double* __restrict__ partials, int nb,
+                                                 DevState* st, double* hist) {
+    double out[3];
+    final_tree<3>(partials, nb, out);
+    if (threadIdx.x == 0) {
+        double bnorm = sqrt(out[0]);
+        if (bnorm < 1e-30) bnorm = 1.0;
+        st->bnorm = bnorm;
+        st->rho = out[2];
+        st->iter = 0;
+        st->broke = 0;
+        st->done = (st->max_iter <= 0);
+        if (hist) {
+            hist[0] = sqrt(out[1]) / bnorm;
+            hist[1] = __builtin_nan("");
+            hist[2] = __builtin_nan("");
+        }
+    }
+}
+
+// after k_pcg_update: convergence, beta, rho, iteration count (:1020-1035)
+__global__ __launch_bounds__(FT) void k_fin_update(const double* __restrict__ partials, int nb,
+                                                   DevState* st, double* hist) {
+    if (st->done) return;
+    double out[2];
+    final_tree<2>(partials, nb, out);
+    if (threadIdx.x == 0) {
+        const double rnorm = sqrt(out[0]);
+        const bool conv = (rnorm / st->bnorm < st->tol);
+        double beta = 0.0;
+        if (!conv) {
+            const double rho_new = out[1];
+            beta = rho_new / st->rho;
+            st->rho = rho_new;
+        }
+        st->beta = beta;
+        st->iter += 1;
+        if (hist) {
+            double* h = hist + 3 * st->iter;
+            h[0] = rnorm / st->bnorm;
+            h[1] = st->alpha;
+            h[2] = beta;
+        }
+        if (conv || st->iter >= st->max_iter) st->done = 1;
+    }
+}
+
+__global__ __launch_bounds__(FT) void k_fin_dot(const double* __restrict__ partials, int nb,
+                                                DevState* st, int slot) {
+    double out[1];
+    final_tree<1>(partials, nb, out);
+    if (threadIdx.x == 0) st->res[slot] = out[0];
+}
+
+__global__ __launch_bounds__(FT) void k_fin_minmax(const double* __restrict__ partials, int nb,
+                                                   DevState* st, int slot) {
+    __shared__ double smin[FT], smax[FT];
+    const int t = threadIdx.x;
+    double a = 1e300, b = -1e300;
+    for (int i = t; i < nb; i += FT) { a = fmin(a, partials[i]); b = fmax(b, partials[nb + i]); }
+    smin[t] = a;
+    smax[t] = b;
+    __syncthreads();
+    for (int s = FT / 2; s > 0; s >>= 1) {
+        if (t < s) { smin[t] = fmin(smin[t], smin[t + s]); smax[t] = fmax(smax[t], smax[t + s]); }
+        __syncthreads();
+    }
+    if (t == 0) { st->res[slot] = smin[0]; st->res[slot + 1] = smax[0]; }
+}
+
+// after k_mixed_residual: outer convergence and the inner solve's start
+__global__ __launch_bounds__(FT) void k_fin_mixed_resid(const double* __restrict__ partials, int nb,
+                                                        DevState* st) {
+    double out[4];
+    final_tree<4>(partials, nb, out);
+    if (threadIdx.x == 0) {
+        double bn = sqrt(out[0]);
+        if (bn < 1e-30) bn = 1.0;
+        st->bnorm_outer = bn;
+        st->rnorm_outer = sqrt(out[1]);
+        st->outer_done = (st->rnorm_outer / bn < st->tol_outer);
+        double ib = sqrt(out[2]);
+        if (ib < 1e-30) ib = 1.0;
+        st->bnorm = ib;          // the inner tolerance is relative to the inner right-hand side
+        st->rho = out[3];
+        st->iter = 0;
+        st->broke = 0;
+        st->done = st->outer_done || (st->max_iter <= 0);
+    }
+}
+
+// after k_mixed_update: inner convergence and beta
+__global__ __launch_bounds__(FT) void k_fin_mixed_update(const double* __restrict__ partials, int nb,
+                                                         DevState* st) {
+    if (st->done) return;
+    double out[2];
+    final_tree<2>(partials, nb, out);
+    if (threadIdx.x == 0) {
+        const double rnorm = sqrt(out[0]);
+        const bool conv = (rnorm / st->bnorm < st->tol);
+        double beta = 0.0;
+        if (!conv) {
+            beta = out[1] / st->rho;
+            st->rho = out[1];
+        }
+        st->beta = beta;
+        st->iter += 1;
+        st->inner_total += 1;
+        if (conv || st->iter >= st->max_iter) st->done = 1;
+    }
+}
+
+}  // namespace heat

@@ -1,0 +1,892 @@
+// heat_lib.hip -- host side of the MI355X heat solver: the C ABI of heat.h.
+//
+// One handle = one mesh on the current HIP device with its own stream.  The
+// hot path (operator apply, PCG vector passes, inner products, ABFT) runs in
+// the kernels of heat_kernels.h; alpha, beta and the convergence state stay
+// in device memory and the host only polls a flag every few iterations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "heat.h"
+#include "heat_kernels.h"
+
+using namespace heat;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(HEAT_E_HIP, "%s:%d: %s: %s", __FILE__, __LINE__, #call,         \
+                        hipGetErrorString(e_));                                         \
+    } while (0)
+
+#define CHECK_LAUNCH() HIP_TRY(hipGetLastError())
+
+inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+struct heat_op {
+    heat_config cfg;
+    Grid g;
+    hipStream_t stream = nullptr;
+    int colored_impl = HEAT_COLORED_GATHER;
+    int pcg_chunk = 8;
+    int use_graph = 1;
+    double m3 = 0, k2 = 0;
+    double Mh[64], Kh[64];
+    double stats[4];
+    double chk_tol = 1e-10;
+    double nwM = 0, nwA = 0;   // ||M 1_int||, ||A 1_int||
+    unsigned nbn = 0;          // node blocks
+    // device memory
+    double* kcell = nullptr;
+    double* rhoc = nullptr;
+    double* cr = nullptr;
+    double* ck = nullptr;
+    float* cr32 = nullptr;
+    float* ck32 = nullptr;
+    double* vec[11] = {};
+    double* partials = nullptr;
+    DevState* st = nullptr;
+    DevState* hst = nullptr;    // pinned host mirror
+    int* hflag = nullptr;       // pinned done-flag mailboxes
+    double* dhist = nullptr;
+    int hist_cap = 0;
+    double* cur_hist = nullptr;  // history rows written by the running solve, or null
+    // FP32 inner-solve vectors
+    float *e32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *d32 = nullptr;
+    // level R arming (host copy, passed by value)
+    FiArm fi{0, 0, 0};
+    int fi_level = 0;
+    // captured PCG chunk
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+// build_reference_matrices (src/heat_solver_het.cu:661-708): 2x2x2 Gauss
+// integration of the Q1 mass and stiffness matrices and their closed-form
+// checks.  Also sets the closed-form entries used by the fast rows.
+int build_reference_matrices(heat_op* op) {
+    const double h = op->g.h;
+    const double detJ = h * h * h / 8.0;
+    const double iJ = 2.0 / h;
+    const double gp = 1.0 / 1.7320508075688772;
+    const double pts[2] = {-gp, gp};
+    static const int sg[8][3] = {{-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},
+                                 {-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1}};
+    double M[64] = {0}, K[64] = {0};
+    for (int a = 0; a < 2; ++a) for (int b = 0; b < 2; ++b) for (int c = 0; c < 2; ++c) {
+        double N[8], dN[8][3];
+        for (int i = 0; i < 8; ++i) {
+            double u = 1 + sg[i][0] * pts[a], v = 1 + sg[i][1] * pts[b], w = 1 + sg[i][2] * pts[c];
+            N[i] = 0.125 * u * v * w;
+            dN[i][0] = 0.125 * sg[i][0] * v * w;
+            dN[i][1] = 0.125 * u * sg[i][1] * w;
+            dN[i][2] = 0.125 * u * v * sg[i][2];
+        }
+        for (int i = 0; i < 8; ++i) for (int j = 0; j < 8; ++j) {
+            M[i * 8 + j] += detJ * N[i] * N[j];
+            K[i * 8 + j] += detJ * (iJ * dN[i][0] * iJ * dN[j][0] + iJ * dN[i][1] * iJ * dN[j][1] +
+                                    iJ * dN[i][2] * iJ * dN[j][2]);
+        }
+    }
+    const double V = h * h * h;
+    const double Kd = (h * h / h + h * h / h + h * h / h) / 9.0;
+    double errM = 0, sumM = 0, errKrow = 0, errKdiag = 0, errSym = 0;
+    for (int i = 0; i < 8; ++i) {
+        double rs = 0;
+        for (int j = 0; j < 8; ++j) {
+            int d = (sg[i][0] != sg[j][0]) + (sg[i][1] != sg[j][1]) + (sg[i][2] != sg[j][2]);
+            errM = std::fmax(errM, std::fabs(M[i * 8 + j] - V / 216.0 * (8 >> d)) / (V / 216.0 * 8));
+            sumM += M[i * 8 + j];
+            rs += K[i * 8 + j];
+            errSym = std::fmax(errSym, std::fabs(K[i * 8 + j] - K[j * 8 + i]) + std::fabs(M[i * 8 + j] - M[j * 8 + i]));
+        }
+        errKrow = std::fmax(errKrow, std::fabs(rs) / Kd);
+        errKdiag = std::fmax(errKdiag, std::fabs(K[i * 9] - Kd) / Kd);
+    }
+    const double errSum = std::fabs(sumM - V) / V;
+    memcpy(op->Mh, M, sizeof M);
+    memcpy(op->Kh, K, sizeof K);
+    op->m3 = (h * h * h) / 216.0;
+    op->k2 = -h / 12.0;
+    if (!(errM < 1e-12 && errSum < 1e-12 && errKrow < 1e-12 && errKdiag < 1e-12 && errSym < 1e-12 * Kd))
+        return fail(HEAT_E_SELFCHECK,
+                    "Reference matrices: M pattern %.1e, sum(M)-V %.1e, K rowsum %.1e, K diag %.1e, "
+                    "symmetry %.1e -> FAIL", errM, errSum, errKrow, errKdiag, errSym);
+    return HEAT_OK;
+}
+
+bool gather_path(const heat_op* op) {
+    return op->cfg.kernel == HEAT_KERNEL_FAST && op->cfg.scatter == HEAT_SCATTER_COLORED &&
+           op->colored_impl == HEAT_COLORED_GATHER;
+}
+
+// y = A x through the element-centric kernels (colour sweep or atomic),
+// without projection.
+int element_apply(heat_op* op, const double* x, double* y, double dt_eff, FiArm f) {
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    k_fill<double><<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, 0.0);
+    CHECK_LAUNCH();
+    const bool gauss = op->cfg.kernel == HEAT_KERNEL_GAUSS;
+    if (op->cfg.scatter == HEAT_SCATTER_ATOMIC) {
+        if (gauss)
+            k_apply_elem<double, 1, true><<<blocks(g.ne, ET), ET, 0, s>>>(x, y, op->kcell, op->rhoc, g, dt_eff, op->m3, op->k2, f, -1);
+        else
+            k_apply_elem<double, 0, true><<<blocks(g.ne, ET), ET, 0, s>>>(x, y, op->kcell, op->rhoc, g, dt_eff, op->m3, op->k2, f, -1);
+        CHECK_LAUNCH();
+    } else {
+        for (int c = 0; c < 8; ++c) {
+            const int cx = (c >> 2) & 1, cy = (c >> 1) & 1, cz = c & 1;
+            const long long cnt = (long long)((g.n - cx + 1) / 2) * ((g.n - cy + 1) / 2) * ((g.n - cz + 1) / 2);
+            if (cnt == 0) continue;
+            if (gauss)
+                k_apply_elem<double, 1, false><<<blocks(cnt, ET), ET, 0, s>>>(x, y, op->kcell, op->rhoc, g, dt_eff, op->m3, op->k2, f, c);
+            else
+                k_apply_elem<double, 0, false><<<blocks(cnt, ET), ET, 0, s>>>(x, y, op->kcell, op->rhoc, g, dt_eff, op->m3, op->k2, f, c);
+            CHECK_LAUNCH();
+        }
+    }
+    return HEAT_OK;
+}
+
+// alpha and/or the D1 record from the partials of an apply pass
+int launch_fin_apply(heat_op* op, int epi, double nw, int check_done) {
+    const int nb = (int)op->nbn;
+    hipStream_t s = op->stream;
+    switch (epi) {
+        case EPI_NONE: return HEAT_OK;
+        case EPI_D1: k_fin_apply<EPI_D1><<<1, FT, 0, s>>>(op->partials, nb, op->st, nw, check_done); break;
+        case EPI_PCG: k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, nb, op->st, nw, check_done); break;
+        default: k_fin_apply<EPI_PCG_D1><<<1, FT, 0, s>>>(op->partials, nb, op->st, nw, check_done); break;
+    }
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+// apply_operator with an epilogue: EPI_NONE / EPI_D1 outside PCG, EPI_PCG /
+// EPI_PCG_D1 inside (check_done: skip once the solve has finished).
+int apply_impl(heat_op* op, const double* x, double* y, double dt_eff, bool allow_fi,
+               bool project, int epi, int check_done) {
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    FiArm f = op->fi;
+    if (!allow_fi || op->fi_level != 2) f.armed = 0;
+    const bool mass = (dt_eff == 0.0);
+    if (!mass && dt_eff != op->cfg.dt)
+        return fail(HEAT_E_ARG, "dt_eff must be 0 or the handle's dt (%g), got %g", op->cfg.dt, dt_eff);
+    const double* w = mass ? op->vec[HEAT_VEC_WM] : op->vec[HEAT_VEC_WA];
+    const int use_d1 = (epi == EPI_D1 || epi == EPI_PCG_D1);
+    if (use_d1 && !w) return fail(HEAT_E_STATE, "D1 requested without detector vectors");
+    // ||w|| was read back to the host at the end of setup
+    const double nw = use_d1 ? (mass ? op->nwM : op->nwA) : 0.0;
+    int rc0 = HEAT_OK;
+    if (gather_path(op) && project) {
+#define GATHER(MASSV, EPIV)                                                                          \
+    k_apply_gather<double, MASSV, EPIV><<<op->nbn, NT, 0, s>>>(x, y, op->cr, op->ck, g, f, w, nw,     \
+                                                              op->partials, op->st, check_done)
+        if (mass) {
+            switch (epi) {
+                case EPI_NONE: GATHER(true, EPI_NONE); break;
+                case EPI_D1: GATHER(true, EPI_D1); break;
+                case EPI_PCG: GATHER(true, EPI_PCG); break;
+                default: GATHER(true, EPI_PCG_D1); break;
+            }
+        } else {
+            switch (epi) {
+                case EPI_NONE: GATHER(false, EPI_NONE); break;
+                case EPI_D1: GATHER(false, EPI_D1); break;
+                case EPI_PCG: GATHER(false, EPI_PCG); break;
+                default: GATHER(false, EPI_PCG_D1); break;
+            }
+        }
+#undef GATHER
+        CHECK_LAUNCH();
+        rc0 = launch_fin_apply(op, epi, nw, check_done);
+        if (rc0) return rc0;
+    } else {
+        if (check_done) return fail(HEAT_E_STATE, "internal: element path inside a captured solve");
+        int rc = element_apply(op, x, y, dt_eff, f);
+        if (rc) return rc;
+        const int pj = project ? 1 : 0;
+        switch (epi) {
+            case EPI_NONE:
+                if (project) k_apply_post<double, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0);
+                break;
+            case EPI_D1: k_apply_post<double, EPI_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
+            case EPI_PCG: k_apply_post<double, EPI_PCG><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
+            default: k_apply_post<double, EPI_PCG_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
+        }
+        CHECK_LAUNCH();
+        rc0 = launch_fin_apply(op, epi, nw, 0);
+        if (rc0) return rc0;
+    }
+    if (allow_fi && op->fi.armed && op->fi_level == 2) op->fi.armed = 0;   // one-shot (:902)
+    return HEAT_OK;
+}
+
+int read_state(heat_op* op) {
+    HIP_TRY(hipMemcpyAsync(op->hst, op->st, sizeof(DevState), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
+__global__ void k_pcg_params(DevState* st, double tol, int max_iter) {
+    st->tol = tol;
+    st->max_iter = max_iter;
+}
+__global__ void k_mixed_params(DevState* st, double tol, double tol_inner, int max_inner) {
+    st->tol_outer = tol;
+    st->tol = tol_inner;
+    st->max_iter = max_inner;
+    st->inner_total = 0;
+}
+__global__ void k_abft_reset(DevState* st) {
+    st->abft_max = 0.0;
+    st->abft_calls = 0;
+}
+__global__ void k_set_norms(DevState* st, int slotM, int slotA) {
+    st->nwM = sqrt(st->res[slotM]);
+    st->nwA = sqrt(st->res[slotA]);
+}
+
+// One PCG iteration: Ap = A p (+ p.Ap -> alpha), x/r/z update (-> beta,
+// convergence), p = z + beta p.  Every kernel returns at once when done.
+int launch_iteration(heat_op* op, double* x, bool detect) {
+    hipStream_t s = op->stream;
+    double* p = op->vec[HEAT_VEC_P];
+    double* Ap = op->vec[HEAT_VEC_AP];
+    double* r = op->vec[HEAT_VEC_R];
+    double* z = op->vec[HEAT_VEC_Z];
+    const double* d = op->vec[HEAT_VEC_DIAG];
+    int rc = apply_impl(op, p, Ap, op->cfg.dt, false, true, detect ? EPI_PCG_D1 : EPI_PCG, 1);
+    if (rc) return rc;
+    k_pcg_update<<<op->nbn, NT, 0, s>>>(x, p, r, Ap, d, z, op->g.nn, op->partials, op->st);
+    CHECK_LAUNCH();
+    k_fin_update<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, op->cur_hist);
+    CHECK_LAUNCH();
+    k_pcg_dir<<<op->nbn, NT, 0, s>>>(p, z, op->g.nn, op->st);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int launch_chunk(heat_op* op, double* x, bool detect) {
+    const void* key[4] = {x, (const void*)(intptr_t)(detect ? 1 : 0),
+                          (const void*)(intptr_t)op->pcg_chunk, op->cur_hist};
+    if (op->use_graph) {
+        if (!op->gexec || memcmp(key, op->graph_key, sizeof key) != 0) {
+            if (op->gexec) { (void)hipGraphExecDestroy(op->gexec); op->gexec = nullptr; }
+            if (op->graph) { (void)hipGraphDestroy(op->graph); op->graph = nullptr; }
+            HIP_TRY(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+            int rc = HEAT_OK;
+            for (int i = 0; i < op->pcg_chunk && rc == HEAT_OK; ++i) rc = launch_iteration(op, x, detect);
+            hipGraph_t graph = nullptr;
+            hipError_t ec = hipStreamEndCapture(op->stream, &graph);
+            if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+            if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+            op->graph = graph;
+            HIP_TRY(hipGraphInstantiate(&op->gexec, op->graph, nullptr, nullptr, 0));
+            memcpy(op->graph_key, key, sizeof key);
+        }
+        HIP_TRY(hipGraphLaunch(op->gexec, op->stream));
+        return HEAT_OK;
+    }
+    for (int i = 0; i < op->pcg_chunk; ++i) {
+        int rc = launch_iteration(op, x, detect);
+        if (rc) return rc;
+    }
+    return HEAT_OK;
+}
+
+// Launch chunks until the device reports done; the flags of chunk k are read
+// while chunk k+1 runs.
+template <typename LaunchFn>
+int run_until_done(heat_op* op, int max_iter, LaunchFn launch) {
+    const int max_chunks = max_iter / (op->pcg_chunk > 0 ? op->pcg_chunk : 1) + 3;
+    int k = 0;
+    int rc = launch();
+    if (rc) return rc;
+    int* slot[2] = {&op->hflag[0], &op->hflag[1]};   // pinned mailboxes, one per chunk in flight
+    HIP_TRY(hipMemcpyAsync(slot[0], &op->st->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(hipEventRecord(op->ev[0], op->stream));
+    for (;;) {
+        if (++k > max_chunks) return fail(HEAT_E_STATE, "PCG did not report completion");
+        rc = launch();
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(slot[k & 1], &op->st->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(hipEventRecord(op->ev[k & 1], op->stream));
+        HIP_TRY(hipEventSynchronize(op->ev[(k - 1) & 1]));
+        if (*slot[(k - 1) & 1]) break;
+    }
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
+void free_all(heat_op* op) {
+    if (op->gexec) (void)hipGraphExecDestroy(op->gexec);
+    if (op->graph) (void)hipGraphDestroy(op->graph);
+    for (auto& e : op->ev) if (e) (void)hipEventDestroy(e);
+    (void)hipFree(op->kcell); (void)hipFree(op->rhoc); (void)hipFree(op->cr); (void)hipFree(op->ck);
+    (void)hipFree(op->cr32); (void)hipFree(op->ck32);
+    for (auto& v : op->vec) (void)hipFree(v);
+    (void)hipFree(op->partials); (void)hipFree(op->st); (void)hipFree(op->dhist);
+    (void)hipFree(op->e32); (void)hipFree(op->r32); (void)hipFree(op->z32); (void)hipFree(op->p32); (void)hipFree(op->ap32); (void)hipFree(op->d32);
+    if (op->hst) (void)hipHostFree(op->hst);
+    if (op->hflag) (void)hipHostFree(op->hflag);
+    if (op->stream) (void)hipStreamDestroy(op->stream);
+}
+
+int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat_op* op) {
+    op->cfg = *cfg;
+    const int n = cfg->n;
+    Grid& g = op->g;
+    g.n = n; g.nn1 = n + 1; g.sy = n + 1; g.sx = (n + 1) * (n + 1);
+    g.nn = (n + 1) * (n + 1) * (n + 1); g.ne = n * n * n; g.h = 1.0 / n;
+    op->nbn = blocks(g.nn, NT);
+    // load_fields checks (src/heat_solver_het.cu:734-746)
+    double st4[4] = {k[0], k[0], rhoc[0], rhoc[0]};
+    for (int e = 0; e < g.ne; ++e) {
+        if (!(k[e] > 0) || !(rhoc[e] > 0))
+            return fail(HEAT_E_FIELD, "rank 0: non-positive or NaN coefficient at element %d", e);
+        st4[0] = std::fmin(st4[0], k[e]); st4[1] = std::fmax(st4[1], k[e]);
+        st4[2] = std::fmin(st4[2], rhoc[e]); st4[3] = std::fmax(st4[3], rhoc[e]);
+    }
+    memcpy(op->stats, st4, sizeof st4);
+    int rc = build_reference_matrices(op);
+    if (rc) return rc;
+    HIP_TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&op->ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&op->ev[1], hipEventDisableTiming));
+    const size_t nb8 = sizeof(double) * (size_t)g.nn, eb8 = sizeof(double) * (size_t)g.ne;
+    HIP_TRY(hipMalloc(&op->kcell, eb8));
+    HIP_TRY(hipMalloc(&op->rhoc, eb8));
+    HIP_TRY(hipMalloc(&op->cr, eb8));
+    HIP_TRY(hipMalloc(&op->ck, eb8));
+    for (int v = HEAT_VEC_T; v <= HEAT_VEC_F; ++v) HIP_TRY(hipMalloc(&op->vec[v], nb8));
+    HIP_TRY(hipMalloc(&op->partials, sizeof(double) * MAXSUM * (size_t)op->nbn));
+    HIP_TRY(hipMalloc(&op->st, sizeof(DevState)));
+    HIP_TRY(hipHostMalloc(&op->hst, sizeof(DevState), hipHostMallocDefault));
+    memset(op->hst, 0, sizeof(DevState));
+    HIP_TRY(hipHostMalloc(&op->hflag, 2 * sizeof(int), hipHostMallocDefault));
+    hipStream_t s = op->stream;
+    HIP_TRY(hipMemcpyAsync(op->st, op->hst, sizeof(DevState), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(op->kcell, k, eb8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(op->rhoc, rhoc, eb8, hipMemcpyHostToDevice, s));
+    k_coef64<<<blocks(g.ne, 256), 256, 0, s>>>(op->kcell, op->rhoc, cfg->dt, op->m3, op->k2, op->cr, op->ck, g.ne);
+    CHECK_LAUNCH();
+    for (int v = HEAT_VEC_T; v <= HEAT_VEC_F; ++v) {
+        k_fill<double><<<blocks(g.nn, NT), NT, 0, s>>>(op->vec[v], g.nn, 0.0);
+        CHECK_LAUNCH();
+    }
+    // Jacobi diagonal (src/heat_solver_het.cu:798-804)
+    const bool gauss = cfg->kernel == HEAT_KERNEL_GAUSS;
+    const double mdiag = gauss ? op->Mh[0] : 8.0 * op->m3;
+    const double kdiag = gauss ? op->Kh[0] : -4.0 * op->k2;
+    double* d = op->vec[HEAT_VEC_DIAG];
+    double* F = op->vec[HEAT_VEC_F];
+    const bool atomic = cfg->scatter == HEAT_SCATTER_ATOMIC;
+    for (int c = atomic ? -1 : 0; c < (atomic ? 0 : 8); ++c) {
+        long long cnt = g.ne;
+        if (c >= 0) {
+            const int cx = (c >> 2) & 1, cy = (c >> 1) & 1, cz = c & 1;
+            cnt = (long long)((n - cx + 1) / 2) * ((n - cy + 1) / 2) * ((n - cz + 1) / 2);
+        }
+        if (cnt == 0) continue;
+        if (atomic) k_diag<true><<<blocks(cnt, ET), ET, 0, s>>>(d, op->kcell, op->rhoc, g, cfg->dt, mdiag, kdiag, c);
+        else k_diag<false><<<blocks(cnt, ET), ET, 0, s>>>(d, op->kcell, op->rhoc, g, cfg->dt, mdiag, kdiag, c);
+        CHECK_LAUNCH();
+        if (!cfg->verify) {
+            if (atomic) k_source<true><<<blocks(cnt, ET), ET, 0, s>>>(F, g, cfg->source_mode, cfg->source_value, c);
+            else k_source<false><<<blocks(cnt, ET), ET, 0, s>>>(F, g, cfg->source_mode, cfg->source_value, c);
+            CHECK_LAUNCH();
+        }
+    }
+    k_dirichlet<double><<<op->nbn, NT, 0, s>>>(d, g, 1.0);
+    CHECK_LAUNCH();
+    k_dirichlet<double><<<op->nbn, NT, 0, s>>>(F, g, 0.0);
+    CHECK_LAUNCH();
+    // detectors (src/heat_solver_het.cu:940-953), computed with detect still off
+    if (cfg->detect) {
+        for (int v = HEAT_VEC_WM; v <= HEAT_VEC_ONES; ++v) HIP_TRY(hipMalloc(&op->vec[v], nb8));
+        double* ones = op->vec[HEAT_VEC_ONES];
+        k_ones_interior<<<op->nbn, NT, 0, s>>>(ones, g);
+        CHECK_LAUNCH();
+        FiArm none{0, 0, 0};
+        rc = element_apply(op, ones, op->vec[HEAT_VEC_WM], 0.0, none);
+        if (rc) return rc;
+        rc = element_apply(op, ones, op->vec[HEAT_VEC_WA], cfg->dt, none);
+        if (rc) return rc;
+        k_dot<<<op->nbn, NT, 0, s>>>(op->vec[HEAT_VEC_WM], op->vec[HEAT_VEC_WM], g.nn, op->partials, op->st, 0);
+        CHECK_LAUNCH();
+        k_fin_dot<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0);
+        CHECK_LAUNCH();
+        k_dot<<<op->nbn, NT, 0, s>>>(op->vec[HEAT_VEC_WA], op->vec[HEAT_VEC_WA], g.nn, op->partials, op->st, 1);
+        CHECK_LAUNCH();
+        k_fin_dot<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 1);
+        CHECK_LAUNCH();
+        k_set_norms<<<1, 1, 0, s>>>(op->st, 0, 1);
+        CHECK_LAUNCH();
+    }
+    int rc2 = read_state(op);
+    if (rc2) return rc2;
+    op->nwM = op->hst->nwM;
+    op->nwA = op->hst->nwA;
+    return HEAT_OK;
+}
+
+int ensure_mixed(heat_op* op) {
+    if (op->e32) return HEAT_OK;
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    const size_t nb4 = sizeof(float) * (size_t)g.nn, eb4 = sizeof(float) * (size_t)g.ne;
+    HIP_TRY(hipMalloc(&op->cr32, eb4));
+    HIP_TRY(hipMalloc(&op->ck32, eb4));
+    HIP_TRY(hipMalloc(&op->e32, nb4));
+    HIP_TRY(hipMalloc(&op->r32, nb4));
+    HIP_TRY(hipMalloc(&op->z32, nb4));
+    HIP_TRY(hipMalloc(&op->p32, nb4));
+    HIP_TRY(hipMalloc(&op->ap32, nb4));
+    HIP_TRY(hipMalloc(&op->d32, nb4));
+    k_coef32<<<blocks(g.ne, 256), 256, 0, s>>>(op->kcell, op->rhoc, op->cfg.dt, op->m3, op->k2, op->cr32, op->ck32, g.ne);
+    CHECK_LAUNCH();
+    k_convert<float, double><<<blocks(g.nn, 256), 256, 0, s>>>(op->d32, op->vec[HEAT_VEC_DIAG], g.nn);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* heat_last_error(void) { return g_err.c_str(); }
+
+int heat_setup(const heat_config* cfg, const double* k_cell, const double* rhoc_cell, heat_op** out) {
+    if (!cfg || !k_cell || !rhoc_cell || !out) return fail(HEAT_E_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->n < 1 || cfg->n > 1000) return fail(HEAT_E_ARG, "N must be in [1, 1000], got %d", cfg->n);
+    if (!(cfg->dt > 0)) return fail(HEAT_E_ARG, "dt must be > 0");
+    if (cfg->kernel != HEAT_KERNEL_FAST && cfg->kernel != HEAT_KERNEL_GAUSS) return fail(HEAT_E_ARG, "bad kernel");
+    if (cfg->scatter != HEAT_SCATTER_ATOMIC && cfg->scatter != HEAT_SCATTER_COLORED) return fail(HEAT_E_ARG, "bad scatter");
+    heat_op* op = new heat_op();
+    int rc = setup_impl(cfg, k_cell, rhoc_cell, op);
+    if (rc) {
+        std::string keep = g_err;
+        free_all(op);
+        delete op;
+        g_err = keep;
+        return rc;
+    }
+    *out = op;
+    return HEAT_OK;
+}
+
+int heat_cleanup(heat_op* op) {
+    if (!op) return HEAT_OK;
+    if (op->stream) (void)hipStreamSynchronize(op->stream);
+    free_all(op);
+    delete op;
+    return HEAT_OK;
+}
+
+int heat_sizes(const heat_op* op, int64_t* nn, int64_t* ne) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    if (nn) *nn = op->g.nn;
+    if (ne) *ne = op->g.ne;
+    return HEAT_OK;
+}
+
+int heat_field_stats(const heat_op* op, double out4[4]) {
+    if (!op || !out4) return fail(HEAT_E_ARG, "null argument");
+    memcpy(out4, op->stats, sizeof op->stats);
+    return HEAT_OK;
+}
+
+int heat_vector(heat_op* op, int which, double** dev_ptr) {
+    if (!op || !dev_ptr || which < 0 || which > HEAT_VEC_ONES) return fail(HEAT_E_ARG, "bad vector id");
+    if (!op->vec[which]) return fail(HEAT_E_STATE, "vector %d needs detect", which);
+    *dev_ptr = op->vec[which];
+    return HEAT_OK;
+}
+
+int heat_stream(heat_op* op, void** stream) {
+    if (!op || !stream) return fail(HEAT_E_ARG, "null argument");
+    *stream = (void*)op->stream;
+    return HEAT_OK;
+}
+
+int heat_synchronize(heat_op* op) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
+int heat_upload(heat_op* op, double* dev_dst, const double* host_src, int64_t count) {
+    if (!op || !dev_dst || !host_src || count < 0) return fail(HEAT_E_ARG, "bad upload");
+    HIP_TRY(hipMemcpyAsync(dev_dst, host_src, sizeof(double) * count, hipMemcpyHostToDevice, op->stream));
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
+int heat_download(heat_op* op, double* host_dst, const double* dev_src, int64_t count) {
+    if (!op || !dev_src || !host_dst || count < 0) return fail(HEAT_E_ARG, "bad download");
+    HIP_TRY(hipMemcpyAsync(host_dst, dev_src, sizeof(double) * count, hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
+int heat_set_colored_impl(heat_op* op, int impl) {
+    if (!op || (impl != HEAT_COLORED_GATHER && impl != HEAT_COLORED_SWEEP)) return fail(HEAT_E_ARG, "bad impl");
+    op->colored_impl = impl;
+    return HEAT_OK;
+}
+
+int heat_apply_operator(heat_op* op, const double* x, double* y, double dt_eff, int allow_fi, int project) {
+    if (!op || !x || !y) return fail(HEAT_E_ARG, "null argument");
+    const int epi = (op->cfg.detect && project) ? EPI_D1 : EPI_NONE;
+    return apply_impl(op, x, y, dt_eff, allow_fi != 0, project != 0, epi, 0);
+}
+
+int heat_apply_operator_f32(heat_op* op, const float* x, float* y, double dt_eff, int project) {
+    if (!op || !x || !y) return fail(HEAT_E_ARG, "null argument");
+    if (!gather_path(op) || !project)
+        return fail(HEAT_E_STATE, "the FP32 operator is the projected colour-ordered fast operator");
+    const bool mass = (dt_eff == 0.0);
+    if (!mass && dt_eff != op->cfg.dt) return fail(HEAT_E_ARG, "dt_eff must be 0 or the handle's dt");
+    int rc = ensure_mixed(op);
+    if (rc) return rc;
+    FiArm none{0, 0, 0};
+    if (mass)
+        k_apply_gather<float, true, EPI_NONE><<<op->nbn, NT, 0, op->stream>>>(x, y, op->cr32, op->ck32, op->g, none, nullptr, 0.0, op->partials, op->st, 0);
+    else
+        k_apply_gather<float, false, EPI_NONE><<<op->nbn, NT, 0, op->stream>>>(x, y, op->cr32, op->ck32, op->g, none, nullptr, 0.0, op->partials, op->st, 0);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_apply_matvec(heat_op* op, const double* x, double* y) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    return heat_apply_operator(op, x, y, op->cfg.dt, 1, 1);
+}
+
+int heat_apply_preconditioner(heat_op* op, const double* r, double* z) {
+    if (!op || !r || !z) return fail(HEAT_E_ARG, "null argument");
+    k_jacobi<<<op->nbn, NT, 0, op->stream>>>(r, op->vec[HEAT_VEC_DIAG], z, op->g.nn);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_build_rhs(heat_op* op, const double* Tn, double* b) {
+    if (!op || !Tn || !b) return fail(HEAT_E_ARG, "null argument");
+    int rc = heat_apply_operator(op, Tn, b, 0.0, 0, 1);
+    if (rc) return rc;
+    if (!op->cfg.verify && !op->cfg.no_source) {
+        k_axpy<<<op->nbn, NT, 0, op->stream>>>(op->cfg.dt, op->vec[HEAT_VEC_F], b, op->g.nn);
+        CHECK_LAUNCH();
+    }
+    return HEAT_OK;
+}
+
+int heat_dot(heat_op* op, const double* x, const double* y, double* out) {
+    if (!op || !x || !y || !out) return fail(HEAT_E_ARG, "null argument");
+    k_dot<<<op->nbn, NT, 0, op->stream>>>(x, y, op->g.nn, op->partials, op->st, 2);
+    CHECK_LAUNCH();
+    k_fin_dot<<<1, FT, 0, op->stream>>>(op->partials, (int)op->nbn, op->st, 2);
+    CHECK_LAUNCH();
+    int rc = read_state(op);
+    if (rc) return rc;
+    *out = op->hst->res[2];
+    return HEAT_OK;
+}
+
+int heat_norm2(heat_op* op, const double* x, double* out) {
+    double d2 = 0;
+    int rc = heat_dot(op, x, x, &d2);
+    if (rc) return rc;
+    *out = std::sqrt(d2);
+    return HEAT_OK;
+}
+
+int heat_minmax(heat_op* op, const double* x, double* vmin, double* vmax) {
+    if (!op || !x || !vmin || !vmax) return fail(HEAT_E_ARG, "null argument");
+    k_minmax<<<op->nbn, NT, 0, op->stream>>>(x, op->g.nn, op->partials, op->st, 4);
+    CHECK_LAUNCH();
+    k_fin_minmax<<<1, FT, 0, op->stream>>>(op->partials, (int)op->nbn, op->st, 4);
+    CHECK_LAUNCH();
+    int rc = read_state(op);
+    if (rc) return rc;
+    *vmin = op->hst->res[4];
+    *vmax = op->hst->res[5];
+    return HEAT_OK;
+}
+
+int heat_set_dirichlet(heat_op* op, double* v, double value) {
+    if (!op || !v) return fail(HEAT_E_ARG, "null argument");
+    k_dirichlet<double><<<op->nbn, NT, 0, op->stream>>>(v, op->g, value);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_copy(heat_op* op, const double* x, double* y) {
+    if (!op || !x || !y) return fail(HEAT_E_ARG, "null argument");
+    HIP_TRY(hipMemcpyAsync(y, x, sizeof(double) * (size_t)op->g.nn, hipMemcpyDeviceToDevice, op->stream));
+    return HEAT_OK;
+}
+
+int heat_axpy(heat_op* op, double a, const double* x, double* y) {
+    if (!op || !x || !y) return fail(HEAT_E_ARG, "null argument");
+    k_axpy<<<op->nbn, NT, 0, op->stream>>>(a, x, y, op->g.nn);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_initial_condition(heat_op* op, double* v, int mode, double t, double alpha) {
+    if (!op || !v || mode < 0 || mode > 2) return fail(HEAT_E_ARG, "bad initial condition");
+    k_initial<<<op->nbn, NT, 0, op->stream>>>(v, op->g, mode, t, alpha);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double tol, int* iters,
+                   double* hist) {
+    if (!op || !b || !x) return fail(HEAT_E_ARG, "null argument");
+    if (max_iter < 0) return fail(HEAT_E_ARG, "max_iter must be >= 0");
+    hipStream_t s = op->stream;
+    if (hist && op->hist_cap < max_iter + 1) {
+        if (op->dhist) HIP_TRY(hipFree(op->dhist));
+        op->dhist = nullptr;
+        op->hist_cap = 0;
+        HIP_TRY(hipMalloc(&op->dhist, sizeof(double) * 3 * (size_t)(max_iter + 1)));
+        op->hist_cap = max_iter + 1;
+    }
+    op->cur_hist = hist ? op->dhist : nullptr;
+    const bool detect = op->cfg.detect != 0;
+    k_pcg_params<<<1, 1, 0, s>>>(op->st, tol, max_iter);
+    CHECK_LAUNCH();
+    double* Ap = op->vec[HEAT_VEC_AP];
+    // r = b - A x0, with level R firing here if armed (src/heat_solver_het.cu:999)
+    int rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
+    if (rc) return rc;
+    k_pcg_init<<<op->nbn, NT, 0, s>>>(b, Ap, op->vec[HEAT_VEC_DIAG], op->vec[HEAT_VEC_R],
+                                       op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_P], op->g.nn,
+                                       op->partials);
+    CHECK_LAUNCH();
+    k_fin_init<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, op->cur_hist);
+    CHECK_LAUNCH();
+    if (gather_path(op)) {
+        rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); });
+    } else {
+        // element-centric paths: iterate with a host check per iteration
+        rc = HEAT_OK;
+        for (int it = 0; it <= max_iter && rc == HEAT_OK; ++it) {
+            rc = read_state(op);
+            if (rc || op->hst->done) break;
+            rc = apply_impl(op, op->vec[HEAT_VEC_P], Ap, op->cfg.dt, false, true, detect ? EPI_PCG_D1 : EPI_PCG, 0);
+            if (rc) break;
+            k_pcg_update<<<op->nbn, NT, 0, s>>>(x, op->vec[HEAT_VEC_P], op->vec[HEAT_VEC_R], Ap,
+                                                 op->vec[HEAT_VEC_DIAG], op->vec[HEAT_VEC_Z], op->g.nn,
+                                                 op->partials, op->st);
+            CHECK_LAUNCH();
+            k_fin_update<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, op->cur_hist);
+            CHECK_LAUNCH();
+            k_pcg_dir<<<op->nbn, NT, 0, s>>>(op->vec[HEAT_VEC_P], op->vec[HEAT_VEC_Z], op->g.nn, op->st);
+            CHECK_LAUNCH();
+        }
+    }
+    if (rc) return rc;
+    rc = read_state(op);
+    if (rc) return rc;
+    if (iters) *iters = op->hst->iter;
+    if (hist) {
+        const int rows = op->hst->iter + 1;
+        for (int i = 0; i < 3 * (max_iter + 1); ++i) hist[i] = std::nan("");
+        HIP_TRY(hipMemcpy(hist, op->dhist, sizeof(double) * 3 * rows, hipMemcpyDeviceToHost));
+    }
+    return HEAT_OK;
+}
+
+int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer, int max_inner,
+                         double tol, double tol_inner, int* outer, int* inner_iters) {
+    if (!op || !b || !x) return fail(HEAT_E_ARG, "null argument");
+    if (!gather_path(op)) return fail(HEAT_E_STATE, "the FP32-refined solve needs --kernel fast --scatter colored");
+    if (max_outer < 0 || max_inner < 0) return fail(HEAT_E_ARG, "iteration limits must be >= 0");
+    int rc = ensure_mixed(op);
+    if (rc) return rc;
+    hipStream_t s = op->stream;
+    const Grid& g = op->g;
+    k_mixed_params<<<1, 1, 0, s>>>(op->st, tol, tol_inner, max_inner);
+    CHECK_LAUNCH();
+    FiArm none{0, 0, 0};
+    int nout = 0;
+    for (;;) {
+        k_mixed_residual<<<op->nbn, NT, 0, s>>>(x, b, op->cr, op->ck, g, op->e32, op->r32, op->z32,
+                                                 op->p32, op->d32, op->partials, op->st);
+        CHECK_LAUNCH();
+        k_fin_mixed_resid<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
+        CHECK_LAUNCH();
+        auto launch = [&]() -> int {
+            for (int i = 0; i < op->pcg_chunk; ++i) {
+                k_apply_gather<float, false, EPI_PCG><<<op->nbn, NT, 0, s>>>(
+                    op->p32, op->ap32, op->cr32, op->ck32, g, none, nullptr, 0.0, op->partials, op->st, 1);
+                CHECK_LAUNCH();
+                k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1);
+                CHECK_LAUNCH();
+                k_mixed_update<<<op->nbn, NT, 0, s>>>(op->e32, op->p32, op->r32, op->ap32, op->d32,
+                                                       op->z32, g.nn, op->partials, op->st);
+                CHECK_LAUNCH();
+                k_fin_mixed_update<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
+                CHECK_LAUNCH();
+                k_mixed_dir<<<op->nbn, NT, 0, s>>>(op->p32, op->z32, g.nn, op->st);
+                CHECK_LAUNCH();
+            }
+            return HEAT_OK;
+        };
+        rc = run_until_done(op, max_inner, launch);
+        if (rc) return rc;
+        rc = read_state(op);
+        if (rc) return rc;
+        if (op->hst->outer_done || nout >= max_outer) break;
+        k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
+        CHECK_LAUNCH();
+        ++nout;
+    }
+    if (outer) *outer = nout;
+    if (inner_iters) *inner_iters = op->hst->inner_total;
+    return HEAT_OK;
+}
+
+int heat_arm_fault(heat_op* op, int level, int64_t target, int bit) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    if (bit < 0 || bit > 63) return fail(HEAT_E_ARG, "BIT must be in 0..63");
+    op->fi_level = level;
+    op->fi.armed = 1;
+    op->fi.bit = bit;
+    op->fi.target = target;
+    return HEAT_OK;
+}
+
+int heat_disarm_fault(heat_op* op) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    op->fi.armed = 0;
+    op->fi_level = 0;
+    return HEAT_OK;
+}
+
+int heat_flip_bit(heat_op* op, double* v, int64_t idx, int bit) {
+    if (!op || !v) return fail(HEAT_E_ARG, "null argument");
+    if (bit < 0 || bit > 63) return fail(HEAT_E_ARG, "BIT must be in 0..63");
+    if (idx < 0 || idx >= op->g.nn) return fail(HEAT_E_ARG, "index %lld outside [0, %d)", (long long)idx, op->g.nn);
+    k_flip<<<1, 1, 0, op->stream>>>(v, idx, bit);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_abft_take(heat_op* op, double* step_max, int64_t* calls) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    int rc = read_state(op);
+    if (rc) return rc;
+    if (step_max) *step_max = op->hst->abft_max;
+    if (calls) *calls = op->hst->abft_calls;
+    k_abft_reset<<<1, 1, 0, op->stream>>>(op->st);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_abft_threshold(heat_op* op, double* chk_tol) {
+    if (!op || !chk_tol) return fail(HEAT_E_ARG, "null argument");
+    *chk_tol = op->chk_tol;
+    return HEAT_OK;
+}
+
+int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t flush_bytes,
+                     double* avg_ms, double* min_ms) {
+    if (!op || !x || !y || reps < 1 || flush_bytes < 0) return fail(HEAT_E_ARG, "bad timing request");
+    hipStream_t s = op->stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    FiArm none = op->fi;
+    const int saved_level = op->fi_level;
+    op->fi_level = 0;   // never fire a pending fault from a timing loop
+    (void)none;
+    double total = 0.0, best = 1e300;
+    int rc = HEAT_OK;
+    unsigned* flush = nullptr;
+    const long long nflush = flush_bytes / 4;
+    if (nflush > 0 && hipMalloc(&flush, sizeof(unsigned) * nflush) != hipSuccess) {
+        op->fi_level = saved_level;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(HEAT_E_HIP, "cannot allocate %lld flush bytes", (long long)flush_bytes);
+    }
+    if (nflush > 0) {
+        for (int i = 0; i < reps && rc == HEAT_OK; ++i) {
+            k_fill<unsigned><<<blocks(nflush, 256), 256, 0, s>>>(flush, (int)std::min<long long>(nflush, INT32_MAX), (unsigned)i);
+            (void)hipEventRecord(e0, s);
+            rc = apply_impl(op, x, y, op->cfg.dt, false, true, EPI_NONE, 0);
+            (void)hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) rc = fail(HEAT_E_HIP, "event sync failed");
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            total += ms;
+            best = std::min(best, (double)ms);
+        }
+    } else {
+        (void)hipEventRecord(e0, s);
+        for (int i = 0; i < reps && rc == HEAT_OK; ++i) rc = apply_impl(op, x, y, op->cfg.dt, false, true, EPI_NONE, 0);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess && rc == HEAT_OK) rc = fail(HEAT_E_HIP, "event sync failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        total = ms;
+        best = ms / reps;
+    }
+    op->fi_level = saved_level;
+    if (flush) (void)hipFree(flush);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (avg_ms) *avg_ms = total / reps;
+    if (min_ms) *min_ms = best;
+    return HEAT_OK;
+}
+
+int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph) {
+    if (!op || pcg_chunk < 1 || pcg_chunk > 256) return fail(HEAT_E_ARG, "pcg_chunk must be in [1, 256]");
+    op->pcg_chunk = pcg_chunk;
+    op->use_graph = use_graph ? 1 : 0;
+    return HEAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,276 @@
+"""Python binding of the MI355X heat solver's C ABI (include/heat.h).
+
+``HeatOperator`` mirrors the reference's operator class
+(src/heat_solver_het.cu:618-989: setup, apply_operator, apply_matvec,
+apply_preconditioner, build_rhs, dot, norm2, minmax, setup_detectors,
+cleanup) and ``pcg_solve`` its solver (src/heat_solver_het.cu:994-1039).
+Vector arguments are device pointers: an int, an object with ``data_ptr()``
+(a torch CUDA tensor) or the name of a handle-owned vector ("T", "b", "r",
+"z", "p", "Ap", "diag", "F", "wM", "wA", "ones").
+
+The library is loaded from mnhack8_amd/_build/libheat_hip.so; there is no CPU
+fallback: a missing library raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libheat_hip.so")
+DRIVER_PATH = os.path.join(_HERE, "_build", "heat_solver_het")
+
+KERNEL = {"gauss": 0, "fast": 1}
+SCATTER = {"atomic": 0, "colored": 1}
+SOURCE = {"gaussian": 0, "const": 1}
+COLORED_IMPL = {"gather": 0, "sweep": 1}
+VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
+ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "bad field", 4: "self-check failed", 5: "bad state"}
+
+
+class HeatError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{ERRORS.get(code, code)}] {msg}")
+        self.code = code
+
+
+class HeatConfig(C.Structure):
+    _fields_ = [("n", C.c_int), ("dt", C.c_double), ("kernel", C.c_int), ("scatter", C.c_int),
+                ("verify", C.c_int), ("no_source", C.c_int), ("source_mode", C.c_int),
+                ("source_value", C.c_double), ("detect", C.c_int)]
+
+
+_P = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_I64 = C.c_int64
+SIGNATURES = {
+    "heat_last_error": (C.c_char_p, []),
+    "heat_setup": (_I, [C.POINTER(HeatConfig), _P, _P, C.POINTER(_P)]),
+    "heat_cleanup": (_I, [_P]),
+    "heat_sizes": (_I, [_P, C.POINTER(_I64), C.POINTER(_I64)]),
+    "heat_field_stats": (_I, [_P, C.POINTER(_D)]),
+    "heat_vector": (_I, [_P, _I, C.POINTER(_P)]),
+    "heat_stream": (_I, [_P, C.POINTER(_P)]),
+    "heat_synchronize": (_I, [_P]),
+    "heat_upload": (_I, [_P, _P, _P, _I64]),
+    "heat_download": (_I, [_P, _P, _P, _I64]),
+    "heat_set_colored_impl": (_I, [_P, _I]),
+    "heat_apply_operator": (_I, [_P, _P, _P, _D, _I, _I]),
+    "heat_apply_operator_f32": (_I, [_P, _P, _P, _D, _I]),
+    "heat_apply_matvec": (_I, [_P, _P, _P]),
+    "heat_apply_preconditioner": (_I, [_P, _P, _P]),
+    "heat_build_rhs": (_I, [_P, _P, _P]),
+    "heat_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
+    "heat_norm2": (_I, [_P, _P, C.POINTER(_D)]),
+    "heat_minmax": (_I, [_P, _P, C.POINTER(_D), C.POINTER(_D)]),
+    "heat_set_dirichlet": (_I, [_P, _P, _D]),
+    "heat_copy": (_I, [_P, _P, _P]),
+    "heat_axpy": (_I, [_P, _D, _P, _P]),
+    "heat_initial_condition": (_I, [_P, _P, _I, _D, _D]),
+    "heat_pcg_solve": (_I, [_P, _P, _P, _I, _D, C.POINTER(_I), _P]),
+    "heat_pcg_solve_mixed": (_I, [_P, _P, _P, _I, _I, _D, _D, C.POINTER(_I), C.POINTER(_I)]),
+    "heat_arm_fault": (_I, [_P, _I, _I64, _I]),
+    "heat_disarm_fault": (_I, [_P]),
+    "heat_flip_bit": (_I, [_P, _P, _I64, _I]),
+    "heat_abft_take": (_I, [_P, C.POINTER(_D), C.POINTER(_I64)]),
+    "heat_abft_threshold": (_I, [_P, C.POINTER(_D)]),
+    "heat_set_tuning": (_I, [_P, _I, _I]),
+    "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libheat_hip.so and declare every C-ABI signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"HIP library not built: {path} (run python -m mnhack8_amd.build)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise HeatError(rc, load_library().heat_last_error().decode())
+
+
+class HeatOperator:
+    """One mesh on the current HIP device (HeatOperator::setup)."""
+
+    def __init__(self, n, dt, k, rhoc, kernel="fast", scatter="colored", verify=False,
+                 no_source=False, source="gaussian", source_value=0.0, detect=False):
+        lib = load_library()
+        self._lib = lib
+        k = np.ascontiguousarray(k, dtype=np.float64).ravel()
+        rhoc = np.ascontiguousarray(rhoc, dtype=np.float64).ravel()
+        if k.size != n ** 3 or rhoc.size != n ** 3:
+            raise ValueError(f"expected {n ** 3} element values")
+        cfg = HeatConfig(n, float(dt), KERNEL[kernel], SCATTER[scatter], int(verify), int(no_source),
+                         SOURCE[source], float(source_value), int(detect))
+        h = C.c_void_p()
+        _check(lib.heat_setup(C.byref(cfg), k.ctypes.data, rhoc.ctypes.data, C.byref(h)))
+        self.h = h
+        self.n, self.dt, self.detect = n, float(dt), bool(detect)
+        nn, ne = _I64(), _I64()
+        _check(lib.heat_sizes(h, C.byref(nn), C.byref(ne)))
+        self.nn, self.ne = nn.value, ne.value
+
+    # -- lifetime ----------------------------------------------------------
+    def cleanup(self):
+        if getattr(self, "h", None):
+            _check(self._lib.heat_cleanup(self.h))
+            self.h = None
+
+    close = cleanup
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.cleanup()
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+    # -- vectors -----------------------------------------------------------
+    def ptr(self, v):
+        if isinstance(v, str):
+            p = C.c_void_p()
+            _check(self._lib.heat_vector(self.h, VEC[v], C.byref(p)))
+            return p.value
+        if hasattr(v, "data_ptr"):
+            import torch
+            torch.cuda.synchronize()
+            return v.data_ptr()
+        return int(v)
+
+    def upload(self, dst, host):
+        a = np.ascontiguousarray(host, dtype=np.float64).ravel()
+        _check(self._lib.heat_upload(self.h, self.ptr(dst), a.ctypes.data, a.size))
+
+    def download(self, src, count=None):
+        count = self.nn if count is None else count
+        out = np.empty(count, dtype=np.float64)
+        _check(self._lib.heat_download(self.h, out.ctypes.data, self.ptr(src), count))
+        return out
+
+    def synchronize(self):
+        _check(self._lib.heat_synchronize(self.h))
+
+    def field_stats(self):
+        out = (C.c_double * 4)()
+        _check(self._lib.heat_field_stats(self.h, out))
+        return tuple(out)
+
+    def set_colored_impl(self, impl):
+        _check(self._lib.heat_set_colored_impl(self.h, COLORED_IMPL[impl]))
+
+    def set_tuning(self, pcg_chunk=8, use_graph=True):
+        _check(self._lib.heat_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))
+
+    # -- HeatOperator methods ----------------------------------------------
+    def apply_operator(self, x, y, dt_eff, allow_fi=False, project=True):
+        _check(self._lib.heat_apply_operator(self.h, self.ptr(x), self.ptr(y), float(dt_eff),
+                                             int(allow_fi), int(project)))
+
+    def apply_operator_f32(self, x, y, dt_eff, project=True):
+        _check(self._lib.heat_apply_operator_f32(self.h, self.ptr(x), self.ptr(y), float(dt_eff), int(project)))
+
+    def apply_matvec(self, x, y):
+        _check(self._lib.heat_apply_matvec(self.h, self.ptr(x), self.ptr(y)))
+
+    def apply_preconditioner(self, r, z):
+        _check(self._lib.heat_apply_preconditioner(self.h, self.ptr(r), self.ptr(z)))
+
+    def build_rhs(self, Tn, b):
+        _check(self._lib.heat_build_rhs(self.h, self.ptr(Tn), self.ptr(b)))
+
+    def dot(self, x, y):
+        out = C.c_double()
+        _check(self._lib.heat_dot(self.h, self.ptr(x), self.ptr(y), C.byref(out)))
+        return out.value
+
+    def norm2(self, x):
+        out = C.c_double()
+        _check(self._lib.heat_norm2(self.h, self.ptr(x), C.byref(out)))
+        return out.value
+
+    def minmax(self, x):
+        a, b = C.c_double(), C.c_double()
+        _check(self._lib.heat_minmax(self.h, self.ptr(x), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_dirichlet(self, v, value):
+        _check(self._lib.heat_set_dirichlet(self.h, self.ptr(v), float(value)))
+
+    def copy(self, x, y):
+        _check(self._lib.heat_copy(self.h, self.ptr(x), self.ptr(y)))
+
+    def axpy(self, a, x, y):
+        _check(self._lib.heat_axpy(self.h, float(a), self.ptr(x), self.ptr(y)))
+
+    def initial_condition(self, v, mode, t=0.0, alpha=1.0):
+        modes = {"zero": 0, "gaussian": 1, "sine": 2}
+        _check(self._lib.heat_initial_condition(self.h, self.ptr(v), modes.get(mode, mode), float(t), float(alpha)))
+
+    # -- fault injection and detectors --------------------------------------
+    def arm_fault(self, level, target, bit):
+        _check(self._lib.heat_arm_fault(self.h, int(level), int(target), int(bit)))
+
+    def disarm_fault(self):
+        _check(self._lib.heat_disarm_fault(self.h))
+
+    def flip_bit(self, v, idx, bit):
+        _check(self._lib.heat_flip_bit(self.h, self.ptr(v), int(idx), int(bit)))
+
+    def abft_take(self):
+        m, c = C.c_double(), _I64()
+        _check(self._lib.heat_abft_take(self.h, C.byref(m), C.byref(c)))
+        return m.value, c.value
+
+    def stream(self):
+        p = C.c_void_p()
+        _check(self._lib.heat_stream(self.h, C.byref(p)))
+        return p.value
+
+    def time_matvec(self, x, y, reps, flush_bytes=0):
+        """Mean and minimum milliseconds per apply_matvec, timed on the device."""
+        a, m = C.c_double(), C.c_double()
+        _check(self._lib.heat_time_matvec(self.h, self.ptr(x), self.ptr(y), int(reps), int(flush_bytes),
+                                          C.byref(a), C.byref(m)))
+        return a.value, m.value
+
+    # -- solvers --------------------------------------------------------------
+    def pcg_solve(self, b, x, max_iter=500, tol=1e-8, history=False):
+        it = C.c_int()
+        hist = np.empty(3 * (max_iter + 1)) if history else None
+        _check(self._lib.heat_pcg_solve(self.h, self.ptr(b), self.ptr(x), int(max_iter), float(tol),
+                                        C.byref(it), hist.ctypes.data if history else None))
+        if history:
+            return it.value, hist.reshape(-1, 3)[: it.value + 1]
+        return it.value
+
+    def pcg_solve_mixed(self, b, x, max_outer=50, max_inner=500, tol=1e-8, tol_inner=1e-4):
+        outer, inner = C.c_int(), C.c_int()
+        _check(self._lib.heat_pcg_solve_mixed(self.h, self.ptr(b), self.ptr(x), int(max_outer), int(max_inner),
+                                              float(tol), float(tol_inner), C.byref(outer), C.byref(inner)))
+        return outer.value, inner.value
+
+
+def pcg_solve(op: HeatOperator, b, x, max_iter=500, tol=1e-8, history=False):
+    """pcg_solve(op, b, x, max_iter, tol) of src/heat_solver_het.cu:994."""
+    return op.pcg_solve(b, x, max_iter, tol, history)

@@ -1,0 +1,307 @@
+"""Parity of the HIP library (through its C ABI) against the CPU oracle.
+
+Bars: integer results (iteration counts, fault sites) exact; the colour-ordered
+FP64 operator, diagonal, inner products and PCG iterates bit-identical to the
+oracle's closed-form (STRUCT) restatement; the atomic scatter within 1e-14 of
+it (order-dependent rounding only); FP64 fields within 1e-10 relative of the
+oracle; FP32 operator bit-identical to the oracle's FP32 restatement.
+"""
+import numpy as np
+import pytest
+
+from mnhack8_amd import fields
+from oracle_lib import OracleOp, flip_bit
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def heat():
+    from mnhack8_amd import heat as h
+    return h
+
+
+def relerr(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def make_field(n, seed=3, sigma=1.0):
+    k, rhoc = fields.iid_lognormal(n, sigma=sigma, seed=seed)
+    # a heterogeneous rho*c as well, so that the mass part is exercised
+    rhoc = np.exp(0.3 * fields.counter_normal(seed + 100, np.arange(n ** 3, dtype=np.uint64))).reshape(n, n, n)
+    return k.ravel(), rhoc.ravel()
+
+
+def dev(arr):
+    t = torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+    torch.cuda.synchronize()
+    return t
+
+
+def host(op, t):
+    op.synchronize()
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [3, 4, 7, 16])
+@pytest.mark.parametrize("dt_eff", [0.0, 1e-3])
+def test_apply_colored_bitwise(n, dt_eff):
+    print(f"[case] apply colored n={n} dt_eff={dt_eff}", flush=True)
+    k, rhoc = make_field(n)
+    dt = 1e-3
+    op = heat().HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="colored")
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="colored")
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(op.nn)
+    xd, yd = dev(x), dev(np.zeros(op.nn))
+    op.apply_operator(xd, yd, dt_eff, project=True)
+    y_gather = host(op, yd)
+    op.set_colored_impl("sweep")
+    yd2 = dev(np.zeros(op.nn))
+    op.apply_operator(xd, yd2, dt_eff, project=True)
+    y_sweep = host(op, yd2)
+    y_orc = orc.apply(x, dt_eff)
+    assert np.array_equal(y_gather, y_sweep), "gather != colour sweep"
+    assert np.array_equal(y_gather, y_orc), f"gather != oracle, max rel {relerr(y_gather, y_orc):.3e}"
+    # the closed-form rows agree with the reference's quadrature matrices
+    y_ref = OracleOp(n, dt, k, rhoc, kernel="fast", scatter="colored").apply(x, dt_eff)
+    assert relerr(y_gather, y_ref) < 1e-14
+    op.cleanup()
+
+
+@pytest.mark.parametrize("n", [4, 9])
+def test_apply_unprojected_and_atomic(n):
+    print(f"[case] atomic/unprojected n={n}", flush=True)
+    k, rhoc = make_field(n, seed=5)
+    dt = 2e-3
+    x = np.random.default_rng(1).standard_normal((n + 1) ** 3)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="colored")
+    opc = heat().HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="colored")
+    xd, yd = dev(x), dev(np.zeros_like(x))
+    opc.apply_operator(xd, yd, dt, project=False)
+    assert np.array_equal(host(opc, yd), orc.apply(x, dt, project=False))
+    opa = heat().HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="atomic")
+    orca = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="atomic")
+    for proj in (True, False):
+        opa.apply_operator(xd, yd, dt, project=proj)
+        assert relerr(host(opa, yd), orca.apply(x, dt, project=proj)) < 1e-14
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_apply_gauss(scatter):
+    n = 6
+    print(f"[case] gauss {scatter}", flush=True)
+    k, rhoc = make_field(n, seed=9)
+    dt = 1e-3
+    x = np.random.default_rng(2).standard_normal((n + 1) ** 3)
+    op = heat().HeatOperator(n, dt, k, rhoc, kernel="gauss", scatter=scatter)
+    orc = OracleOp(n, dt, k, rhoc, kernel="gauss", scatter=scatter)
+    xd, yd = dev(x), dev(np.zeros_like(x))
+    op.apply_operator(xd, yd, dt, project=True)
+    y = host(op, yd)
+    yo = orc.apply(x, dt)
+    if scatter == "colored":
+        assert np.array_equal(y, yo)
+    assert relerr(y, yo) < 1e-14
+    assert relerr(op.download("diag"), orc.vector("diag")) < 1e-15
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_setup_vectors(scatter):
+    n = 8
+    print(f"[case] setup vectors {scatter}", flush=True)
+    k, rhoc = make_field(n, seed=11)
+    dt = 5e-4
+    op = heat().HeatOperator(n, dt, k, rhoc, scatter=scatter, detect=True)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter=scatter, detect=True)
+    d, do = op.download("diag"), orc.vector("diag")
+    if scatter == "colored":
+        assert np.array_equal(d, do)
+    assert relerr(d, do) < 1e-15
+    assert relerr(op.download("F"), orc.vector("F")) < 1e-13      # exp() of two libms
+    assert relerr(op.download("wM"), orc.vector("wM")) < 1e-14
+    assert relerr(op.download("wA"), orc.vector("wA")) < 1e-14
+    assert np.array_equal(op.download("ones"), orc.vector("ones"))
+    opc = heat().HeatOperator(n, dt, k, rhoc, scatter=scatter, source="const", source_value=1.0)
+    orcc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter=scatter, source="const", source_value=1.0)
+    assert relerr(opc.download("F"), orcc.vector("F")) < 1e-15
+
+
+@pytest.mark.parametrize("nn_n", [2, 16, 40])
+def test_dot_and_minmax_bitwise(nn_n):
+    print(f"[case] dot n={nn_n}", flush=True)
+    n = nn_n
+    k, rhoc = make_field(n)
+    op = heat().HeatOperator(n, 1e-3, k, rhoc)
+    orc = OracleOp(n, 1e-3, k, rhoc, kernel="struct")
+    rng = np.random.default_rng(7)
+    x, y = rng.standard_normal(op.nn), rng.standard_normal(op.nn)
+    xd, yd = dev(x), dev(y)
+    assert op.dot(xd, yd) == orc.dot(x, y)
+    assert op.norm2(xd) == np.sqrt(orc.dot(x, x))
+    assert op.minmax(xd) == (x.min(), x.max())
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("chunk", [1, 8])
+def test_pcg_bitwise_with_oracle(graph, chunk):
+    n, dt = 16, 5e-4
+    print(f"[case] pcg graph={graph} chunk={chunk}", flush=True)
+    k, rhoc = fields.lognormal(n, sigma=1.0, L=0.125, seed=7)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    op = heat().HeatOperator(n, dt, k, rhoc)
+    op.set_tuning(chunk, graph)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct")
+    T0 = np.zeros(op.nn)
+    b_or = orc.build_rhs(T0)
+    op.upload("T", T0)
+    op.build_rhs("T", "b")
+    assert np.array_equal(op.download("b"), b_or) or relerr(op.download("b"), b_or) < 1e-14
+    op.upload("b", b_or)
+    it, hist = op.pcg_solve("b", "T", 500, 1e-10, history=True)
+    ito, xo, histo = orc.pcg(b_or, T0, 500, 1e-10)
+    assert it == ito
+    x = op.download("T")
+    assert np.array_equal(x, xo), f"x differs: {relerr(x, xo):.3e}"
+    np.testing.assert_array_equal(hist, histo)
+    # replaying the captured chunk again from the same start gives the same bits
+    for _ in range(3):
+        op.upload("T", T0)
+        it2 = op.pcg_solve("b", "T", 500, 1e-10)
+        assert it2 == ito and np.array_equal(op.download("T"), xo)
+
+
+def test_pcg_edge_cases():
+    n, dt = 5, 1e-3
+    print("[case] pcg edges", flush=True)
+    k, rhoc = make_field(n)
+    op = heat().HeatOperator(n, dt, k, rhoc)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct")
+    zero = np.zeros(op.nn)
+    # b = 0: bnorm clamps to 1, r = 0 -> p.Ap = 0 -> break after 0 iterations
+    op.upload("b", zero)
+    op.upload("T", zero)
+    assert op.pcg_solve("b", "T", 500, 1e-8) == orc.pcg(zero, zero, 500, 1e-8)[0] == 0
+    # max_iter reached
+    b = orc.build_rhs(np.random.default_rng(0).standard_normal(op.nn))
+    op.upload("b", b)
+    op.upload("T", zero)
+    assert op.pcg_solve("b", "T", 3, 1e-30) == orc.pcg(b, zero, 3, 1e-30)[0] == 3
+    assert np.array_equal(op.download("T"), orc.pcg(b, zero, 3, 1e-30)[1])
+    # max_iter = 0
+    op.upload("T", zero)
+    assert op.pcg_solve("b", "T", 0, 1e-8) == 0
+
+
+@pytest.mark.parametrize("bit", [30, 55, 62, 63])
+def test_fault_level_R_and_G(bit):
+    n, dt = 8, 5e-4
+    print(f"[case] faults bit={bit}", flush=True)
+    k, rhoc = make_field(n, seed=21)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=True)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True)
+    T0 = orc.initial_condition("gaussian")
+    target = (n // 2) * n * n + (n // 2) * n + n // 2
+    b = orc.build_rhs(T0)
+    orc.abft_take()
+    # level R: first matvec of the solve
+    orc.arm_fault(2, target, bit)
+    ito, xo, _ = orc.pcg(b, T0, 500, 1e-8)
+    d1o, _ = orc.abft_take()
+    op.upload("T", T0)
+    op.upload("b", b)
+    op.abft_take()
+    op.arm_fault(2, target, bit)
+    it = op.pcg_solve("b", "T", 500, 1e-8)
+    d1, calls = op.abft_take()
+    x = op.download("T")
+    assert it == ito
+    assert calls == it + 1
+    fin = np.isfinite(xo)
+    assert np.array_equal(np.isfinite(x), fin)
+    assert np.array_equal(x[fin], xo[fin])
+    assert (d1 > 1e-10) == (d1o > 1e-10)
+    # level G: XOR of T[target % nn]
+    node = target % op.nn
+    op.upload("T", T0)
+    op.flip_bit("T", node, bit)
+    assert np.array_equal(op.download("T"), flip_bit(T0, node, bit), equal_nan=True)
+
+
+def test_abft_clean_floor():
+    n, dt = 12, 5e-4
+    print("[case] abft clean", flush=True)
+    k, rhoc = fields.lognormal(n, sigma=1.0, seed=7)
+    op = heat().HeatOperator(n, dt, k.ravel(), rhoc.ravel(), detect=True)
+    T0 = np.zeros(op.nn)
+    op.upload("T", T0)
+    op.abft_take()
+    op.build_rhs("T", "b")
+    it = op.pcg_solve("b", "T", 500, 1e-8)
+    d1, calls = op.abft_take()
+    assert calls == it + 2          # build_rhs + initial residual + one per iteration
+    assert d1 < 1e-14
+
+
+@pytest.mark.parametrize("dt_eff", [0.0, 1e-3])
+def test_fp32_operator_bitwise(dt_eff):
+    n = 10
+    print(f"[case] fp32 apply {dt_eff}", flush=True)
+    k, rhoc = make_field(n, seed=4)
+    op = heat().HeatOperator(n, 1e-3, k, rhoc)
+    orc = OracleOp(n, 1e-3, k, rhoc, kernel="struct")
+    x = np.random.default_rng(3).standard_normal(op.nn).astype(np.float32)
+    xd = dev(x)
+    yd = dev(np.zeros(op.nn, dtype=np.float32))
+    op.apply_operator_f32(xd, yd, dt_eff)
+    y = host(op, yd)
+    yo = orc.apply_f32(x, dt_eff)
+    assert np.array_equal(y, yo)
+    y64 = orc.apply(x.astype(np.float64), dt_eff)
+    assert relerr(y, y64) < 1e-5
+
+
+def test_mixed_precision_solve():
+    n, dt = 24, 1e-4
+    print("[case] mixed solve", flush=True)
+    k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=1)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+    T0 = np.zeros(op.nn)
+    b = orc.build_rhs(T0)
+    _, xo, _ = orc.pcg(b, T0, 2000, 1e-13)
+    op.upload("b", b)
+    op.upload("T", T0)
+    outer, inner = op.pcg_solve_mixed("b", "T", 50, 500, 1e-10, 1e-4)
+    x = op.download("T")
+    assert outer >= 1 and inner > 0
+    # residual-tolerance 1e-10 -> error within a small multiple of it
+    assert relerr(x, xo) < 1e-8
+    # the FP64 path to the same tolerance for comparison
+    op.upload("T", T0)
+    op.pcg_solve("b", "T", 2000, 1e-10)
+    assert relerr(op.download("T"), xo) < 1e-8
+
+
+def test_errors_are_returned_not_raised_by_abort():
+    n = 4
+    k, rhoc = make_field(n)
+    h = heat()
+    bad = k.copy()
+    bad[5] = -1.0
+    with pytest.raises(h.HeatError) as e:
+        h.HeatOperator(n, 1e-3, bad, rhoc)
+    assert "non-positive" in str(e.value)
+    op = h.HeatOperator(n, 1e-3, k, rhoc)
+    with pytest.raises(h.HeatError):
+        op.apply_operator("T", "b", 0.5)          # dt_eff must be 0 or dt
+    with pytest.raises(h.HeatError):
+        op.flip_bit("T", op.nn, 3)               # index out of range
+    with pytest.raises(h.HeatError):
+        op.arm_fault(2, 0, 64)
