@@ -107,6 +107,16 @@ def main():
     for _ in range(a.probe_copy):
         op.copy("p", "r")
     op.synchronize()
+    schedules = {}
+    for impl, planes in [("gather", 8), ("march", 2), ("march", 4), ("march", 8), ("march", 16), ("march", 32)]:
+        op.set_colored_impl(impl)
+        op.set_march_chunk(planes)
+        op.time_matvec("p", "Ap", 3)
+        ms, _ = op.time_matvec("p", "Ap", 50)
+        schedules[f"{impl}{planes if impl == 'march' else ''}"] = round(ms * 1e3, 2)
+    log(f"schedules (us/apply): {schedules}")
+    pick = op.autotune(50)
+    log(f"autotuner picked {pick[0]} planes={pick[1]} ({pick[2] * 1e3:.2f} us)")
     log(f"warmup {a.warmup}")
     op.time_matvec("p", "Ap", max(1, a.warmup))
     torch.cuda.synchronize()
@@ -146,6 +156,7 @@ def main():
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
                        "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
         "algorithmic_bytes_per_apply": alg_bytes,
+        "schedule": {"picked": f"{pick[0]}{pick[1] if pick[0] == 'march' else ''}", "us_per_apply": schedules},
     }
 
     if not a.no_solve:

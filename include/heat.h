@@ -43,9 +43,10 @@ extern "C" {
 /* source models */
 #define HEAT_SOURCE_GAUSSIAN 0  /* 1000 exp(-|x-c|^2 / (2*0.1^2)) (source_at) */
 #define HEAT_SOURCE_CONST    1  /* Q = source_value everywhere */
-/* colored-scatter implementations (both give the same bits) */
-#define HEAT_COLORED_GATHER 0   /* one node-centric launch, contributions summed in colour order */
+/* colored-scatter implementations (all give the same bits) */
+#define HEAT_COLORED_GATHER 0   /* one thread per node, contributions summed in colour order */
 #define HEAT_COLORED_SWEEP  1   /* eight element launches, one per colour */
+#define HEAT_COLORED_MARCH  2   /* threads walk x-columns of nodes with a register window (default) */
 
 /* handle-owned vectors for heat_vector() */
 #define HEAT_VEC_T    0
@@ -161,6 +162,8 @@ int heat_abft_take(heat_op* op, double* step_max, int64_t* calls);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);
+/* x-planes walked per thread block by HEAT_COLORED_MARCH (2, 4, 8, 16 or 32). */
+int heat_set_march_chunk(heat_op* op, int planes);
 /* Benchmark helper (new): time `reps` back-to-back apply_matvec launches
  * (y = P A x, dt_eff = dt, no fault, no D1) with HIP events on the handle's
  * stream.  With flush_bytes > 0 a buffer of that size is overwritten before
@@ -168,6 +171,11 @@ int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);
  * timed alone.  *avg_ms and *min_ms receive the mean and minimum per apply. */
 int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t flush_bytes,
                      double* avg_ms, double* min_ms);
+/* Autotuner (new): time every colour-ordered apply schedule (one thread per
+ * node, and plane marching with 2..32 planes per block) on this handle's
+ * mesh with HIP events, keep the fastest for all later applies, and report it
+ * (*impl = HEAT_COLORED_*, *planes, *best_ms per apply). */
+int heat_autotune(heat_op* op, int reps, int* impl, int* planes, double* best_ms);
 /* The D1 threshold (chk_tol, src/heat_solver_het.cu:637) used by the driver. */
 int heat_abft_threshold(heat_op* op, double* chk_tol);
 

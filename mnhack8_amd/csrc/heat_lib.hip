@@ -53,7 +53,8 @@ struct heat_op {
     heat_config cfg;
     Grid g;
     hipStream_t stream = nullptr;
-    int colored_impl = HEAT_COLORED_GATHER;
+    int colored_impl = HEAT_COLORED_MARCH;
+    int march_cx = 8;
     int pcg_chunk = 8;
     int use_graph = 1;
     double m3 = 0, k2 = 0;
@@ -145,9 +146,46 @@ int build_reference_matrices(heat_op* op) {
     return HEAT_OK;
 }
 
+// the projected colour-ordered fast apply runs in one node-centric launch
 bool gather_path(const heat_op* op) {
     return op->cfg.kernel == HEAT_KERNEL_FAST && op->cfg.scatter == HEAT_SCATTER_COLORED &&
-           op->colored_impl == HEAT_COLORED_GATHER;
+           op->colored_impl != HEAT_COLORED_SWEEP;
+}
+
+// y = P A x, colour order, node-centric (march or one-thread-per-node gather)
+template <typename T>
+int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
+                   int check_done) {
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    if (op->colored_impl == HEAT_COLORED_GATHER) {
+        if (mass)
+            k_apply_gather<T, true, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, cr, ck, g, f, nullptr, 0.0, op->partials, op->st, check_done);
+        else
+            k_apply_gather<T, false, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, cr, ck, g, f, nullptr, 0.0, op->partials, op->st, check_done);
+        CHECK_LAUNCH();
+        return HEAT_OK;
+    }
+    const int nqb = (g.sx + NT - 1) / NT;
+    const int cx = op->march_cx;
+    const int nchunks = (g.nn1 + cx - 1) / cx;
+    const int nblocks = nqb * nchunks;
+    const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
+#define MARCH(CXV)                                                                                   \
+    do {                                                                                             \
+        if (mass) k_apply_march<T, true, CXV><<<grid, NT, 0, s>>>(x, y, cr, ck, g, f, nqb, nblocks, op->st, check_done);  \
+        else k_apply_march<T, false, CXV><<<grid, NT, 0, s>>>(x, y, cr, ck, g, f, nqb, nblocks, op->st, check_done);     \
+    } while (0)
+    switch (cx) {
+        case 2: MARCH(2); break;
+        case 4: MARCH(4); break;
+        case 16: MARCH(16); break;
+        case 32: MARCH(32); break;
+        default: MARCH(8); break;
+    }
+#undef MARCH
+    CHECK_LAUNCH();
+    return HEAT_OK;
 }
 
 // y = A x through the element-centric kernels (colour sweep or atomic),
@@ -211,28 +249,19 @@ int apply_impl(heat_op* op, const double* x, double* y, double dt_eff, bool allo
     const double nw = use_d1 ? (mass ? op->nwM : op->nwA) : 0.0;
     int rc0 = HEAT_OK;
     if (gather_path(op) && project) {
-#define GATHER(MASSV, EPIV)                                                                          \
-    k_apply_gather<double, MASSV, EPIV><<<op->nbn, NT, 0, s>>>(x, y, op->cr, op->ck, g, f, w, nw,     \
-                                                              op->partials, op->st, check_done)
-        if (mass) {
-            switch (epi) {
-                case EPI_NONE: GATHER(true, EPI_NONE); break;
-                case EPI_D1: GATHER(true, EPI_D1); break;
-                case EPI_PCG: GATHER(true, EPI_PCG); break;
-                default: GATHER(true, EPI_PCG_D1); break;
-            }
-        } else {
-            switch (epi) {
-                case EPI_NONE: GATHER(false, EPI_NONE); break;
-                case EPI_D1: GATHER(false, EPI_D1); break;
-                case EPI_PCG: GATHER(false, EPI_PCG); break;
-                default: GATHER(false, EPI_PCG_D1); break;
-            }
-        }
-#undef GATHER
-        CHECK_LAUNCH();
-        rc0 = launch_fin_apply(op, epi, nw, check_done);
+        rc0 = launch_colored<double>(op, x, y, op->cr, op->ck, mass, f, check_done);
         if (rc0) return rc0;
+        if (epi != EPI_NONE) {
+            // inner products of the pass, in the reference's block order
+            switch (epi) {
+                case EPI_D1: k_apply_post<double, EPI_D1><<<op->nbn, NT, 0, s>>>(x, y, g, 0, w, nw, op->partials, op->st, check_done); break;
+                case EPI_PCG: k_apply_post<double, EPI_PCG><<<op->nbn, NT, 0, s>>>(x, y, g, 0, w, nw, op->partials, op->st, check_done); break;
+                default: k_apply_post<double, EPI_PCG_D1><<<op->nbn, NT, 0, s>>>(x, y, g, 0, w, nw, op->partials, op->st, check_done); break;
+            }
+            CHECK_LAUNCH();
+            rc0 = launch_fin_apply(op, epi, nw, check_done);
+            if (rc0) return rc0;
+        }
     } else {
         if (check_done) return fail(HEAT_E_STATE, "internal: element path inside a captured solve");
         int rc = element_apply(op, x, y, dt_eff, f);
@@ -566,8 +595,15 @@ int heat_download(heat_op* op, double* host_dst, const double* dev_src, int64_t 
     return HEAT_OK;
 }
 
+int heat_set_march_chunk(heat_op* op, int planes) {
+    if (!op || (planes != 2 && planes != 4 && planes != 8 && planes != 16 && planes != 32))
+        return fail(HEAT_E_ARG, "march chunk must be 2, 4, 8, 16 or 32 planes");
+    op->march_cx = planes;
+    return HEAT_OK;
+}
+
 int heat_set_colored_impl(heat_op* op, int impl) {
-    if (!op || (impl != HEAT_COLORED_GATHER && impl != HEAT_COLORED_SWEEP)) return fail(HEAT_E_ARG, "bad impl");
+    if (!op || impl < HEAT_COLORED_GATHER || impl > HEAT_COLORED_MARCH) return fail(HEAT_E_ARG, "bad impl");
     op->colored_impl = impl;
     return HEAT_OK;
 }
@@ -587,12 +623,7 @@ int heat_apply_operator_f32(heat_op* op, const float* x, float* y, double dt_eff
     int rc = ensure_mixed(op);
     if (rc) return rc;
     FiArm none{0, 0, 0};
-    if (mass)
-        k_apply_gather<float, true, EPI_NONE><<<op->nbn, NT, 0, op->stream>>>(x, y, op->cr32, op->ck32, op->g, none, nullptr, 0.0, op->partials, op->st, 0);
-    else
-        k_apply_gather<float, false, EPI_NONE><<<op->nbn, NT, 0, op->stream>>>(x, y, op->cr32, op->ck32, op->g, none, nullptr, 0.0, op->partials, op->st, 0);
-    CHECK_LAUNCH();
-    return HEAT_OK;
+    return launch_colored<float>(op, x, y, op->cr32, op->ck32, mass, none, 0);
 }
 
 int heat_apply_matvec(heat_op* op, const double* x, double* y) {
@@ -757,8 +788,10 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         CHECK_LAUNCH();
         auto launch = [&]() -> int {
             for (int i = 0; i < op->pcg_chunk; ++i) {
-                k_apply_gather<float, false, EPI_PCG><<<op->nbn, NT, 0, s>>>(
-                    op->p32, op->ap32, op->cr32, op->ck32, g, none, nullptr, 0.0, op->partials, op->st, 1);
+                int rci = launch_colored<float>(op, op->p32, op->ap32, op->cr32, op->ck32, false, none, 1);
+                if (rci) return rci;
+                k_apply_post<float, EPI_PCG><<<op->nbn, NT, 0, s>>>(op->p32, op->ap32, g, 0, nullptr, 0.0,
+                                                                   op->partials, op->st, 1);
                 CHECK_LAUNCH();
                 k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1);
                 CHECK_LAUNCH();
@@ -879,6 +912,38 @@ int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t 
     if (rc) return rc;
     if (avg_ms) *avg_ms = total / reps;
     if (min_ms) *min_ms = best;
+    return HEAT_OK;
+}
+
+int heat_autotune(heat_op* op, int reps, int* impl, int* planes, double* best_ms) {
+    if (!op || reps < 1) return fail(HEAT_E_ARG, "bad autotune request");
+    if (!gather_path(op)) {
+        if (impl) *impl = op->colored_impl;
+        if (planes) *planes = op->march_cx;
+        if (best_ms) *best_ms = 0.0;
+        return HEAT_OK;   // only the colour-ordered fast apply has schedules to choose from
+    }
+    struct Cand { int impl, cx; };
+    const Cand cands[] = {{HEAT_COLORED_GATHER, 8}, {HEAT_COLORED_MARCH, 2}, {HEAT_COLORED_MARCH, 4},
+                          {HEAT_COLORED_MARCH, 8}, {HEAT_COLORED_MARCH, 16}, {HEAT_COLORED_MARCH, 32}};
+    double* x = op->vec[HEAT_VEC_P];
+    double* y = op->vec[HEAT_VEC_AP];
+    double best = 1e300;
+    Cand pick = {op->colored_impl, op->march_cx};
+    for (const Cand& c : cands) {
+        op->colored_impl = c.impl;
+        op->march_cx = c.cx;
+        double avg = 0, mn = 0;
+        int rc = heat_time_matvec(op, x, y, 3, 0, &avg, &mn);
+        if (rc == HEAT_OK) rc = heat_time_matvec(op, x, y, reps, 0, &avg, &mn);
+        if (rc) return rc;
+        if (avg < best) { best = avg; pick = c; }
+    }
+    op->colored_impl = pick.impl;
+    op->march_cx = pick.cx;
+    if (impl) *impl = pick.impl;
+    if (planes) *planes = pick.cx;
+    if (best_ms) *best_ms = best;
     return HEAT_OK;
 }
 
