@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="run this many device copies of one node vector first (PMC byte-counter calibration)")
     a = ap.parse_args()
@@ -107,16 +108,21 @@ def main():
     for _ in range(a.probe_copy):
         op.copy("p", "r")
     op.synchronize()
-    schedules = {}
-    for impl, planes in [("gather", 8), ("march", 2), ("march", 4), ("march", 8), ("march", 16), ("march", 32)]:
-        op.set_colored_impl(impl)
-        op.set_march_chunk(planes)
-        op.time_matvec("p", "Ap", 3)
-        ms, _ = op.time_matvec("p", "Ap", 50)
-        schedules[f"{impl}{planes if impl == 'march' else ''}"] = round(ms * 1e3, 2)
-    log(f"schedules (us/apply): {schedules}")
+    if a.sweep:
+        table = {}
+        cands = [("gather", 8, 0), ("march", 4, 0), ("march", 8, 0)] + \
+                [("tile", p, sh) for sh in (0, 1, 2) for p in (4, 8, 16)]
+        for impl, planes, shape in cands:
+            op.set_colored_impl(impl)
+            op.set_march_chunk(planes)
+            op.set_tile_shape(shape)
+            op.time_matvec("p", "Ap", 3)
+            ms, _ = op.time_matvec("p", "Ap", 100)
+            table[f"{impl}/p{planes}/s{shape}"] = round(ms * 1e3, 2)
+        log(f"schedules (us/apply): {table}")
     pick = op.autotune(50)
-    log(f"autotuner picked {pick[0]} planes={pick[1]} ({pick[2] * 1e3:.2f} us)")
+    picked = f"{pick[0]} planes={pick[1]} tile={pick[2][0]}x{pick[2][1]}"
+    log(f"autotuner picked {picked} ({pick[3] * 1e3:.2f} us)")
     log(f"warmup {a.warmup}")
     op.time_matvec("p", "Ap", max(1, a.warmup))
     torch.cuda.synchronize()
@@ -156,7 +162,7 @@ def main():
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
                        "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
         "algorithmic_bytes_per_apply": alg_bytes,
-        "schedule": {"picked": f"{pick[0]}{pick[1] if pick[0] == 'march' else ''}", "us_per_apply": schedules},
+        "schedule": {"picked": picked, "us_per_apply": round(pick[3] * 1e3, 2)},
     }
 
     if not a.no_solve:

@@ -46,7 +46,8 @@ extern "C" {
 /* colored-scatter implementations (all give the same bits) */
 #define HEAT_COLORED_GATHER 0   /* one thread per node, contributions summed in colour order */
 #define HEAT_COLORED_SWEEP  1   /* eight element launches, one per colour */
-#define HEAT_COLORED_MARCH  2   /* threads walk x-columns of nodes with a register window (default) */
+#define HEAT_COLORED_MARCH  2   /* threads walk x-columns of nodes with a register window */
+#define HEAT_COLORED_TILE   3   /* blocks walk x with LDS-staged plane tiles (default) */
 
 /* handle-owned vectors for heat_vector() */
 #define HEAT_VEC_T    0
@@ -162,8 +163,11 @@ int heat_abft_take(heat_op* op, double* step_max, int64_t* calls);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);
-/* x-planes walked per thread block by HEAT_COLORED_MARCH (2, 4, 8, 16 or 32). */
+/* x-planes walked per thread block by HEAT_COLORED_MARCH / _TILE (2, 4, 8,
+ * 16 or 32; the tiled kernel takes 4, 8 or 16). */
 int heat_set_march_chunk(heat_op* op, int planes);
+/* Node tile of HEAT_COLORED_TILE: 0 = 8 x 32 (y x z), 1 = 4 x 64, 2 = 16 x 16. */
+int heat_set_tile_shape(heat_op* op, int shape);
 /* Benchmark helper (new): time `reps` back-to-back apply_matvec launches
  * (y = P A x, dt_eff = dt, no fault, no D1) with HIP events on the handle's
  * stream.  With flush_bytes > 0 a buffer of that size is overwritten before
@@ -174,8 +178,8 @@ int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t 
 /* Autotuner (new): time every colour-ordered apply schedule (one thread per
  * node, and plane marching with 2..32 planes per block) on this handle's
  * mesh with HIP events, keep the fastest for all later applies, and report it
- * (*impl = HEAT_COLORED_*, *planes, *best_ms per apply). */
-int heat_autotune(heat_op* op, int reps, int* impl, int* planes, double* best_ms);
+ * (*impl = HEAT_COLORED_*, *planes, *shape, *best_ms per apply). */
+int heat_autotune(heat_op* op, int reps, int* impl, int* planes, int* shape, double* best_ms);
 /* The D1 threshold (chk_tol, src/heat_solver_het.cu:637) used by the driver. */
 int heat_abft_threshold(heat_op* op, double* chk_tol);
 

@@ -378,6 +378,181 @@ __global__ __launch_bounds__(NT) void k_apply_march(const T* __restrict__ x, T* 
 }
 
 // ---------------------------------------------------------------------------
+// LDS-tiled plane walk (same sums again; the default colour-ordered apply).
+//
+// A block of TY*TZ threads owns a TY x TZ tile of node columns (iy, iz) and
+// walks the x-planes of its chunk.  Each step, every thread fetches its share
+// (about 1.3 values of x and 1.2 elements) of the block's NEXT x-plane tile
+// (with a one-node halo) and element-plane tile into registers, computes the
+// current plane from its register window, then parks the fetched values in
+// double-buffered LDS; one barrier per plane.  The window's new 3x3 plane and
+// the new elements are read from LDS at the parity-mirrored offsets.  Global
+// traffic per node is one x value, one element pair and one y value plus the
+// tile halos; the global loads of a step are in flight during the previous
+// step's arithmetic.
+// Boundary rows/columns/planes are written as 0 (projection); the rows iy = 0
+// and iz = 0, which no tile owns, are zeroed by the edge threads of the first
+// tile row/column.
+// ---------------------------------------------------------------------------
+template <typename T, bool MASS, int TY, int TZ, int CX>
+__global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, T* __restrict__ y,
+                                                       const T* __restrict__ cr,
+                                                       const T* __restrict__ ck, Grid g, FiArm fi,
+                                                       int nby, int nbz, int nblocks,
+                                                       const DevState* st, int check_done) {
+    constexpr int NTH = TY * TZ;
+    constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;   // x tile with halo
+    constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;   // element tile
+    constexpr int XR = (XN + NTH - 1) / NTH;                 // x values fetched per thread
+    constexpr int ER = (EN + NTH - 1) / NTH;                 // elements fetched per thread
+    __shared__ T xs[2][XN];
+    __shared__ T es_r[2][EN];
+    __shared__ T es_k[2][MASS ? 1 : EN];
+    if (check_done && st->done) return;
+    const int b = xcd_slab_block(nblocks);
+    if (b >= nblocks) return;                 // whole block: no barrier is skipped by part of it
+    const int bz = b % nbz;
+    const int by = (b / nbz) % nby;
+    const int chunk = b / (nbz * nby);
+    const int n = g.n, S = g.sx, nn1 = g.nn1, nsq = n * n;
+    const int tid = threadIdx.x;
+    const int ty = tid / TZ, tz = tid - (tid / TZ) * TZ;
+    const int y0 = 1 + by * TY, z0 = 1 + bz * TZ;
+    const int iy = y0 + ty, iz = z0 + tz;
+    const bool inmesh = (iy <= n) && (iz <= n);
+    const bool interior = (iy < n) && (iz < n);
+    const int ix0 = chunk * CX;
+    const int ix1 = min(ix0 + CX, n + 1);
+    const int py = iy & 1, pz = iz & 1;
+    const int sy = py ? -1 : 1, sz = pz ? -1 : 1;          // mirror signs
+    // cooperative fetch slots
+    int xoff[XR], eoff_t[ER];
+    bool xok[XR], eok[ER];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int i = tid + k * NTH;
+        const int r = i / XW, c = i - (i / XW) * XW;
+        const int yy = y0 - 1 + r, zz = z0 - 1 + c;
+        xok[k] = (i < XN) && yy <= n && zz <= n;
+        xoff[k] = yy * nn1 + zz;
+    }
+#pragma unroll
+    for (int k = 0; k < ER; ++k) {
+        const int i = tid + k * NTH;
+        const int r = i / EW, c = i - (i / EW) * EW;
+        const int ey = y0 - 1 + r, ez = z0 - 1 + c;
+        eok[k] = (i < EN) && ey < n && ez < n;
+        eoff_t[k] = ey * n + ez;
+    }
+    T xf[XR], ef_r[ER], ef_k[ER];
+    auto fetch = [&](int xplane, int eplane) {
+#pragma unroll
+        for (int k = 0; k < XR; ++k)
+            xf[k] = (xplane <= n && xok[k]) ? x[(size_t)xplane * S + xoff[k]] : T(0);
+#pragma unroll
+        for (int k = 0; k < ER; ++k) {
+            const bool ok = eplane < n && eok[k];
+            const size_t e = (size_t)eplane * nsq + eoff_t[k];
+            ef_r[k] = ok ? cr[e] : T(0);
+            ef_k[k] = (!MASS && ok) ? ck[e] : T(0);
+        }
+    };
+    auto park = [&](int xslot, int eslot) {
+#pragma unroll
+        for (int k = 0; k < XR; ++k)
+            if (tid + k * NTH < XN) xs[xslot][tid + k * NTH] = xf[k];
+#pragma unroll
+        for (int k = 0; k < ER; ++k)
+            if (tid + k * NTH < EN) {
+                es_r[eslot][tid + k * NTH] = ef_r[k];
+                if (!MASS) es_k[eslot][tid + k * NTH] = ef_k[k];
+            }
+    };
+    // element (cy, cz) of a plane in tile coordinates, and window plane reads
+    int eidx[2][2];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EW + (tz + 1 - (cz ^ pz));
+    int eoff[2][2];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
+    T W[3][3][3];
+    Coef<T> Eprev[2][2], Enew[2][2];
+    // prologue: planes ix0-1 and ix0 and element plane ix0-1 straight from memory
+    if (interior) {
+        const size_t q = (size_t)iy * nn1 + iz;
+        if (ix0 >= 1) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    W[0][bb][c] = x[(size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    const size_t e = (size_t)(ix0 - 1) * nsq + eoff[cy][cz];
+                    Eprev[cy][cz].cr = cr[e];
+                    Eprev[cy][cz].ck = MASS ? T(0) : ck[e];
+                }
+        }
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+    }
+    fetch(ix0 + 1, ix0);
+    park((ix0 + 1) & 1, ix0 & 1);
+    __syncthreads();
+    const int par_yz = (py << 1) | pz;
+    for (int ix = ix0; ix < ix1; ++ix) {
+        fetch(ix + 2, ix + 1);                     // in flight during this step
+        if (interior && ix <= n - 1) {
+            const T* xp = xs[(ix + 1) & 1];
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) W[2][bb][c] = xp[(ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz)];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    Enew[cy][cz].cr = es_r[ix & 1][eidx[cy][cz]];
+                    Enew[cy][cz].ck = MASS ? T(0) : es_k[ix & 1][eidx[cy][cz]];
+                }
+        }
+        if (inmesh) {
+            T acc = T(0);
+            if (interior && ix >= 1 && ix <= n - 1) {
+                if (ix & 1) acc = march_node<T, MASS, 1>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+                else acc = march_node<T, MASS, 0>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+            }
+            const size_t base = (size_t)ix * S;
+            y[base + (size_t)iy * nn1 + iz] = acc;
+            if (by == 0 && ty == 0) y[base + iz] = T(0);                       // row iy = 0
+            if (bz == 0 && tz == 0) y[base + (size_t)iy * nn1] = T(0);         // column iz = 0
+            if (by == 0 && bz == 0 && tid == 0) y[base] = T(0);                // corner
+        }
+        park(ix & 1, (ix + 1) & 1);
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                W[0][bb][c] = W[1][bb][c];
+                W[1][bb][c] = W[2][bb][c];
+            }
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Element-centric apply (operator_kernel, src/heat_solver_het.cu:351-378):
 // thread per element of one colour (color >= 0, plain +=) or of the whole
 // mesh (color < 0, atomic add).  KIND 0: closed-form rows, KIND 1: Gauss.

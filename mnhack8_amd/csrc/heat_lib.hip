@@ -53,8 +53,9 @@ struct heat_op {
     heat_config cfg;
     Grid g;
     hipStream_t stream = nullptr;
-    int colored_impl = HEAT_COLORED_MARCH;
+    int colored_impl = HEAT_COLORED_TILE;
     int march_cx = 8;
+    int tile_shape = 0;
     int pcg_chunk = 8;
     int use_graph = 1;
     double m3 = 0, k2 = 0;
@@ -152,7 +153,42 @@ bool gather_path(const heat_op* op) {
            op->colored_impl != HEAT_COLORED_SWEEP;
 }
 
-// y = P A x, colour order, node-centric (march or one-thread-per-node gather)
+template <typename T, int TY, int TZ, int CX>
+void tile_launch1(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
+                  int check_done) {
+    const Grid& g = op->g;
+    const int nby = (g.n + TY - 1) / TY, nbz = (g.n + TZ - 1) / TZ, nch = (g.nn1 + CX - 1) / CX;
+    const int nblocks = nby * nbz * nch;
+    const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
+    if (mass)
+        k_apply_tile<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, op->stream>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+    else
+        k_apply_tile<T, false, TY, TZ, CX><<<grid, TY * TZ, 0, op->stream>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+}
+
+template <typename T, int TY, int TZ>
+void tile_launch_cx(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
+                    int check_done) {
+    switch (op->march_cx) {
+        case 4: tile_launch1<T, TY, TZ, 4>(op, x, y, cr, ck, mass, f, check_done); break;
+        case 16: tile_launch1<T, TY, TZ, 16>(op, x, y, cr, ck, mass, f, check_done); break;
+        default: tile_launch1<T, TY, TZ, 8>(op, x, y, cr, ck, mass, f, check_done); break;
+    }
+}
+
+template <typename T>
+int launch_tile(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
+                int check_done) {
+    switch (op->tile_shape) {
+        case 1: tile_launch_cx<T, 4, 64>(op, x, y, cr, ck, mass, f, check_done); break;
+        case 2: tile_launch_cx<T, 16, 16>(op, x, y, cr, ck, mass, f, check_done); break;
+        default: tile_launch_cx<T, 8, 32>(op, x, y, cr, ck, mass, f, check_done); break;
+    }
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+// y = P A x, colour order, node-centric (tile, march or one-thread-per-node gather)
 template <typename T>
 int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
                    int check_done) {
@@ -166,6 +202,7 @@ int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool
         CHECK_LAUNCH();
         return HEAT_OK;
     }
+    if (op->colored_impl == HEAT_COLORED_TILE) return launch_tile<T>(op, x, y, cr, ck, mass, f, check_done);
     const int nqb = (g.sx + NT - 1) / NT;
     const int cx = op->march_cx;
     const int nchunks = (g.nn1 + cx - 1) / cx;
@@ -603,7 +640,7 @@ int heat_set_march_chunk(heat_op* op, int planes) {
 }
 
 int heat_set_colored_impl(heat_op* op, int impl) {
-    if (!op || impl < HEAT_COLORED_GATHER || impl > HEAT_COLORED_MARCH) return fail(HEAT_E_ARG, "bad impl");
+    if (!op || impl < HEAT_COLORED_GATHER || impl > HEAT_COLORED_TILE) return fail(HEAT_E_ARG, "bad impl");
     op->colored_impl = impl;
     return HEAT_OK;
 }
@@ -915,24 +952,28 @@ int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t 
     return HEAT_OK;
 }
 
-int heat_autotune(heat_op* op, int reps, int* impl, int* planes, double* best_ms) {
+int heat_autotune(heat_op* op, int reps, int* impl, int* planes, int* shape, double* best_ms) {
     if (!op || reps < 1) return fail(HEAT_E_ARG, "bad autotune request");
     if (!gather_path(op)) {
         if (impl) *impl = op->colored_impl;
         if (planes) *planes = op->march_cx;
+        if (shape) *shape = op->tile_shape;
         if (best_ms) *best_ms = 0.0;
         return HEAT_OK;   // only the colour-ordered fast apply has schedules to choose from
     }
-    struct Cand { int impl, cx; };
-    const Cand cands[] = {{HEAT_COLORED_GATHER, 8}, {HEAT_COLORED_MARCH, 2}, {HEAT_COLORED_MARCH, 4},
-                          {HEAT_COLORED_MARCH, 8}, {HEAT_COLORED_MARCH, 16}, {HEAT_COLORED_MARCH, 32}};
+    struct Cand { int impl, cx, shape; };
+    const Cand cands[] = {{HEAT_COLORED_GATHER, 8, 0}, {HEAT_COLORED_MARCH, 4, 0}, {HEAT_COLORED_MARCH, 8, 0},
+                          {HEAT_COLORED_TILE, 4, 0},   {HEAT_COLORED_TILE, 8, 0},  {HEAT_COLORED_TILE, 16, 0},
+                          {HEAT_COLORED_TILE, 4, 1},   {HEAT_COLORED_TILE, 8, 1},  {HEAT_COLORED_TILE, 16, 1},
+                          {HEAT_COLORED_TILE, 4, 2},   {HEAT_COLORED_TILE, 8, 2},  {HEAT_COLORED_TILE, 16, 2}};
     double* x = op->vec[HEAT_VEC_P];
     double* y = op->vec[HEAT_VEC_AP];
     double best = 1e300;
-    Cand pick = {op->colored_impl, op->march_cx};
+    Cand pick = {op->colored_impl, op->march_cx, op->tile_shape};
     for (const Cand& c : cands) {
         op->colored_impl = c.impl;
         op->march_cx = c.cx;
+        op->tile_shape = c.shape;
         double avg = 0, mn = 0;
         int rc = heat_time_matvec(op, x, y, 3, 0, &avg, &mn);
         if (rc == HEAT_OK) rc = heat_time_matvec(op, x, y, reps, 0, &avg, &mn);
@@ -941,9 +982,17 @@ int heat_autotune(heat_op* op, int reps, int* impl, int* planes, double* best_ms
     }
     op->colored_impl = pick.impl;
     op->march_cx = pick.cx;
+    op->tile_shape = pick.shape;
     if (impl) *impl = pick.impl;
     if (planes) *planes = pick.cx;
+    if (shape) *shape = pick.shape;
     if (best_ms) *best_ms = best;
+    return HEAT_OK;
+}
+
+int heat_set_tile_shape(heat_op* op, int shape) {
+    if (!op || shape < 0 || shape > 2) return fail(HEAT_E_ARG, "tile shape must be 0, 1 or 2");
+    op->tile_shape = shape;
     return HEAT_OK;
 }
 

@@ -25,7 +25,8 @@ DRIVER_PATH = os.path.join(_HERE, "_build", "heat_solver_het")
 KERNEL = {"gauss": 0, "fast": 1}
 SCATTER = {"atomic": 0, "colored": 1}
 SOURCE = {"gaussian": 0, "const": 1}
-COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2}
+COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3}
+TILE_SHAPES = {0: (8, 32), 1: (4, 64), 2: (16, 16)}
 VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "bad field", 4: "self-check failed", 5: "bad state"}
 
@@ -79,7 +80,8 @@ SIGNATURES = {
     "heat_abft_threshold": (_I, [_P, C.POINTER(_D)]),
     "heat_set_tuning": (_I, [_P, _I, _I]),
     "heat_set_march_chunk": (_I, [_P, _I]),
-    "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
+    "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
+    "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
 }
 
@@ -186,10 +188,14 @@ class HeatOperator:
 
     def autotune(self, reps=20):
         """Time every colour-ordered schedule on this mesh and keep the fastest."""
-        impl, planes, ms = C.c_int(), C.c_int(), C.c_double()
-        _check(self._lib.heat_autotune(self.h, int(reps), C.byref(impl), C.byref(planes), C.byref(ms)))
+        impl, planes, shape, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.heat_autotune(self.h, int(reps), C.byref(impl), C.byref(planes), C.byref(shape),
+                                       C.byref(ms)))
         names = {v: k for k, v in COLORED_IMPL.items()}
-        return names[impl.value], planes.value, ms.value
+        return names[impl.value], planes.value, TILE_SHAPES[shape.value], ms.value
+
+    def set_tile_shape(self, shape):
+        _check(self._lib.heat_set_tile_shape(self.h, int(shape)))
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))

@@ -77,6 +77,7 @@ def write_report(outdir, name, passed, details):
 
 
 def gen(model, n, out, **kw):
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     if not os.path.exists(out + "_k.bin"):
         if model == "homogeneous":
             k, r = fields.homogeneous(n, kw.get("k", 1.0), kw.get("rhoc", 1.0))

@@ -1,0 +1,259 @@
+"""The CPU oracle and the HIP solver's command-line driver against the
+reference's own recorded outputs.
+
+Golden data: tests/golden/v10/ (copied from the reference's
+tests/baseline_validation/v10 by tests/golden/collect_fixtures.py).  Integers
+and flags (CG counts, classes, detectors, first-firing steps) must match
+exactly; printed floating-point values to the precision they were printed
+with, widened only where the value is a difference of two runs that sits at
+the CG tolerance's noise level.
+"""
+import csv
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mnhack8_amd import build, fields, validations
+from oracle_lib import ORACLE_CLI
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "v10")
+THREADS = str(min(8, os.cpu_count() or 1))
+
+
+def gold_csv(rel):
+    with open(os.path.join(GOLD, rel)) as f:
+        return list(csv.DictReader(f))
+
+
+@pytest.fixture(params=["oracle", pytest.param("hip", marks=pytest.mark.gpu)])
+def solver(request):
+    """The binary under test: the CPU oracle, or the HIP driver (GPU)."""
+    return request.param
+
+
+def run_oracle(args, tmp, solver="oracle"):
+    if solver == "oracle":
+        cmd = [ORACLE_CLI, *map(str, args), "--threads", THREADS]
+    else:
+        cmd = [build.DRIVER, *map(str, args)]
+        print(f"[case] {' '.join(cmd)}", flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.fixture(scope="module")
+def ln32(tmp_path_factory):
+    d = tmp_path_factory.mktemp("fields")
+    p = str(d / "ln32")
+    k, r = fields.lognormal(32, sigma=1.0, L=0.125, seed=7)
+    fields.write_field(p, k, r, {"n": 32})
+    return p
+
+
+@pytest.fixture(scope="module")
+def ln16(tmp_path_factory):
+    d = tmp_path_factory.mktemp("fields16")
+    p = str(d / "ln16")
+    k, r = fields.lognormal(16, sigma=1.0, L=0.125, seed=7)
+    fields.write_field(p, k, r, {"n": 16})
+    return p
+
+
+def close_print(a, b, digits):
+    """a (computed) equals b (printed with `digits` significant digits)."""
+    a, b = float(a), float(b)
+    if b == 0.0:
+        return abs(a) < 10.0 ** (-digits)
+    return abs(a - b) <= 0.51 * 10.0 ** (np.floor(np.log10(abs(b))) - digits + 1) * 1.0001
+
+
+@pytest.mark.parametrize("n,rel", [(16, "ln16_meta.json"), (32, "v6/ln32_meta.json")])
+def test_lognormal_field_matches_reference_meta(n, rel):
+    meta = json.load(open(os.path.join(GOLD, rel)))
+    k, rhoc = fields.lognormal(n, sigma=meta["sigma"], L=meta["L"], seed=meta["seed"])
+    assert float(k.min()) == meta["kmin"] and float(k.max()) == meta["kmax"]
+    assert float(rhoc.min()) == meta["rhocmin"] == 1.0
+
+
+@pytest.mark.parametrize("kernel,scatter", [("fast", "colored"), ("gauss", "colored"), ("fast", "atomic")])
+def test_v0_temporal_errors(tmp_path, kernel, scatter, solver):
+    """V0 known answers: rel L2 error of the sine eigenmode at N=32 for four dt."""
+    gold = {(int(r["N"]), float(r["dt"])): r for r in gold_csv(f"v0_{kernel}_{scatter}_np1/verify.csv")}
+    fp = str(tmp_path / "homog32")
+    fields.write_field(fp, *fields.homogeneous(32), {})
+    for N, t, dt in validations.TEMPORAL:
+        run_oracle([N, t, dt, fp, tmp_path, "--verify", "--kernel", kernel, "--scatter", scatter], tmp_path, solver)
+    rows = list(csv.DictReader(open(tmp_path / "verify.csv")))
+    for r in rows:
+        g = gold[(int(r["N"]), float(r["dt"]))]
+        assert close_print(r["rel_L2_err"], g["rel_L2_err"], 11), (r, g)
+        assert r["avg_cg_iters"] == g["avg_cg_iters"]
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_v0_spatial_errors(tmp_path, N, solver):
+    gold = {(int(r["N"]), float(r["dt"])): r for r in gold_csv("v0_fast_colored_np1/verify.csv")}
+    fp = str(tmp_path / f"homog{N}")
+    fields.write_field(fp, *fields.homogeneous(N), {})
+    run_oracle([N, 0.02, 1e-5, fp, tmp_path, "--verify", "--kernel", "fast", "--scatter", "colored"], tmp_path, solver)
+    r = list(csv.DictReader(open(tmp_path / "verify.csv")))[0]
+    g = gold[(N, 1e-5)]
+    assert close_print(r["rel_L2_err"], g["rel_L2_err"], 11)
+    assert r["avg_cg_iters"] == g["avg_cg_iters"] == "1.000"
+
+
+def test_v0_planted_mistake_is_caught(tmp_path, solver):
+    """The V0 pin catches a wrong time step of one part in 1e6."""
+    fp = str(tmp_path / "homog32")
+    fields.write_field(fp, *fields.homogeneous(32), {})
+    run_oracle([32, 0.04, 4e-3 * (1 + 1e-6), fp, tmp_path, "--verify"], tmp_path, solver)
+    r = list(csv.DictReader(open(tmp_path / "verify.csv")))[0]
+    g = gold_csv("v0_fast_colored_np1/verify.csv")[3]
+    assert not close_print(r["rel_L2_err"], g["rel_L2_err"], 11)
+
+
+def test_v4_verify_value(tmp_path, solver):
+    fp = str(tmp_path / "homog16")
+    fields.write_field(fp, *fields.homogeneous(16), {})
+    run_oracle([16, 0.02, 1e-4, fp, tmp_path, "--verify", "--kernel", "fast", "--scatter", "colored"], tmp_path, solver)
+    r = list(csv.DictReader(open(tmp_path / "verify.csv")))[0]
+    g = gold_csv("v4/verify_np1/verify.csv")[0]
+    assert close_print(r["rel_L2_err"], g["rel_L2_err"], 11)
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_v6_clean_trace(tmp_path, ln32, scatter, solver):
+    """Per-step CG counts, D2 and D3 of the N=32 lognormal run with --detect."""
+    run_oracle([32, 0.02, 5e-4, ln32, tmp_path, "--tag", "det", "--snap", "1", "--kernel", "fast", "--scatter",
+                scatter, "--detect"], tmp_path, solver)
+    mine = list(csv.DictReader(open(tmp_path / "fi_trace_det.csv")))
+    gold = gold_csv(f"v6/clean_{scatter}/fi_trace_det.csv")
+    assert len(mine) == len(gold) == 40
+    for m, g in zip(mine, gold):
+        assert m["cg_A"] == g["cg_A"] and m["cg_B"] == g["cg_B"]
+        assert close_print(m["d2_resid_A"], g["d2_resid_A"], 4)
+        assert close_print(m["d3_tmin_F"], g["d3_tmin_F"], 7)
+        assert close_print(m["d3_tmax_F"], g["d3_tmax_F"], 7)
+        assert float(m["d1_abft_A"]) < 1e-15 and float(g["d1_abft_A"]) < 1e-15
+    sm = list(csv.DictReader(open(tmp_path / "fi_summary_det.csv")))[0]
+    gs = gold_csv(f"v6/clean_{scatter}/fi_summary_det.csv")[0]
+    for key in ("mean_cg_iters_A", "d1_first", "d2_first", "d3_first", "detected_by", "class", "kmin", "kmax"):
+        assert sm[key] == gs[key], key
+    if scatter == "colored":
+        assert sm["E_floor_max"] == gs["E_floor_max"] == "0.000000e+00"
+
+
+@pytest.mark.parametrize("level,bit", [(lv, b) for lv in (1, 2) for b in (30, 51, 55, 62, 63)])
+def test_v6_fault_campaign(tmp_path, ln32, level, bit, solver):
+    """Classes, detectors and first-firing steps of the G/R campaign (S = 10)."""
+    N = 32
+    target = validations.node_index(16, 16, 16, N) if level == 1 else validations.elem_index(16, 16, 16, N)
+    run_oracle([N, 0.02, 5e-4, ln32, tmp_path, "--tag", "det", "--snap", "1", "--kernel", "fast", "--scatter",
+                "colored", "--detect", "--fi", level, 10, target, bit], tmp_path, solver)
+    sm = list(csv.DictReader(open(tmp_path / "fi_summary_det.csv")))[0]
+    gs = gold_csv(f"v6/L{level}_b{bit}/fi_summary_det.csv")[0]
+    for key in ("class", "detected_by", "d1_first", "d2_first", "d3_first", "iters_delta_max", "target_on_boundary",
+                "step_at_max", "fi_target", "fi_bit", "fi_level"):
+        assert sm[key] == gs[key], (key, sm[key], gs[key])
+    tol_digits = 4 if bit == 30 else 7        # bit 30 sits at the CG tolerance's noise level
+    if gs["class"] != "detected":
+        assert close_print(sm["E_fault_max"], gs["E_fault_max"], tol_digits)
+    trace = list(csv.DictReader(open(tmp_path / "fi_trace_det.csv")))
+    gtrace = gold_csv(f"v6/L{level}_b{bit}/fi_trace_det.csv")
+    assert [r["cg_F"] for r in trace] == [r["cg_F"] for r in gtrace]
+
+
+def test_v5_gauss_vs_fast(tmp_path, ln16, solver):
+    for kern in ("gauss", "fast"):
+        d = tmp_path / kern
+        d.mkdir()
+        run_oracle([16, 0.02, 5e-4, ln16, d, "--tag", "eq", "--snap", "0", "--kernel", kern, "--scatter", "colored"],
+                   tmp_path, solver)
+        sm = list(csv.DictReader(open(d / "fi_summary_eq.csv")))[0]
+        gs = gold_csv(f"v5/{kern}/fi_summary_eq.csv")[0]
+        assert sm["mean_cg_iters_A"] == gs["mean_cg_iters_A"] == "18.32"
+        tr = list(csv.DictReader(open(d / "fi_trace_eq.csv")))
+        assert tr[-1]["cg_A"] == gold_csv(f"v5/{kern}/fi_trace_eq.csv")[-1]["cg_A"] == "17"
+    Tg = np.fromfile(tmp_path / "gauss" / "T_final_eq_clean_rank0.bin")
+    Tf = np.fromfile(tmp_path / "fast" / "T_final_eq_clean_rank0.bin")
+    assert np.linalg.norm(Tf - Tg) / np.linalg.norm(Tg) <= 1e-12
+
+
+def test_v1_quadrants_norm(tmp_path, solver):
+    """V1: ||T_final|| of the quadrants field, N=16, t=0.01, dt=1e-4, gauss/atomic."""
+    gold = json.load(open(os.path.join(GOLD, "v1/report.json")))["details"]
+    fp = str(tmp_path / "quad16")
+    fields.write_field(fp, *fields.quadrants(16), {})
+    run_oracle([16, 0.01, 1e-4, fp, tmp_path, "--tag", "reg", "--snap", "0", "--kernel", "gauss", "--scatter",
+                "atomic"], tmp_path, solver)
+    T = np.fromfile(tmp_path / "T_final_reg_clean_rank0.bin")
+    nrm = float(np.linalg.norm(T))
+    assert abs(nrm - gold["even"]["v2_norm"]) / gold["even"]["v2_norm"] < 1e-12
+    assert abs(nrm - gold["even"]["ref_norm"]) / gold["even"]["ref_norm"] <= 1e-6
+
+
+@pytest.mark.parametrize("level,bit", [(1, 0), (2, 0), (1, 55), (2, 62)])
+def test_v3_injection_runs(tmp_path, ln32, level, bit, solver):
+    N = 32
+    target = validations.node_index(16, 16, 16, N) if level == 1 else validations.elem_index(16, 16, 16, N)
+    run_oracle([N, 0.02, 5e-4, ln32, tmp_path, "--tag", "fi", "--snap", "1", "--kernel", "fast", "--scatter",
+                "colored", "--fi", level, 10, target, bit], tmp_path, solver)
+    sm = list(csv.DictReader(open(tmp_path / "fi_summary_fi.csv")))[0]
+    gs = gold_csv(f"v3/L{level}_b{bit}/fi_summary_fi.csv")[0]
+    keys = ("class", "iters_delta_max", "target_on_boundary")
+    if bit != 0:                      # bit 0 leaves E at rounding level: its argmax is noise
+        keys += ("step_at_max",)
+    for key in keys:
+        assert sm[key] == gs[key], key
+    tr = list(csv.DictReader(open(tmp_path / "fi_trace_fi.csv")))
+    E = np.array([float(r["E_fault"]) for r in tr])
+    assert (E[:10] == 0).all() and (E[10] > 0 or not np.isfinite(E[10]))
+
+
+def test_v3_boundary_control(tmp_path, ln32, solver):
+    N = 32
+    b = validations.node_index(0, 16, 16, N)
+    out = {}
+    for tag, extra in (("G_boundary", []), ("G_boundary_reproject", ["--reproject-bc"])):
+        d = tmp_path / tag
+        d.mkdir()
+        run_oracle([N, 0.02, 5e-4, ln32, d, "--tag", "fi", "--snap", "1", "--kernel", "fast", "--scatter", "colored",
+                    "--fi", 1, 10, b, 62, *extra], tmp_path, solver)
+        out[tag] = list(csv.DictReader(open(d / "fi_summary_fi.csv")))[0]
+        gs = gold_csv(f"v3/{tag}/fi_summary_fi.csv")[0]
+        assert out[tag]["class"] == gs["class"] and out[tag]["target_on_boundary"] == gs["target_on_boundary"] == "1"
+        assert close_print(out[tag]["E_fault_max"], gs["E_fault_max"], 7)
+    red = float(out["G_boundary"]["E_fault_final"]) / float(out["G_boundary_reproject"]["E_fault_final"])
+    assert red >= 100
+
+
+def test_level_h_and_qrank_rejected(tmp_path, ln32, solver):
+    binary = ORACLE_CLI if solver == "oracle" else build.DRIVER
+    r = subprocess.run([binary, "32", "0.02", "5e-4", ln32, str(tmp_path), "--fi", "3", "10", "5", "51"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "level H needs >= 2 ranks" in r.stderr
+
+
+def test_noise_floor_colored_is_zero(tmp_path, ln32, solver):
+    run_oracle([32, 0.02, 5e-4, ln32, tmp_path, "--tag", "nf", "--snap", "1", "--kernel", "fast", "--scatter",
+                "colored"], tmp_path, solver)
+    sm = list(csv.DictReader(open(tmp_path / "fi_summary_nf.csv")))[0]
+    gs = gold_csv("v2/colored_00/fi_summary_nf.csv")[0]
+    assert sm["E_floor_max"] == gs["E_floor_max"] == "0.000000e+00"
+    assert sm["mean_cg_iters_A"] == gs["mean_cg_iters_A"]
+
+
+def test_struct_kernel_reproduces_pins(tmp_path, ln32):
+    """The closed-form rows the GPU uses give the same integer pins and the
+    same printed traces as the reference's matrix form."""
+    run_oracle([32, 0.02, 5e-4, ln32, tmp_path, "--tag", "det", "--snap", "1", "--kernel", "struct", "--scatter",
+                "colored", "--detect"], tmp_path)
+    mine = list(csv.DictReader(open(tmp_path / "fi_trace_det.csv")))
+    gold = gold_csv("v6/clean_colored/fi_trace_det.csv")
+    for m, g in zip(mine, gold):
+        assert m["cg_A"] == g["cg_A"]
+        assert close_print(m["d3_tmax_F"], g["d3_tmax_F"], 7)

@@ -68,6 +68,14 @@ def test_apply_colored_bitwise(n, dt_eff):
         ym = dev(np.full(op.nn, np.nan))
         op.apply_operator(xd, ym, dt_eff, project=True)
         assert np.array_equal(host(op, ym), y_gather), f"march({planes}) != gather"
+    for shape in (0, 1, 2):
+        for planes in (4, 8, 16):
+            op.set_colored_impl("tile")
+            op.set_tile_shape(shape)
+            op.set_march_chunk(planes)
+            ym = dev(np.full(op.nn, np.nan))
+            op.apply_operator(xd, ym, dt_eff, project=True)
+            assert np.array_equal(host(op, ym), y_gather), f"tile({shape},{planes}) != gather"
     op.set_colored_impl("sweep")
     yd2 = dev(np.zeros(op.nn))
     op.apply_operator(xd, yd2, dt_eff, project=True)
@@ -205,7 +213,7 @@ def test_pcg_edge_cases():
     assert op.pcg_solve("b", "T", 0, 1e-8) == 0
 
 
-@pytest.mark.parametrize("impl", ["march", "gather"])
+@pytest.mark.parametrize("impl", ["tile", "march", "gather"])
 @pytest.mark.parametrize("bit", [30, 55, 62, 63])
 def test_fault_level_R_and_G(bit, impl):
     n, dt = 8, 5e-4
@@ -324,8 +332,8 @@ def test_autotune_keeps_bits():
     orc = OracleOp(n, dt, k, rhoc, kernel="struct")
     x = np.random.default_rng(4).standard_normal(op.nn)
     op.upload("p", x)
-    impl, planes, ms = op.autotune(5)
-    assert impl in ("gather", "march") and ms > 0
+    impl, planes, shape, ms = op.autotune(5)
+    assert impl in ("gather", "march", "tile") and ms > 0
     op.upload("p", x)
     op.apply_operator("p", "Ap", dt)
     assert np.array_equal(op.download("Ap"), orc.apply(x, dt))

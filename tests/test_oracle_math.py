@@ -1,0 +1,219 @@
+"""Pins of the CPU oracle that follow from the mathematics, independent of
+the reference's recorded runs: closed forms of the element matrices, an exact
+discrete eigenvector, symmetry, the constant null space of K, the reduction
+order against a correctly rounded sum, and one-iteration CG convergence on
+the eigenvector."""
+import math
+
+import numpy as np
+import pytest
+
+from mnhack8_amd import fields
+from oracle_lib import OracleOp
+
+# local node offsets N0..N7 (src/heat_solver_het.cu:261-263)
+OFF = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+
+def hamming():
+    return np.array([[int(np.sum(OFF[i] != OFF[j])) for j in range(8)] for i in range(8)])
+
+
+def kron_matrices(h):
+    """M and K of the cube from 1D factors M1 = h/6 [2 1; 1 2], K1 = 1/h [1 -1; -1 1]."""
+    M1 = h / 6 * np.array([[2.0, 1.0], [1.0, 2.0]])
+    K1 = 1 / h * np.array([[1.0, -1.0], [-1.0, 1.0]])
+    M = np.zeros((8, 8))
+    K = np.zeros((8, 8))
+    for i in range(8):
+        for j in range(8):
+            a, b = OFF[i], OFF[j]
+            m = [M1[a[d], b[d]] for d in range(3)]
+            k = [K1[a[d], b[d]] for d in range(3)]
+            M[i, j] = m[0] * m[1] * m[2]
+            K[i, j] = k[0] * m[1] * m[2] + m[0] * k[1] * m[2] + m[0] * m[1] * k[2]
+    return M, K
+
+
+@pytest.mark.parametrize("n", [1, 7, 32])
+def test_reference_matrices_closed_forms(n):
+    h = 1.0 / n
+    op = OracleOp(n, 1e-3, np.ones(n ** 3), np.ones(n ** 3), kernel="fast")
+    M, K = op.ref_matrices()
+    Mk, Kk = kron_matrices(h)
+    d = hamming()
+    assert np.allclose(M, h ** 3 / 216 * (8 >> d), rtol=1e-14, atol=0)
+    assert np.allclose(M, Mk, rtol=1e-14, atol=0)
+    assert np.allclose(K, Kk, rtol=0, atol=1e-14 * h)
+    assert np.allclose(K[d == 0], h / 3, rtol=1e-14)
+    assert np.allclose(K[d == 1], 0.0, atol=1e-15 * h)
+    assert np.allclose(K[(d == 2) | (d == 3)], -h / 12, rtol=1e-14)
+    assert np.allclose(K.sum(axis=1), 0.0, atol=1e-15 * h)
+    assert abs(M.sum() - h ** 3) < 1e-15 * h ** 3
+
+
+def test_struct_rows_equal_matrix_rows():
+    """The closed-form rows (the GPU's arithmetic) against the quadrature 8x8
+    matrices on random elements, coefficients and vectors."""
+    n = 6
+    rng = np.random.default_rng(0)
+    k = np.exp(rng.standard_normal(n ** 3))
+    rhoc = np.exp(0.5 * rng.standard_normal(n ** 3))
+    x = rng.standard_normal((n + 1) ** 3)
+    for dt in (0.0, 1e-3, 0.7):
+        a = OracleOp(n, 0.7, k, rhoc, kernel="struct").apply(x, dt, project=False)
+        b = OracleOp(n, 0.7, k, rhoc, kernel="fast").apply(x, dt, project=False)
+        c = OracleOp(n, 0.7, k, rhoc, kernel="gauss").apply(x, dt, project=False)
+        assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+        assert np.max(np.abs(c - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+def node_grid(n):
+    c = np.arange(n + 1) / n
+    return np.meshgrid(c, c, c, indexing="ij")
+
+
+@pytest.mark.parametrize("kernel", ["struct", "fast", "gauss"])
+@pytest.mark.parametrize("n", [4, 9])
+def test_sine_mode_is_exact_eigenvector(kernel, n):
+    """v = sin(pi x) sin(pi y) sin(pi z): M v = mu_M v and K v = mu_K v on the
+    interior nodes, with mu_M = ((h/3)(2 + cos pi h))^3 and
+    mu_K = 3 (2 - 2 cos pi h)/h ((h/3)(2 + cos pi h))^2."""
+    h = 1.0 / n
+    X, Y, Z = node_grid(n)
+    v = (np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)).ravel()
+    interior = ((X > 0) & (X < 1) & (Y > 0) & (Y < 1) & (Z > 0) & (Z < 1)).ravel()
+    v[~interior] = 0.0
+    m1 = h / 3 * (2 + math.cos(math.pi * h))
+    muM = m1 ** 3
+    muK = 3 * (2 - 2 * math.cos(math.pi * h)) / h * m1 ** 2
+    dt = 0.37
+    op = OracleOp(n, dt, np.ones(n ** 3), np.ones(n ** 3), kernel=kernel)
+    Mv = op.apply(v, 0.0)
+    Av = op.apply(v, dt)
+    assert np.allclose(Mv[interior], muM * v[interior], rtol=0, atol=1e-14 * muM)
+    assert np.allclose(Av[interior], (muM + dt * muK) * v[interior], rtol=0, atol=1e-13 * (muM + dt * muK))
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_symmetry_and_constant_nullspace(scatter):
+    n = 5
+    rng = np.random.default_rng(1)
+    k = np.exp(rng.standard_normal(n ** 3))
+    rhoc = np.exp(rng.standard_normal(n ** 3))
+    op = OracleOp(n, 0.01, k, rhoc, kernel="struct", scatter=scatter)
+    ones = op.vector("diag") * 0 + 1.0
+    # K 1 = 0 without Dirichlet: A 1 = M 1
+    assert np.allclose(op.apply(ones, 0.01, project=False), op.apply(ones, 0.0, project=False), rtol=1e-14)
+    # x^T A y = y^T A x for interior-supported x, y
+    X, Y, Z = node_grid(n)
+    interior = ((X > 0) & (X < 1) & (Y > 0) & (Y < 1) & (Z > 0) & (Z < 1)).ravel()
+    x = rng.standard_normal(op.nn) * interior
+    y = rng.standard_normal(op.nn) * interior
+    a = np.dot(x, op.apply(y, 0.01))
+    b = np.dot(y, op.apply(x, 0.01))
+    assert abs(a - b) <= 1e-13 * abs(a)
+
+
+def test_diag_is_matrix_diagonal():
+    n = 4
+    rng = np.random.default_rng(2)
+    k = np.exp(rng.standard_normal(n ** 3))
+    rhoc = np.exp(rng.standard_normal(n ** 3))
+    dt = 0.05
+    op = OracleOp(n, dt, k, rhoc, kernel="struct")
+    d = op.vector("diag")
+    X, Y, Z = node_grid(n)
+    interior = ((X > 0) & (X < 1) & (Y > 0) & (Y < 1) & (Z > 0) & (Z < 1)).ravel()
+    for i in np.flatnonzero(interior)[:20]:
+        e = np.zeros(op.nn)
+        e[i] = 1.0
+        assert abs(op.apply(e, dt)[i] - d[i]) <= 1e-15 * d[i]
+    assert (d[~interior] == 1.0).all()
+
+
+def ref_order_dot(x, y):
+    """The reference's two-stage order, restated for the check."""
+    n = x.size
+    nb = -(-n // 256)
+    p = np.zeros(nb)
+    prod = np.zeros(nb * 256)
+    prod[:n] = x * y
+    for b in range(nb):
+        s = list(prod[b * 256:(b + 1) * 256])
+        st = 128
+        while st:
+            for t in range(st):
+                s[t] += s[t + st]
+            st //= 2
+        p[b] = s[0]
+    f = [0.0] * 1024
+    for t in range(1024):
+        a = 0.0
+        for i in range(t, nb, 1024):
+            a += p[i]
+        f[t] = a
+    st = 512
+    while st:
+        for t in range(st):
+            f[t] += f[t + st]
+        st //= 2
+    return f[0]
+
+
+@pytest.mark.parametrize("n", [1, 3, 12])
+def test_dot_order_and_accuracy(n):
+    op = OracleOp(n, 1e-3, np.ones(n ** 3), np.ones(n ** 3), kernel="struct")
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(op.nn)
+    y = rng.standard_normal(op.nn)
+    d = op.dot(x, y)
+    exact = math.fsum((x * y).tolist())
+    assert abs(d - exact) <= 1e-14 * np.sum(np.abs(x * y))
+    assert d == ref_order_dot(x, y)
+
+
+def test_cg_one_iteration_on_eigenvector():
+    """b = M v with v the sine mode: backward Euler's exact answer is
+    mu_M / (mu_M + dt mu_K) v and Jacobi-PCG reaches it in one iteration."""
+    n, dt = 8, 1e-3
+    h = 1.0 / n
+    X, Y, Z = node_grid(n)
+    v = (np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)).ravel()
+    op = OracleOp(n, dt, np.ones(n ** 3), np.ones(n ** 3), kernel="struct", verify=True)
+    v0 = op.initial_condition("sine", 0.0, 1.0)
+    b = op.build_rhs(v0)
+    it, xs, hist = op.pcg(b, np.zeros(op.nn), 500, 1e-10)
+    m1 = h / 3 * (2 + math.cos(math.pi * h))
+    muM, muK = m1 ** 3, 3 * (2 - 2 * math.cos(math.pi * h)) / h * m1 ** 2
+    assert it == 1
+    assert np.allclose(xs, muM / (muM + dt * muK) * v0, rtol=0, atol=1e-13)
+    assert hist[1, 0] < 1e-10
+
+
+def test_fp32_operator_close_to_fp64():
+    n = 7
+    rng = np.random.default_rng(3)
+    k = np.exp(rng.standard_normal(n ** 3))
+    op = OracleOp(n, 1e-3, k, np.ones(n ** 3), kernel="struct")
+    x = rng.standard_normal(op.nn)
+    y32 = op.apply_f32(x.astype(np.float32), 1e-3)
+    y64 = op.apply(x.astype(np.float32).astype(np.float64), 1e-3)
+    assert np.max(np.abs(y32 - y64)) <= 1e-6 * np.max(np.abs(y64))
+
+
+def test_field_validation_rejects_nonpositive():
+    n = 2
+    k = np.ones(8)
+    k[3] = 0.0
+    with pytest.raises(RuntimeError, match="non-positive"):
+        OracleOp(n, 1e-3, k, np.ones(8))
+    k[3] = np.nan
+    with pytest.raises(RuntimeError, match="non-positive"):
+        OracleOp(n, 1e-3, k, np.ones(8))
+
+
+def test_quadrants_matches_analytic_model():
+    k, rhoc = fields.quadrants(4)
+    assert k[0, 0, 0] == 100 and k[3, 0, 0] == 200 and k[0, 3, 0] == 300 and k[0, 0, 3] == 400
+    assert rhoc[3, 0, 0] == 0.8 * 1.5 and rhoc[3, 3, 3] == 0.5 * 2.5
