@@ -111,7 +111,7 @@ def main():
     if a.sweep:
         table = {}
         cands = [("gather", 8, 0), ("march", 4, 0), ("march", 8, 0)] + \
-                [("tile", p, sh) for sh in (0, 1, 2) for p in (4, 8, 16)]
+                [(im, p, sh) for im in ("tile", "tdma") for sh in (0, 1, 2) for p in (4, 8, 16)]
         for impl, planes, shape in cands:
             op.set_colored_impl(impl)
             op.set_march_chunk(planes)

@@ -47,7 +47,8 @@ extern "C" {
 #define HEAT_COLORED_GATHER 0   /* one thread per node, contributions summed in colour order */
 #define HEAT_COLORED_SWEEP  1   /* eight element launches, one per colour */
 #define HEAT_COLORED_MARCH  2   /* threads walk x-columns of nodes with a register window */
-#define HEAT_COLORED_TILE   3   /* blocks walk x with LDS-staged plane tiles (default) */
+#define HEAT_COLORED_TILE   3   /* blocks walk x with LDS-staged plane tiles */
+#define HEAT_COLORED_TDMA   4   /* as TILE, tiles copied to LDS asynchronously two planes ahead (default) */
 
 /* handle-owned vectors for heat_vector() */
 #define HEAT_VEC_T    0

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace heat {
 
 constexpr int NT = 256;   // node-kernel block (threads_node, src/heat_solver_het.cu:653)
@@ -550,6 +552,206 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
 #pragma unroll
             for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
     }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled plane walk with asynchronous copies (k_apply_tile's sums again).
+//
+// Same tiles, window and arithmetic as k_apply_tile, but the block's x-plane
+// and element-plane tiles reach LDS through global_load_lds (the hardware
+// copies global memory straight into LDS; no register holds the data).  The
+// tiles of two consecutive planes form a batch; while a batch is computed,
+// the copies of the next batch are in flight into the other half of a
+// two-half LDS ring.  Ordering needs only "all of this wave's copies done"
+// (s_waitcnt 0) and one barrier per batch.
+// ---------------------------------------------------------------------------
+template <typename T, int TY, int TZ>
+struct TdmaGeom {
+    static constexpr int NTH = TY * TZ;
+    static constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
+    static constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
+    static constexpr int WPV = (int)(sizeof(T) / 4);          // dwords per value
+    static constexpr int XJ = (XN * WPV + NTH - 1) / NTH;     // copies per thread per x tile
+    static constexpr int EJ = (EN * WPV + NTH - 1) / NTH;     // copies per thread per element tile
+};
+
+template <typename T, bool MASS, int TY, int TZ>
+struct TdmaHalf {
+    T xs[2][TdmaGeom<T, TY, TZ>::XN];
+    T er[2][TdmaGeom<T, TY, TZ>::EN];
+    T ek[2][MASS ? 1 : TdmaGeom<T, TY, TZ>::EN];
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <typename T, bool MASS, int TY, int TZ, int CX>
+__global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
+                                                       const T* __restrict__ cr,
+                                                       const T* __restrict__ ck, Grid g, FiArm fi,
+                                                       int nby, int nbz, int nblocks,
+                                                       const DevState* st, int check_done) {
+    using G = TdmaGeom<T, TY, TZ>;
+    constexpr int NTH = G::NTH, XW = G::XW, XN = G::XN, EW = G::EW, EN = G::EN;
+    constexpr int WPV = G::WPV, XJ = G::XJ, EJ = G::EJ;
+    __shared__ TdmaHalf<T, MASS, TY, TZ> H0, H1;
+    if (check_done && st->done) return;
+    const int b = xcd_slab_block(nblocks);
+    if (b >= nblocks) return;
+    const int bz = b % nbz;
+    const int by = (b / nbz) % nby;
+    const int chunk = b / (nbz * nby);
+    const int n = g.n, S = g.sx, nn1 = g.nn1, nsq = n * n;
+    const int tid = threadIdx.x;
+    const int lane_base = tid & ~63;                     // first thread of this wave
+    const int ty = tid / TZ, tz = tid - (tid / TZ) * TZ;
+    const int y0 = 1 + by * TY, z0 = 1 + bz * TZ;
+    const int iy = y0 + ty, iz = z0 + tz;
+    const bool inmesh = (iy <= n) && (iz <= n);
+    const bool interior = (iy < n) && (iz < n);
+    const int ix0 = chunk * CX;
+    const int ix1 = min(ix0 + CX, n + 1);
+    const int py = iy & 1, pz = iz & 1;
+    const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
+    // byte offsets (within a plane) of this thread's copy slots; -1 = no copy
+    int xo[XJ], eo[EJ];
+#pragma unroll
+    for (int j = 0; j < XJ; ++j) {
+        const int d = j * NTH + tid;
+        const int i = d / WPV, half = d - i * WPV;
+        const int r = i / XW, c = i - r * XW;
+        const int yy = y0 - 1 + r, zz = z0 - 1 + c;
+        xo[j] = (i < XN && yy <= n && zz <= n) ? (int)((yy * nn1 + zz) * sizeof(T)) + 4 * half : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < EJ; ++j) {
+        const int d = j * NTH + tid;
+        const int i = d / WPV, half = d - i * WPV;
+        const int r = i / EW, c = i - r * EW;
+        const int ey = y0 - 1 + r, ez = z0 - 1 + c;
+        eo[j] = (i < EN && ey < n && ez < n) ? (int)((ey * n + ez) * sizeof(T)) + 4 * half : -1;
+    }
+    // copies of the batch starting at node plane ixb into half Hh:
+    // x planes ixb+1, ixb+2 and element planes ixb, ixb+1
+    auto issue = [&](TdmaHalf<T, MASS, TY, TZ>& Hh, int ixb) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int xp = ixb + 1 + p, ep = ixb + p;
+            if (xp <= n) {
+                const char* base = reinterpret_cast<const char*>(x + (size_t)xp * S);
+#pragma unroll
+                for (int j = 0; j < XJ; ++j)
+                    if (xo[j] >= 0)
+                        __builtin_amdgcn_global_load_lds(base + xo[j],
+                                                         (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 4 * (j * NTH + lane_base)),
+                                                         4, 0, 0);
+            }
+            if (ep < n) {
+                const char* bcr = reinterpret_cast<const char*>(cr + (size_t)ep * nsq);
+                const char* bck = reinterpret_cast<const char*>(ck + (size_t)ep * nsq);
+#pragma unroll
+                for (int j = 0; j < EJ; ++j)
+                    if (eo[j] >= 0) {
+                        __builtin_amdgcn_global_load_lds(bcr + eo[j],
+                                                         (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 4 * (j * NTH + lane_base)),
+                                                         4, 0, 0);
+                        if (!MASS)
+                            __builtin_amdgcn_global_load_lds(bck + eo[j],
+                                                             (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 4 * (j * NTH + lane_base)),
+                                                             4, 0, 0);
+                    }
+            }
+        }
+    };
+    int eidx[2][2], eoff[2][2];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EW + (tz + 1 - (cz ^ pz));
+            eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
+        }
+    int xidx[3][3];
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
+    T W[3][3][3];
+    Coef<T> Eprev[2][2], Enew[2][2];
+    if (interior) {
+        const size_t q = (size_t)iy * nn1 + iz;
+        if (ix0 >= 1) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    W[0][bb][c] = x[(size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    const size_t e = (size_t)(ix0 - 1) * nsq + eoff[cy][cz];
+                    Eprev[cy][cz].cr = cr[e];
+                    Eprev[cy][cz].ck = MASS ? T(0) : ck[e];
+                }
+        }
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+    }
+    const int par_yz = (py << 1) | pz;
+    // one node plane: p = position in the batch (ix parity, since batches start even)
+    auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto pc, int ix) {
+        constexpr int p = decltype(pc)::value;
+        if (interior && ix <= n - 1) {
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) W[2][bb][c] = Hh.xs[p][xidx[bb][c]];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    Enew[cy][cz].cr = Hh.er[p][eidx[cy][cz]];
+                    Enew[cy][cz].ck = MASS ? T(0) : Hh.ek[p][eidx[cy][cz]];
+                }
+        }
+        if (inmesh) {
+            T acc = T(0);
+            if (interior && ix >= 1 && ix <= n - 1)
+                acc = march_node<T, MASS, p>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+            const size_t base = (size_t)ix * S;
+            y[base + (size_t)iy * nn1 + iz] = acc;
+            if (by == 0 && ty == 0) y[base + iz] = T(0);
+            if (bz == 0 && tz == 0) y[base + (size_t)iy * nn1] = T(0);
+            if (by == 0 && bz == 0 && tid == 0) y[base] = T(0);
+        }
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                W[0][bb][c] = W[1][bb][c];
+                W[1][bb][c] = W[2][bb][c];
+            }
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
+    };
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    issue(H0, ix0);
+    int h = 0;
+    for (int ixb = ix0; ixb < ix1; ixb += 2, h ^= 1) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        TdmaHalf<T, MASS, TY, TZ>& cur = h ? H1 : H0;
+        TdmaHalf<T, MASS, TY, TZ>& nxt = h ? H0 : H1;
+        if (ixb + 2 < ix1) issue(nxt, ixb + 2);
+        step(cur, P0{}, ixb);
+        if (ixb + 1 < ix1) step(cur, P1{}, ixb + 1);
+    }
+    __builtin_amdgcn_s_waitcnt(0);     // no copy outlives the block
 }
 
 // ---------------------------------------------------------------------------

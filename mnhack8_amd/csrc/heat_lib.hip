@@ -53,7 +53,7 @@ struct heat_op {
     heat_config cfg;
     Grid g;
     hipStream_t stream = nullptr;
-    int colored_impl = HEAT_COLORED_TILE;
+    int colored_impl = HEAT_COLORED_TDMA;
     int march_cx = 8;
     int tile_shape = 0;
     int pcg_chunk = 8;
@@ -160,10 +160,18 @@ void tile_launch1(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool 
     const int nby = (g.n + TY - 1) / TY, nbz = (g.n + TZ - 1) / TZ, nch = (g.nn1 + CX - 1) / CX;
     const int nblocks = nby * nbz * nch;
     const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
+    hipStream_t s = op->stream;
+    if (op->colored_impl == HEAT_COLORED_TDMA) {
+        if (mass)
+            k_apply_tdma<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+        else
+            k_apply_tdma<T, false, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+        return;
+    }
     if (mass)
-        k_apply_tile<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, op->stream>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+        k_apply_tile<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
     else
-        k_apply_tile<T, false, TY, TZ, CX><<<grid, TY * TZ, 0, op->stream>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
+        k_apply_tile<T, false, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
 }
 
 template <typename T, int TY, int TZ>
@@ -202,7 +210,8 @@ int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool
         CHECK_LAUNCH();
         return HEAT_OK;
     }
-    if (op->colored_impl == HEAT_COLORED_TILE) return launch_tile<T>(op, x, y, cr, ck, mass, f, check_done);
+    if (op->colored_impl == HEAT_COLORED_TILE || op->colored_impl == HEAT_COLORED_TDMA)
+        return launch_tile<T>(op, x, y, cr, ck, mass, f, check_done);
     const int nqb = (g.sx + NT - 1) / NT;
     const int cx = op->march_cx;
     const int nchunks = (g.nn1 + cx - 1) / cx;
@@ -640,7 +649,7 @@ int heat_set_march_chunk(heat_op* op, int planes) {
 }
 
 int heat_set_colored_impl(heat_op* op, int impl) {
-    if (!op || impl < HEAT_COLORED_GATHER || impl > HEAT_COLORED_TILE) return fail(HEAT_E_ARG, "bad impl");
+    if (!op || impl < HEAT_COLORED_GATHER || impl > HEAT_COLORED_TDMA) return fail(HEAT_E_ARG, "bad impl");
     op->colored_impl = impl;
     return HEAT_OK;
 }
@@ -965,7 +974,10 @@ int heat_autotune(heat_op* op, int reps, int* impl, int* planes, int* shape, dou
     const Cand cands[] = {{HEAT_COLORED_GATHER, 8, 0}, {HEAT_COLORED_MARCH, 4, 0}, {HEAT_COLORED_MARCH, 8, 0},
                           {HEAT_COLORED_TILE, 4, 0},   {HEAT_COLORED_TILE, 8, 0},  {HEAT_COLORED_TILE, 16, 0},
                           {HEAT_COLORED_TILE, 4, 1},   {HEAT_COLORED_TILE, 8, 1},  {HEAT_COLORED_TILE, 16, 1},
-                          {HEAT_COLORED_TILE, 4, 2},   {HEAT_COLORED_TILE, 8, 2},  {HEAT_COLORED_TILE, 16, 2}};
+                          {HEAT_COLORED_TILE, 4, 2},   {HEAT_COLORED_TILE, 8, 2},  {HEAT_COLORED_TILE, 16, 2},
+                          {HEAT_COLORED_TDMA, 4, 0},   {HEAT_COLORED_TDMA, 8, 0},  {HEAT_COLORED_TDMA, 16, 0},
+                          {HEAT_COLORED_TDMA, 4, 1},   {HEAT_COLORED_TDMA, 8, 1},  {HEAT_COLORED_TDMA, 16, 1},
+                          {HEAT_COLORED_TDMA, 4, 2},   {HEAT_COLORED_TDMA, 8, 2},  {HEAT_COLORED_TDMA, 16, 2}};
     double* x = op->vec[HEAT_VEC_P];
     double* y = op->vec[HEAT_VEC_AP];
     double best = 1e300;

@@ -25,7 +25,7 @@ DRIVER_PATH = os.path.join(_HERE, "_build", "heat_solver_het")
 KERNEL = {"gauss": 0, "fast": 1}
 SCATTER = {"atomic": 0, "colored": 1}
 SOURCE = {"gaussian": 0, "const": 1}
-COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3}
+COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3, "tdma": 4}
 TILE_SHAPES = {0: (8, 32), 1: (4, 64), 2: (16, 16)}
 VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "bad field", 4: "self-check failed", 5: "bad state"}

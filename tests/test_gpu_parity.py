@@ -68,14 +68,14 @@ def test_apply_colored_bitwise(n, dt_eff):
         ym = dev(np.full(op.nn, np.nan))
         op.apply_operator(xd, ym, dt_eff, project=True)
         assert np.array_equal(host(op, ym), y_gather), f"march({planes}) != gather"
-    for shape in (0, 1, 2):
-        for planes in (4, 8, 16):
-            op.set_colored_impl("tile")
+    for shape, planes, impl in [(sh, pl, im) for sh in (0, 1, 2) for pl in (4, 8, 16) for im in ("tile", "tdma")]:
+        if True:
+            op.set_colored_impl(impl)
             op.set_tile_shape(shape)
             op.set_march_chunk(planes)
             ym = dev(np.full(op.nn, np.nan))
             op.apply_operator(xd, ym, dt_eff, project=True)
-            assert np.array_equal(host(op, ym), y_gather), f"tile({shape},{planes}) != gather"
+            assert np.array_equal(host(op, ym), y_gather), f"{impl}({shape},{planes}) != gather"
     op.set_colored_impl("sweep")
     yd2 = dev(np.zeros(op.nn))
     op.apply_operator(xd, yd2, dt_eff, project=True)
@@ -213,7 +213,7 @@ def test_pcg_edge_cases():
     assert op.pcg_solve("b", "T", 0, 1e-8) == 0
 
 
-@pytest.mark.parametrize("impl", ["tile", "march", "gather"])
+@pytest.mark.parametrize("impl", ["tdma", "tile", "march", "gather"])
 @pytest.mark.parametrize("bit", [30, 55, 62, 63])
 def test_fault_level_R_and_G(bit, impl):
     n, dt = 8, 5e-4
@@ -333,7 +333,7 @@ def test_autotune_keeps_bits():
     x = np.random.default_rng(4).standard_normal(op.nn)
     op.upload("p", x)
     impl, planes, shape, ms = op.autotune(5)
-    assert impl in ("gather", "march", "tile") and ms > 0
+    assert impl in ("gather", "march", "tile", "tdma") and ms > 0
     op.upload("p", x)
     op.apply_operator("p", "Ap", dt)
     assert np.array_equal(op.download("Ap"), orc.apply(x, dt))
