@@ -174,13 +174,16 @@ def main():
         op.pcg_solve("b", "T", 5000, 1e-13)
         xref = op.download("T")
         res = {}
-        for label in ("fp64", "fp32_refined"):
+        runs = (("fp64", "fused"), ("fp32_refined", "fused"),
+                ("fp64_reference_order", "reference"), ("fp32_refined_reference_order", "reference"))
+        for label, mode in runs:
+            op.set_pcg_mode(mode)
             best = None
             for rep in range(3):
                 op.upload("T", np.zeros(nn))
                 op.synchronize()
                 t0 = time.perf_counter()
-                if label == "fp64":
+                if label.startswith("fp64"):
                     its = op.pcg_solve("b", "T", 5000, 1e-8)
                     extra = {"iterations": its}
                 else:
@@ -194,7 +197,9 @@ def main():
             err = float(np.linalg.norm(xs - xref) / np.linalg.norm(xref))
             res[label] = {"ms": round(best, 3), "rel_err_vs_fp64_1e-13": err, **extra}
             log(f"time-to-1e-8 {label}: {best:.2f} ms {extra} err {err:.2e}")
+        op.set_pcg_mode("fused")
         res["speedup_fp32_refined"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
+        res["speedup_fused_vs_reference_order"] = round(res["fp64_reference_order"]["ms"] / res["fp64"]["ms"], 3)
         result["time_to_1e-8"] = res
 
     if not a.no_cpu:

@@ -50,6 +50,11 @@ extern "C" {
 #define HEAT_COLORED_TILE   3   /* blocks walk x with LDS-staged plane tiles */
 #define HEAT_COLORED_TDMA   4   /* as TILE, tiles copied to LDS asynchronously two planes ahead (default) */
 
+/* PCG execution (both run the reference's algorithm; they differ in the fixed
+ * order in which inner products are summed) */
+#define HEAT_PCG_FUSED           0  /* two launches per iteration, tile/super-block order (default) */
+#define HEAT_PCG_REFERENCE_ORDER 1  /* six launches per iteration, the reference's 256/1024 order */
+
 /* handle-owned vectors for heat_vector() */
 #define HEAT_VEC_T    0
 #define HEAT_VEC_B    1
@@ -161,6 +166,8 @@ int heat_flip_bit(heat_op* op, double* v, int64_t idx, int bit);
 /* D1 state accumulated since the last call (abft_step_max, abft_step_calls,
  * src/heat_solver_het.cu:642-643, 1204, 1233); reading resets it. */
 int heat_abft_take(heat_op* op, double* step_max, int64_t* calls);
+/* PCG execution mode (HEAT_PCG_*), new. */
+int heat_set_pcg_mode(heat_op* op, int mode);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);

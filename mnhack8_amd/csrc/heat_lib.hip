@@ -18,6 +18,7 @@
 
 #include "heat.h"
 #include "heat_kernels.h"
+#include "heat_pcg_fused.h"
 
 using namespace heat;
 
@@ -80,14 +81,22 @@ struct heat_op {
     int hist_cap = 0;
     double* cur_hist = nullptr;  // history rows written by the running solve, or null
     // FP32 inner-solve vectors
-    float *e32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *ap32 = nullptr, *d32 = nullptr;
+    float *e32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *ap32 = nullptr, *d32 = nullptr;
+    double* p2 = nullptr;       // second search-direction buffer of the fused PCG
     // level R arming (host copy, passed by value)
     FiArm fi{0, 0, 0};
     int fi_level = 0;
+    // fused PCG (two launches per iteration)
+    int pcg_mode = HEAT_PCG_FUSED;
+    FusedState* fst = nullptr;
+    FusedState* hfst = nullptr;   // pinned mirror
+    double* fsp = nullptr;        // super-block partials, 5 x nsb
+    double* ftp = nullptr;        // fused-apply tile partials, 4 x ntiles
+    int nsb = 0, fnby = 0, fnbz = 0, fntiles = 0;
     // captured PCG chunk
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    const void* graph_key[4] = {nullptr, nullptr, nullptr, nullptr};
+    const void* graph_key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -335,6 +344,12 @@ int read_state(heat_op* op) {
     return HEAT_OK;
 }
 
+int read_fused(heat_op* op) {
+    HIP_TRY(hipMemcpyAsync(op->hfst, op->fst, sizeof(FusedState), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(hipStreamSynchronize(op->stream));
+    return HEAT_OK;
+}
+
 __global__ void k_pcg_params(DevState* st, double tol, int max_iter) {
     st->tol = tol;
     st->max_iter = max_iter;
@@ -374,50 +389,100 @@ int launch_iteration(heat_op* op, double* x, bool detect) {
     return HEAT_OK;
 }
 
-int launch_chunk(heat_op* op, double* x, bool detect) {
-    const void* key[4] = {x, (const void*)(intptr_t)(detect ? 1 : 0),
-                          (const void*)(intptr_t)op->pcg_chunk, op->cur_hist};
+__global__ void k_fused_params(FusedState* fs, double tol, int max_iter, int check_outer,
+                               double tol_outer) {
+    fs->tol = tol;
+    fs->max_iter = max_iter;
+    fs->check_outer = check_outer;
+    fs->tol_outer = tol_outer;
+    fs->inner_total = 0;
+    fs->outer_done = 0;
+    fs->done = 0;
+}
+
+// one fused iteration (K1 + K2) on vectors of type T
+template <typename T>
+int launch_fused_iteration(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, const T* d, const T* cr,
+                           const T* ck, bool detect, double* hist) {
+    hipStream_t s = op->stream;
+    const unsigned grid1 = (unsigned)(((op->fntiles + NXCD - 1) / NXCD) * NXCD);
+    const double* w = op->vec[HEAT_VEC_WA];
+    if (detect)
+        k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(p0, p1, z, Ap, cr, ck, op->g, w, op->fsp, op->nsb, op->ftp,
+                                                          op->fnby, op->fnbz, op->fntiles, op->fst, hist);
+    else
+        k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(p0, p1, z, Ap, cr, ck, op->g, w, op->fsp, op->nsb, op->ftp,
+                                                           op->fnby, op->fnbz, op->fntiles, op->fst, hist);
+    CHECK_LAUNCH();
+    if (detect)
+        k_fused_update<T, true><<<op->nsb, NT, 0, s>>>(x, p0, p1, r, Ap, d, z, op->g.nn, op->ftp, op->fntiles, op->fsp,
+                                                      op->nsb, op->fst, op->st, op->nwA);
+    else
+        k_fused_update<T, false><<<op->nsb, NT, 0, s>>>(x, p0, p1, r, Ap, d, z, op->g.nn, op->ftp, op->fntiles, op->fsp,
+                                                       op->nsb, op->fst, op->st, op->nwA);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+// Launch `pcg_chunk` iterations of `body`, captured once as a hipGraph per key.
+template <typename Body>
+int launch_chunk_generic(heat_op* op, const void* const key[6], Body body) {
     if (op->use_graph) {
-        if (!op->gexec || memcmp(key, op->graph_key, sizeof key) != 0) {
+        if (!op->gexec || memcmp(key, op->graph_key, 6 * sizeof(void*)) != 0) {
             if (op->gexec) { (void)hipGraphExecDestroy(op->gexec); op->gexec = nullptr; }
             if (op->graph) { (void)hipGraphDestroy(op->graph); op->graph = nullptr; }
             HIP_TRY(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
             int rc = HEAT_OK;
-            for (int i = 0; i < op->pcg_chunk && rc == HEAT_OK; ++i) rc = launch_iteration(op, x, detect);
+            for (int i = 0; i < op->pcg_chunk && rc == HEAT_OK; ++i) rc = body();
             hipGraph_t graph = nullptr;
             hipError_t ec = hipStreamEndCapture(op->stream, &graph);
             if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
             if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
             op->graph = graph;
             HIP_TRY(hipGraphInstantiate(&op->gexec, op->graph, nullptr, nullptr, 0));
-            memcpy(op->graph_key, key, sizeof key);
+            memcpy(op->graph_key, key, 6 * sizeof(void*));
         }
         HIP_TRY(hipGraphLaunch(op->gexec, op->stream));
         return HEAT_OK;
     }
     for (int i = 0; i < op->pcg_chunk; ++i) {
-        int rc = launch_iteration(op, x, detect);
+        int rc = body();
         if (rc) return rc;
     }
     return HEAT_OK;
 }
 
+int launch_chunk(heat_op* op, double* x, bool detect) {
+    const void* key6[6] = {x, (const void*)(intptr_t)(detect ? 1 : 0), (const void*)(intptr_t)op->pcg_chunk,
+                           op->cur_hist, (const void*)(intptr_t)(op->pcg_mode + 10 * op->colored_impl),
+                           (const void*)(intptr_t)(op->march_cx + 100 * op->tile_shape)};
+    if (op->pcg_mode == HEAT_PCG_FUSED) {
+        return launch_chunk_generic(op, key6, [&]() {
+            return launch_fused_iteration<double>(op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R],
+                                                  op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
+                                                  op->vec[HEAT_VEC_DIAG], op->cr, op->ck, detect, op->cur_hist);
+        });
+    }
+    return launch_chunk_generic(op, key6, [&]() { return launch_iteration(op, x, detect); });
+}
+
 // Launch chunks until the device reports done; the flags of chunk k are read
 // while chunk k+1 runs.
 template <typename LaunchFn>
-int run_until_done(heat_op* op, int max_iter, LaunchFn launch) {
+int run_until_done(heat_op* op, int max_iter, LaunchFn launch, const int* dflag = nullptr) {
+    if (!dflag) dflag = &op->st->done;
     const int max_chunks = max_iter / (op->pcg_chunk > 0 ? op->pcg_chunk : 1) + 3;
     int k = 0;
     int rc = launch();
     if (rc) return rc;
     int* slot[2] = {&op->hflag[0], &op->hflag[1]};   // pinned mailboxes, one per chunk in flight
-    HIP_TRY(hipMemcpyAsync(slot[0], &op->st->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+    HIP_TRY(hipMemcpyAsync(slot[0], dflag, sizeof(int), hipMemcpyDeviceToHost, op->stream));
     HIP_TRY(hipEventRecord(op->ev[0], op->stream));
     for (;;) {
         if (++k > max_chunks) return fail(HEAT_E_STATE, "PCG did not report completion");
         rc = launch();
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(slot[k & 1], &op->st->done, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+        HIP_TRY(hipMemcpyAsync(slot[k & 1], dflag, sizeof(int), hipMemcpyDeviceToHost, op->stream));
         HIP_TRY(hipEventRecord(op->ev[k & 1], op->stream));
         HIP_TRY(hipEventSynchronize(op->ev[(k - 1) & 1]));
         if (*slot[(k - 1) & 1]) break;
@@ -434,9 +499,11 @@ void free_all(heat_op* op) {
     (void)hipFree(op->cr32); (void)hipFree(op->ck32);
     for (auto& v : op->vec) (void)hipFree(v);
     (void)hipFree(op->partials); (void)hipFree(op->st); (void)hipFree(op->dhist);
-    (void)hipFree(op->e32); (void)hipFree(op->r32); (void)hipFree(op->z32); (void)hipFree(op->p32); (void)hipFree(op->ap32); (void)hipFree(op->d32);
+    (void)hipFree(op->e32); (void)hipFree(op->r32); (void)hipFree(op->z32); (void)hipFree(op->p32); (void)hipFree(op->p32b); (void)hipFree(op->p2); (void)hipFree(op->ap32); (void)hipFree(op->d32);
     if (op->hst) (void)hipHostFree(op->hst);
     if (op->hflag) (void)hipHostFree(op->hflag);
+    if (op->hfst) (void)hipHostFree(op->hfst);
+    (void)hipFree(op->fst); (void)hipFree(op->fsp); (void)hipFree(op->ftp);
     if (op->stream) (void)hipStreamDestroy(op->stream);
 }
 
@@ -468,12 +535,23 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     HIP_TRY(hipMalloc(&op->ck, eb8));
     for (int v = HEAT_VEC_T; v <= HEAT_VEC_F; ++v) HIP_TRY(hipMalloc(&op->vec[v], nb8));
     HIP_TRY(hipMalloc(&op->partials, sizeof(double) * MAXSUM * (size_t)op->nbn));
+    op->nsb = (int)((g.nn + SB - 1) / SB);
+    op->fnby = (n + FTY - 1) / FTY;
+    op->fnbz = (n + FTZ - 1) / FTZ;
+    op->fntiles = op->fnby * op->fnbz * ((g.nn1 + FCX - 1) / FCX);
+    HIP_TRY(hipMalloc(&op->fsp, sizeof(double) * 5 * (size_t)op->nsb));
+    HIP_TRY(hipMalloc(&op->p2, sizeof(double) * (size_t)g.nn));
+    HIP_TRY(hipMalloc(&op->ftp, sizeof(double) * 4 * (size_t)op->fntiles));
+    HIP_TRY(hipMalloc(&op->fst, sizeof(FusedState)));
+    HIP_TRY(hipHostMalloc(&op->hfst, sizeof(FusedState), hipHostMallocDefault));
+    memset(op->hfst, 0, sizeof(FusedState));
     HIP_TRY(hipMalloc(&op->st, sizeof(DevState)));
     HIP_TRY(hipHostMalloc(&op->hst, sizeof(DevState), hipHostMallocDefault));
     memset(op->hst, 0, sizeof(DevState));
     HIP_TRY(hipHostMalloc(&op->hflag, 2 * sizeof(int), hipHostMallocDefault));
     hipStream_t s = op->stream;
     HIP_TRY(hipMemcpyAsync(op->st, op->hst, sizeof(DevState), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(op->fst, op->hfst, sizeof(FusedState), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(op->kcell, k, eb8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(op->rhoc, rhoc, eb8, hipMemcpyHostToDevice, s));
     k_coef64<<<blocks(g.ne, 256), 256, 0, s>>>(op->kcell, op->rhoc, cfg->dt, op->m3, op->k2, op->cr, op->ck, g.ne);
@@ -549,6 +627,7 @@ int ensure_mixed(heat_op* op) {
     HIP_TRY(hipMalloc(&op->r32, nb4));
     HIP_TRY(hipMalloc(&op->z32, nb4));
     HIP_TRY(hipMalloc(&op->p32, nb4));
+    HIP_TRY(hipMalloc(&op->p32b, nb4));
     HIP_TRY(hipMalloc(&op->ap32, nb4));
     HIP_TRY(hipMalloc(&op->d32, nb4));
     k_coef32<<<blocks(g.ne, 256), 256, 0, s>>>(op->kcell, op->rhoc, op->cfg.dt, op->m3, op->k2, op->cr32, op->ck32, g.ne);
@@ -769,11 +848,33 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
     }
     op->cur_hist = hist ? op->dhist : nullptr;
     const bool detect = op->cfg.detect != 0;
+    double* Ap = op->vec[HEAT_VEC_AP];
+    int rc = HEAT_OK;
+    if (gather_path(op) && op->pcg_mode == HEAT_PCG_FUSED) {
+        k_fused_params<<<1, 1, 0, s>>>(op->fst, tol, max_iter, 0, 0.0);
+        CHECK_LAUNCH();
+        // r = b - A x0, level R firing here if armed (src/heat_solver_het.cu:999)
+        rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
+        if (rc) return rc;
+        k_fused_init<double><<<op->nsb, NT, 0, s>>>(b, Ap, op->vec[HEAT_VEC_DIAG], op->vec[HEAT_VEC_R],
+                                                     op->vec[HEAT_VEC_Z], op->g.nn, op->fsp, op->nsb, op->fst);
+        CHECK_LAUNCH();
+        rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); }, &op->fst->done);
+        if (rc) return rc;
+        rc = read_fused(op);
+        if (rc) return rc;
+        if (iters) *iters = op->hfst->iter;
+        if (hist) {
+            const int rows = op->hfst->iter + 1;
+            for (int i = 0; i < 3 * (max_iter + 1); ++i) hist[i] = std::nan("");
+            HIP_TRY(hipMemcpy(hist, op->dhist, sizeof(double) * 3 * rows, hipMemcpyDeviceToHost));
+        }
+        return HEAT_OK;
+    }
     k_pcg_params<<<1, 1, 0, s>>>(op->st, tol, max_iter);
     CHECK_LAUNCH();
-    double* Ap = op->vec[HEAT_VEC_AP];
     // r = b - A x0, with level R firing here if armed (src/heat_solver_het.cu:999)
-    int rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
+    rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
     if (rc) return rc;
     k_pcg_init<<<op->nbn, NT, 0, s>>>(b, Ap, op->vec[HEAT_VEC_DIAG], op->vec[HEAT_VEC_R],
                                        op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_P], op->g.nn,
@@ -822,46 +923,83 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     if (rc) return rc;
     hipStream_t s = op->stream;
     const Grid& g = op->g;
-    k_mixed_params<<<1, 1, 0, s>>>(op->st, tol, tol_inner, max_inner);
+    if (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER) {
+        k_mixed_params<<<1, 1, 0, s>>>(op->st, tol, tol_inner, max_inner);
+        CHECK_LAUNCH();
+        FiArm none{0, 0, 0};
+        int nout = 0;
+        for (;;) {
+            k_mixed_residual<<<op->nbn, NT, 0, s>>>(x, b, op->cr, op->ck, g, op->e32, op->r32, op->z32,
+                                                     op->p32, op->d32, op->partials, op->st);
+            CHECK_LAUNCH();
+            k_fin_mixed_resid<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
+            CHECK_LAUNCH();
+            auto launch = [&]() -> int {
+                for (int i = 0; i < op->pcg_chunk; ++i) {
+                    int rci = launch_colored<float>(op, op->p32, op->ap32, op->cr32, op->ck32, false, none, 1);
+                    if (rci) return rci;
+                    k_apply_post<float, EPI_PCG><<<op->nbn, NT, 0, s>>>(op->p32, op->ap32, g, 0, nullptr, 0.0,
+                                                                       op->partials, op->st, 1);
+                    CHECK_LAUNCH();
+                    k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1);
+                    CHECK_LAUNCH();
+                    k_mixed_update<<<op->nbn, NT, 0, s>>>(op->e32, op->p32, op->r32, op->ap32, op->d32,
+                                                           op->z32, g.nn, op->partials, op->st);
+                    CHECK_LAUNCH();
+                    k_fin_mixed_update<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
+                    CHECK_LAUNCH();
+                    k_mixed_dir<<<op->nbn, NT, 0, s>>>(op->p32, op->z32, g.nn, op->st);
+                    CHECK_LAUNCH();
+                }
+                return HEAT_OK;
+            };
+            rc = run_until_done(op, max_inner, launch);
+            if (rc) return rc;
+            rc = read_state(op);
+            if (rc) return rc;
+            if (op->hst->outer_done || nout >= max_outer) break;
+            k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
+            CHECK_LAUNCH();
+            ++nout;
+        }
+        if (outer) *outer = nout;
+        if (inner_iters) *inner_iters = op->hst->inner_total;
+        return HEAT_OK;
+    }
+    k_fused_params<<<1, 1, 0, s>>>(op->fst, tol_inner, max_inner, 1, tol);
     CHECK_LAUNCH();
-    FiArm none{0, 0, 0};
     int nout = 0;
+    const void* key6[6] = {op->e32, (const void*)(intptr_t)7, (const void*)(intptr_t)op->pcg_chunk, nullptr,
+                           (const void*)(intptr_t)(100 + op->colored_impl), nullptr};
     for (;;) {
-        k_mixed_residual<<<op->nbn, NT, 0, s>>>(x, b, op->cr, op->ck, g, op->e32, op->r32, op->z32,
-                                                 op->p32, op->d32, op->partials, op->st);
-        CHECK_LAUNCH();
-        k_fin_mixed_resid<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
-        CHECK_LAUNCH();
-        auto launch = [&]() -> int {
-            for (int i = 0; i < op->pcg_chunk; ++i) {
-                int rci = launch_colored<float>(op, op->p32, op->ap32, op->cr32, op->ck32, false, none, 1);
-                if (rci) return rci;
-                k_apply_post<float, EPI_PCG><<<op->nbn, NT, 0, s>>>(op->p32, op->ap32, g, 0, nullptr, 0.0,
-                                                                   op->partials, op->st, 1);
-                CHECK_LAUNCH();
-                k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1);
-                CHECK_LAUNCH();
-                k_mixed_update<<<op->nbn, NT, 0, s>>>(op->e32, op->p32, op->r32, op->ap32, op->d32,
-                                                       op->z32, g.nn, op->partials, op->st);
-                CHECK_LAUNCH();
-                k_fin_mixed_update<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st);
-                CHECK_LAUNCH();
-                k_mixed_dir<<<op->nbn, NT, 0, s>>>(op->p32, op->z32, g.nn, op->st);
-                CHECK_LAUNCH();
-            }
-            return HEAT_OK;
-        };
-        rc = run_until_done(op, max_inner, launch);
+        double* Ax = op->vec[HEAT_VEC_AP];
+        rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
         if (rc) return rc;
-        rc = read_state(op);
+        k_fused_mixed_init<<<op->nsb, NT, 0, s>>>(b, Ax, op->d32, op->e32, op->r32, op->z32, g.nn, op->fsp,
+                                                   op->nsb, op->fst);
+        CHECK_LAUNCH();
+        rc = run_until_done(op, max_inner, [&]() {
+            return launch_chunk_generic(op, key6, [&]() {
+                return launch_fused_iteration<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32, op->d32,
+                                                     op->cr32, op->ck32, false, nullptr);
+            });
+        }, &op->fst->done);
         if (rc) return rc;
-        if (op->hst->outer_done || nout >= max_outer) break;
+        rc = read_fused(op);
+        if (rc) return rc;
+        if (op->hfst->outer_done || nout >= max_outer) break;
         k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
         CHECK_LAUNCH();
         ++nout;
     }
     if (outer) *outer = nout;
-    if (inner_iters) *inner_iters = op->hst->inner_total;
+    if (inner_iters) *inner_iters = op->hfst->inner_total;
+    return HEAT_OK;
+}
+
+int heat_set_pcg_mode(heat_op* op, int mode) {
+    if (!op || (mode != HEAT_PCG_FUSED && mode != HEAT_PCG_REFERENCE_ORDER)) return fail(HEAT_E_ARG, "bad PCG mode");
+    op->pcg_mode = mode;
     return HEAT_OK;
 }
 

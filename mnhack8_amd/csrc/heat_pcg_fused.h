@@ -1,0 +1,482 @@
+// heat_pcg_fused.h -- Jacobi-PCG in two launches per iteration.
+//
+// The reference's iteration (src/heat_solver_het.cu:1009-1036) runs matvec,
+// p.Ap, two axpys, ||r||, z = r/d, r.z and p = z + beta p with a host round
+// trip per inner product.  Here one iteration is:
+//   K1 k_fused_apply : beta and convergence from K2's partials, p = z + beta p
+//                      (fused into the tile fetch), Ap = P A p (the LDS-tiled
+//                      colour-ordered apply), partials of p.Ap [and D1]
+//   K2 k_fused_update: alpha = rho / p.Ap from K1's partials, x += alpha p,
+//                      r -= alpha Ap, z = r/d, partials of r.r and r.z
+// Every block reduces the other kernel's few hundred partials itself, in the
+// same fixed order, so no separate finalize launch is needed.
+//
+// Reduction order (deterministic, independent of the autotuned apply schedule):
+//   K1 partials: tile (TY x TZ = 8 x 32 node columns, CX = 8 planes); each
+//     thread sums its column's products over its planes in plane order, then a
+//     256-wide tree gives one partial per tile;
+//   K2 / init partials: super-blocks of SB = 2048 consecutive nodes; thread t
+//     sums nodes base + 256 s + t for s = 0..7 in order, then a 256-wide tree;
+//   final: thread t sums partials t, t+256, ... in order, then a 256-wide tree.
+// The CPU oracle restates exactly this order (ho_pcg_solve_fused).
+//
+// State: values written by K1 are read only by K2 and vice versa (rho is
+// double-buffered by iteration parity), so no block reads a value that another
+// block of the same launch writes.  The search direction is double-buffered
+// the same way: iteration k reads p_{k-1} from P[(k+1)&1] (tile halos included)
+// and writes p_k to P[k&1], so no tile reads a direction another tile has
+// already overwritten.
+//
+// D1 of the fused apply: s = 1_int.Ap sums each thread's own interior nodes;
+// c = w.p and xx = p.p are summed where the tile writes p (every node of the
+// mesh exactly once, boundary included), in plane order, window slot order.
+#pragma once
+#include "heat_kernels.h"
+
+namespace heat {
+
+constexpr int FTY = 8, FTZ = 32, FCX = 8;    // fused apply tile
+constexpr int SB = 2048;                      // nodes per update super-block
+constexpr int SBS = SB / NT;                  // strided nodes per thread
+
+struct FusedState {
+    double rho[2];          // r.z of the last two K1 launches (slot iter & 1)
+    double bnorm, alpha, beta, tol;
+    int iter;               // completed updates, written by K2 (and init)
+    int k1_iter;            // iteration index seen by the current K1, written by K1
+    int max_iter, done;
+    int outer_done, inner_total, check_outer, pad1;
+    double bnorm_outer, rnorm_outer, tol_outer;
+};
+
+// 256-wide tree over one value per thread; result valid in thread 0.
+__device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
+    const int t = threadIdx.x;
+    sd[k][t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if (t < st) sd[k][t] += sd[k][t + st];
+        __syncthreads();
+    }
+    return sd[k][0];
+}
+
+// final stage over `count` partials of K sums (stride count), all threads get the results
+template <int K>
+__device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* __restrict__ part, int count,
+                                                double out[K]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double a = 0.0;
+        for (int i = t; i < count; i += NT) a += part[(size_t)k * count + i];
+        sd[k][t] = a;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int st = NT / 2; st > 0; st >>= 1) {
+        if (t < st) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) sd[k][t] += sd[k][t + st];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = sd[k][0];
+    __syncthreads();
+}
+
+// r = b - Ap; z = r/d; super-block partials of b.b, r.r, r.z.
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fused_init(const T* __restrict__ b, const T* __restrict__ Ap,
+                                                   const T* __restrict__ d, T* __restrict__ r,
+                                                   T* __restrict__ z, int nn,
+                                                   double* __restrict__ sp, int nsb,
+                                                   FusedState* fs) {
+    __shared__ double sd[3][NT];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
+    for (int s = 0; s < SBS; ++s) {
+        const size_t i = base + (size_t)s * NT;
+        if (i < (size_t)nn) {
+            const T bi = b[i];
+            const T ri = (T)fma(-1.0, (double)Ap[i], (double)bi);
+            const T zi = ri / d[i];
+            r[i] = ri;
+            z[i] = zi;
+            a0 += (double)bi * (double)bi;
+            a1 += (double)ri * (double)ri;
+            a2 += (double)ri * (double)zi;
+        }
+    }
+    const double s0 = tree256(sd, 0, a0), s1 = tree256(sd, 1, a1), s2 = tree256(sd, 2, a2);
+    if (threadIdx.x == 0) {
+        sp[blockIdx.x] = s0;
+        sp[nsb + blockIdx.x] = s1;
+        sp[2 * nsb + blockIdx.x] = s2;
+        if (blockIdx.x == 0) {
+            fs->iter = 0;
+            fs->done = 0;
+        }
+    }
+}
+
+// K1.  sp: K2 partials (r.r, r.z) or, at iteration 0, init partials (b.b, r.r, r.z).
+// tp: output partials of this launch [p.Ap | 1.Ap | w.p | p.p] x ntiles.
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
+    T* __restrict__ P0, T* __restrict__ P1, const T* __restrict__ z, T* __restrict__ Ap, const T* __restrict__ cr,
+    const T* __restrict__ ck, Grid g, const double* __restrict__ w, const double* __restrict__ sp,
+    int nsb, double* __restrict__ tp, int nby, int nbz, int ntiles, FusedState* fs,
+    double* __restrict__ hist) {
+    constexpr int TY = FTY, TZ = FTZ, CX = FCX, NTH = TY * TZ;
+    constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
+    constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
+    constexpr int XR = (XN + NTH - 1) / NTH, ER = (EN + NTH - 1) / NTH;
+    __shared__ T xs[2][XN];
+    __shared__ T es_r[2][EN];
+    __shared__ T es_k[2][EN];
+    __shared__ double sd[5][NT];         // reductions (partials in, tile sums out)
+    if (fs->done) return;
+    const int iter = fs->iter;
+    // ---- beta, convergence (reference order of checks, :1020-1035)
+    double beta = 0.0;
+    if (iter == 0) {
+        double o[5];
+        if (fs->check_outer) {
+            reduce_partials<5>(sd, sp, nsb, o);
+            double bo = sqrt(o[3]);
+            if (bo < 1e-30) bo = 1.0;
+            const bool outer_done = sqrt(o[4]) / bo < fs->tol_outer;
+            if (threadIdx.x == 0 && blockIdx.x == 0) {
+                fs->bnorm_outer = bo;
+                fs->rnorm_outer = sqrt(o[4]);
+                fs->outer_done = outer_done;
+                if (outer_done) fs->done = 1;
+            }
+            if (outer_done) return;
+        } else {
+            reduce_partials<3>(sd, sp, nsb, o);
+        }
+        double bnorm = sqrt(o[0]);
+        if (bnorm < 1e-30) bnorm = 1.0;
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            fs->bnorm = bnorm;
+            fs->rho[0] = o[2];
+            fs->k1_iter = 0;
+            if (hist) { hist[0] = sqrt(o[1]) / bnorm; hist[1] = __builtin_nan(""); hist[2] = __builtin_nan(""); }
+            if (fs->max_iter <= 0) fs->done = 1;
+        }
+        if (fs->max_iter <= 0) return;
+    } else {
+        double o[2];
+        reduce_partials<2>(sd, sp, nsb, o);
+        const double bnorm = fs->bnorm;
+        const double rnorm = sqrt(o[0]);
+        const bool conv = (rnorm / bnorm < fs->tol);
+        if (!conv) beta = o[1] / fs->rho[(iter - 1) & 1];
+        if (threadIdx.x == 0 && blockIdx.x == 0) {
+            if (hist) {
+                double* h = hist + 3 * iter;
+                h[0] = rnorm / bnorm; h[1] = fs->alpha; h[2] = conv ? 0.0 : beta;
+            }
+            if (conv || iter >= fs->max_iter) fs->done = 1;
+            else { fs->rho[iter & 1] = o[1]; fs->beta = beta; fs->k1_iter = iter; }
+        }
+        if (conv || iter >= fs->max_iter) return;      // uniform over the grid
+    }
+    // ---- tile apply on p = z + beta p (iteration 0: p = z)
+    const int b = xcd_slab_block(ntiles);
+    if (b >= ntiles) return;
+    const int bz = b % nbz;
+    const int by = (b / nbz) % nby;
+    const int chunk = b / (nbz * nby);
+    const int n = g.n, S = g.sx, nn1 = g.nn1, nsq = n * n;
+    const int tid = threadIdx.x;
+    const int ty = tid / TZ, tz = tid - (tid / TZ) * TZ;
+    const int y0 = 1 + by * TY, z0 = 1 + bz * TZ;
+    const int iy = y0 + ty, iz = z0 + tz;
+    const bool inmesh = (iy <= n) && (iz <= n);
+    const bool interior = (iy < n) && (iz < n);
+    const int ix0 = chunk * CX;
+    const int ix1 = min(ix0 + CX, n + 1);
+    const int py = iy & 1, pz = iz & 1;
+    const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
+    const bool first = (iter == 0);
+    const T* __restrict__ pold = (iter & 1) ? P0 : P1;
+    T* __restrict__ p = (iter & 1) ? P1 : P0;
+    auto pnew = [&](size_t i) -> T {
+        return first ? z[i] : (T)fma(beta, (double)pold[i], (double)z[i]);
+    };
+    double a_pap = 0.0, a_s = 0.0, a_c = 0.0, a_xx = 0.0;
+    int xoff[XR], eoff_t[ER];
+    bool xok[XR], xown[XR], eok[ER];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int i = tid + k * NTH;
+        const int r = i / XW, c = i - (i / XW) * XW;
+        const int yy = y0 - 1 + r, zz = z0 - 1 + c;
+        xok[k] = (i < XN) && yy <= n && zz <= n;
+        // the tile writes the new p of its own nodes, and of row 0 / column 0 on the first tiles
+        xown[k] = xok[k] && r <= TY && c <= TZ && (r >= 1 || by == 0) && (c >= 1 || bz == 0);
+        xoff[k] = yy * nn1 + zz;
+    }
+#pragma unroll
+    for (int k = 0; k < ER; ++k) {
+        const int i = tid + k * NTH;
+        const int r = i / EW, c = i - (i / EW) * EW;
+        const int ey = y0 - 1 + r, ez = z0 - 1 + c;
+        eok[k] = (i < EN) && ey < n && ez < n;
+        eoff_t[k] = ey * n + ez;
+    }
+    // fetch issues the loads of a plane into registers; park (after the
+    // plane in flight has been used) forms p = z + beta p_old, writes the
+    // owned part of it to memory and the window to LDS, so that the load
+    // latency is covered by the compute in between.
+    T zf[XR], pf[XR], ef_r[ER], ef_k[ER];
+    double wf[XR];
+    int fplane = 0;
+    auto fetch = [&](int xplane, int eplane) {
+        fplane = xplane;
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            T zv = T(0), pv = T(0);
+            double wv = 0.0;
+            if (xplane <= n && xok[k]) {
+                const size_t i = (size_t)xplane * S + xoff[k];
+                zv = z[i];
+                if (!first) pv = pold[i];
+                if (DETECT && xown[k]) wv = w[i];
+            }
+            zf[k] = zv;
+            pf[k] = pv;
+            wf[k] = wv;
+        }
+#pragma unroll
+        for (int k = 0; k < ER; ++k) {
+            const bool ok = eplane < n && eok[k];
+            const size_t e = (size_t)eplane * nsq + eoff_t[k];
+            ef_r[k] = ok ? cr[e] : T(0);
+            ef_k[k] = ok ? ck[e] : T(0);
+        }
+    };
+    auto park = [&](int xslot, int eslot) {
+        const bool own_plane = fplane >= ix0 && fplane < ix1 && fplane <= n;
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            const T v = first ? zf[k] : (T)fma(beta, (double)pf[k], (double)zf[k]);
+            if (own_plane && xown[k]) {
+                p[(size_t)fplane * S + xoff[k]] = v;
+                if (DETECT) {
+                    a_c += wf[k] * (double)v;
+                    a_xx += (double)v * (double)v;
+                }
+            }
+            if (tid + k * NTH < XN) xs[xslot][tid + k * NTH] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < ER; ++k)
+            if (tid + k * NTH < EN) {
+                es_r[eslot][tid + k * NTH] = ef_r[k];
+                es_k[eslot][tid + k * NTH] = ef_k[k];
+            }
+    };
+    int eidx[2][2], eoff[2][2], xidx[3][3];
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EW + (tz + 1 - (cz ^ pz));
+            eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
+        }
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
+    T W[3][3][3];
+    Coef<T> Eprev[2][2], Enew[2][2];
+    // prologue: plane ix0-1 of the window and element plane ix0-1 straight from memory
+    if (interior && ix0 >= 1) {
+        const size_t q = (size_t)iy * nn1 + iz;
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                W[0][bb][c] = pnew((size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz);
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz) {
+                const size_t e = (size_t)(ix0 - 1) * nsq + eoff[cy][cz];
+                Eprev[cy][cz].cr = cr[e];
+                Eprev[cy][cz].ck = ck[e];
+            }
+    }
+    // plane ix0 through the tile (it writes the new p of plane ix0), then ix0+1
+    fetch(ix0, ix0);
+    park(0, 0);
+    __syncthreads();
+    if (interior) {
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) W[1][bb][c] = xs[0][xidx[bb][c]];
+    }
+    __syncthreads();
+    fetch(ix0 + 1, ix0);
+    park((ix0 + 1) & 1, ix0 & 1);
+    __syncthreads();
+    const int par_yz = (py << 1) | pz;
+    const FiArm nofi{0, 0, 0};
+    for (int ix = ix0; ix < ix1; ++ix) {
+        fetch(ix + 2, ix + 1);
+        if (interior && ix <= n - 1) {
+            const T* xp = xs[(ix + 1) & 1];
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) W[2][bb][c] = xp[xidx[bb][c]];
+#pragma unroll
+            for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+                for (int cz = 0; cz < 2; ++cz) {
+                    Enew[cy][cz].cr = es_r[ix & 1][eidx[cy][cz]];
+                    Enew[cy][cz].ck = es_k[ix & 1][eidx[cy][cz]];
+                }
+        }
+        if (inmesh) {
+            T acc = T(0);
+            if (interior && ix >= 1 && ix <= n - 1) {
+                if (ix & 1) acc = march_node<T, false, 1>(W, Eprev, Enew, par_yz, ix, eoff, nsq, nofi);
+                else acc = march_node<T, false, 0>(W, Eprev, Enew, par_yz, ix, eoff, nsq, nofi);
+            }
+            const size_t base = (size_t)ix * S;
+            const size_t me = base + (size_t)iy * nn1 + iz;
+            Ap[me] = acc;
+            if (by == 0 && ty == 0) Ap[base + iz] = T(0);
+            if (bz == 0 && tz == 0) Ap[base + (size_t)iy * nn1] = T(0);
+            if (by == 0 && bz == 0 && tid == 0) Ap[base] = T(0);
+            // p.Ap over interior nodes (Ap is 0 elsewhere, and adding a zero
+            // product leaves a sum that starts at +0 unchanged)
+            const bool int_node = interior && ix >= 1 && ix <= n - 1;
+            if (int_node) {
+                a_pap += (double)W[1][1][1] * (double)acc;
+                if (DETECT) a_s += (double)acc;
+            }
+        }
+        park(ix & 1, (ix + 1) & 1);
+        __syncthreads();
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                W[0][bb][c] = W[1][bb][c];
+                W[1][bb][c] = W[2][bb][c];
+            }
+#pragma unroll
+        for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+            for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
+    }
+    const double t0 = tree256(sd, 0, a_pap);
+    double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (DETECT) { t1 = tree256(sd, 1, a_s); t2 = tree256(sd, 2, a_c); t3 = tree256(sd, 3, a_xx); }
+    if (tid == 0) {
+        tp[b] = t0;
+        if (DETECT) { tp[ntiles + b] = t1; tp[2 * ntiles + b] = t2; tp[3 * ntiles + b] = t3; }
+    }
+}
+
+// K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(NT) void k_fused_update(T* __restrict__ x, const T* __restrict__ P0,
+                                                     const T* __restrict__ P1,
+                                                     T* __restrict__ r, const T* __restrict__ Ap,
+                                                     const T* __restrict__ d, T* __restrict__ z,
+                                                     int nn, const double* __restrict__ tp,
+                                                     int ntiles, double* __restrict__ sp, int nsb,
+                                                     FusedState* fs, DevState* st, double nwA) {
+    __shared__ double sd[4][NT];
+    if (fs->done) return;
+    const int k1 = fs->k1_iter;
+    double o[4];
+    if (DETECT) reduce_partials<4>(sd, tp, ntiles, o);
+    else reduce_partials<1>(sd, tp, ntiles, o);
+    const double pAp = o[0];
+    // D1 of this iteration's apply is recorded before the breakdown check, as
+    // the reference's apply_operator does (:908-919 run before :1013)
+    if (DETECT && threadIdx.x == 0 && blockIdx.x == 0) abft_record(st, o[1], o[2], o[3], nwA);
+    if (fabs(pAp) < 1e-30) {                         // :1013
+        if (threadIdx.x == 0 && blockIdx.x == 0) fs->done = 1;
+        return;
+    }
+    const double alpha = fs->rho[k1 & 1] / pAp;
+    const T* __restrict__ p = (k1 & 1) ? P1 : P0;
+    double a_rr = 0.0, a_rz = 0.0;
+    const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
+    for (int s = 0; s < SBS; ++s) {
+        const size_t i = base + (size_t)s * NT;
+        if (i < (size_t)nn) {
+            x[i] = (T)fma(alpha, (double)p[i], (double)x[i]);
+            const T ri = (T)fma(-alpha, (double)Ap[i], (double)r[i]);
+            const T zi = ri / d[i];
+            r[i] = ri;
+            z[i] = zi;
+            a_rr += (double)ri * (double)ri;
+            a_rz += (double)ri * (double)zi;
+        }
+    }
+    const double s0 = tree256(sd, 0, a_rr), s1 = tree256(sd, 1, a_rz);
+    if (threadIdx.x == 0) {
+        sp[blockIdx.x] = s0;
+        sp[nsb + blockIdx.x] = s1;
+        if (blockIdx.x == 0) {
+            fs->alpha = alpha;
+            fs->iter = k1 + 1;
+            fs->inner_total += 1;
+        }
+    }
+}
+
+// FP64 outer residual of the refined solve from Ax (the projected FP64 apply):
+// r64 = b - Ax, inner start e = 0, r32 = (float) r64, z32 = r32 / d32;
+// super-block partials [r32.r32, r32.r32, r32.z32, b.b, r64.r64].
+__global__ __launch_bounds__(NT) void k_fused_mixed_init(
+    const double* __restrict__ b, const double* __restrict__ Ax, const float* __restrict__ d32,
+    float* __restrict__ e32, float* __restrict__ r32, float* __restrict__ z32, int nn,
+    double* __restrict__ sp, int nsb, FusedState* fs) {
+    __shared__ double sd[5][NT];
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
+    for (int s = 0; s < SBS; ++s) {
+        const size_t i = base + (size_t)s * NT;
+        if (i < (size_t)nn) {
+            const double bi = b[i];
+            const double ri = fma(-1.0, Ax[i], bi);
+            const float rf = (float)ri;
+            const float zf = rf / d32[i];
+            e32[i] = 0.0f;
+            r32[i] = rf;
+            z32[i] = zf;
+            a[0] += (double)rf * (double)rf;
+            a[1] += (double)rf * (double)rf;
+            a[2] += (double)rf * (double)zf;
+            a[3] += bi * bi;
+            a[4] += ri * ri;
+        }
+    }
+    double t[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = tree256(sd, k, a[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) sp[(size_t)k * nsb + blockIdx.x] = t[k];
+        if (blockIdx.x == 0) {
+            fs->iter = 0;
+            fs->done = 0;
+        }
+    }
+}
+
+}  // namespace heat

@@ -10,7 +10,8 @@
 // Added flags (no reference counterpart): --tol X and --max-iter N (the
 // reference hard-codes 1e-8 and 500), --source-const Q, --ic sine,
 // --precision fp64|fp32 (FP32-refined solve), --inner-tol X,
-// --colored-impl gather|sweep, --pcg-chunk K, --no-graph.
+// --colored-impl gather|sweep|march|tile|tdma, --pcg-chunk K, --no-graph,
+// --pcg-order fused|reference (summation order of the PCG inner products).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -49,7 +50,7 @@ struct Opts {
     std::string diag_cg;
     int mixed = 0;
     double inner_tol = 1e-4;
-    int colored_impl = HEAT_COLORED_GATHER, pcg_chunk = 8, use_graph = 1;
+    int colored_impl = -1, pcg_chunk = 8, use_graph = 1, pcg_order = HEAT_PCG_FUSED;
 };
 
 void usage(const char* prog) {
@@ -60,7 +61,8 @@ void usage(const char* prog) {
             "   [--reproject-bc] [--detect] [--verify] [--no-source] [--ic zero|gaussian|sine] [--dump-snaps]\n"
             "   [--diag-cg FILE]   (one CG solve, logs iteration,rel_residual to FILE, then exits)\n"
             "   [--tol X] [--max-iter N] [--source-const Q] [--precision fp64|fp32] [--inner-tol X]\n"
-            "   [--colored-impl gather|sweep] [--pcg-chunk K] [--no-graph]\n"
+            "   [--colored-impl gather|sweep|march|tile|tdma] [--pcg-chunk K] [--no-graph]\n"
+            "   [--pcg-order fused|reference]\n"
             "  LEVEL: 1=G global memory, 2=R accumulator, 3=H halo staging buffer\n"
             "  BIT:   0-51 mantissa, 52-62 exponent, 63 sign\n",
             prog);
@@ -316,7 +318,12 @@ int parse(int argc, char** argv, Opts& o) {
         } else if (!strcmp(a, "--precision") && has(1)) { o.mixed = !strcmp(argv[++i], "fp32");
         } else if (!strcmp(a, "--inner-tol") && has(1)) { o.inner_tol = atof(argv[++i]);
         } else if (!strcmp(a, "--colored-impl") && has(1)) {
-            o.colored_impl = !strcmp(argv[++i], "sweep") ? HEAT_COLORED_SWEEP : HEAT_COLORED_GATHER;
+            const char* v = argv[++i];
+            o.colored_impl = !strcmp(v, "sweep") ? HEAT_COLORED_SWEEP : !strcmp(v, "march") ? HEAT_COLORED_MARCH
+                           : !strcmp(v, "tile") ? HEAT_COLORED_TILE : !strcmp(v, "tdma") ? HEAT_COLORED_TDMA
+                           : HEAT_COLORED_GATHER;
+        } else if (!strcmp(a, "--pcg-order") && has(1)) {
+            o.pcg_order = !strcmp(argv[++i], "reference") ? HEAT_PCG_REFERENCE_ORDER : HEAT_PCG_FUSED;
         } else if (!strcmp(a, "--pcg-chunk") && has(1)) { o.pcg_chunk = atoi(argv[++i]);
         } else if (!strcmp(a, "--no-graph")) { o.use_graph = 0;
         } else if (!strcmp(a, "--qrank") || !strcmp(a, "--qblock")) {
@@ -386,7 +393,8 @@ int run(const Opts& o) {
     S.nn = (long)nn;
     API(heat_field_stats(S.op, S.stats));
     API(heat_abft_threshold(S.op, &S.chk_tol));
-    API(heat_set_colored_impl(S.op, o.colored_impl));
+    if (o.colored_impl >= 0) API(heat_set_colored_impl(S.op, o.colored_impl));
+    API(heat_set_pcg_mode(S.op, o.pcg_order));
     API(heat_set_tuning(S.op, o.pcg_chunk, o.use_graph));
     API(heat_vector(S.op, HEAT_VEC_T, &S.T));
     API(heat_vector(S.op, HEAT_VEC_B, &S.b));

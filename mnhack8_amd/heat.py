@@ -80,6 +80,7 @@ SIGNATURES = {
     "heat_abft_threshold": (_I, [_P, C.POINTER(_D)]),
     "heat_set_tuning": (_I, [_P, _I, _I]),
     "heat_set_march_chunk": (_I, [_P, _I]),
+    "heat_set_pcg_mode": (_I, [_P, _I]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
@@ -196,6 +197,10 @@ class HeatOperator:
 
     def set_tile_shape(self, shape):
         _check(self._lib.heat_set_tile_shape(self.h, int(shape)))
+
+    def set_pcg_mode(self, mode):
+        """"fused" (two launches per iteration) or "reference" (the reference's dot order)."""
+        _check(self._lib.heat_set_pcg_mode(self.h, {"fused": 0, "reference": 1}[mode]))
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))

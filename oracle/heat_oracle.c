@@ -61,6 +61,7 @@ struct ho_op {
     long nblk;
     double *r, *z, *p, *Ap;     /* PCG work vectors */
     int nthreads;
+    int pcg_order;              /* 0: the reference's sums, 1: the GPU library's fused order */
 };
 
 static char g_err[512];
@@ -637,6 +638,260 @@ int ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol
     return pcg_impl(op, b, x, max_iter, tol, hist, NULL);
 }
 
+/* ------------------------------------------------------------------------- */
+/* pcg_solve in the GPU library's two-launch order.  The iteration is the    */
+/* reference's (src/heat_solver_het.cu:994-1039); only the fixed order of    */
+/* the inner-product sums differs:                                            */
+/*  - p.Ap and the D1 sums of the apply: tiles of 8 (y) x 32 (z) node columns */
+/*    by 8 planes (x); each of the tile's 256 threads sums its column, then a */
+/*    256-wide tree;                                                          */
+/*  - r.r, r.z (and b.b at the start): super-blocks of 2048 nodes; thread t   */
+/*    sums nodes base + 256 s + t, s = 0..7, then a 256-wide tree;            */
+/*  - final: thread t sums partials t, t+256, ... then a 256-wide tree.       */
+/* ------------------------------------------------------------------------- */
+enum { F_TY = 8, F_TZ = 32, F_CX = 8, F_NT = 256, F_SB = 2048 };
+enum { F_XW = F_TZ + 2, F_XN = (F_TZ + 2) * (F_TY + 2), F_XR = (F_XN + F_NT - 1) / F_NT };
+
+static double tree_256(double* s) {
+    for (int st = F_NT / 2; st > 0; st >>= 1)
+        for (int t = 0; t < st; ++t) s[t] += s[t + st];
+    return s[0];
+}
+
+static double fold_partials(const double* part, long count) {
+    double s[F_NT];
+    for (int t = 0; t < F_NT; ++t) {
+        double a = 0.0;
+        for (long i = t; i < count; i += F_NT) a += part[i];
+        s[t] = a;
+    }
+    return tree_256(s);
+}
+
+/* super-block order of sum_i x_i y_i */
+static int g_fused_threads = 1;
+
+static double sb_dot(long nn, const double* x, const double* y, double* part) {
+    const long nsb = (nn + F_SB - 1) / F_SB;
+#pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
+    for (long sb = 0; sb < nsb; ++sb) {
+        double s[F_NT];
+        for (int t = 0; t < F_NT; ++t) {
+            double a = 0.0;
+            for (int k = 0; k < F_SB / F_NT; ++k) {
+                const long i = sb * F_SB + (long)k * F_NT + t;
+                if (i < nn) a += x[i] * y[i];
+            }
+            s[t] = a;
+        }
+        part[sb] = tree_256(s);
+    }
+    return fold_partials(part, nsb);
+}
+
+/* tile order of p.Ap, 1_int.Ap, w.p and p.p (out[0..3]) */
+static void tile_sums(const ho_grid* g, const double* p, const double* Ap, const double* w,
+                      double* part, double out[4]) {
+    const int n = g->n;
+    const int nby = (n + F_TY - 1) / F_TY, nbz = (n + F_TZ - 1) / F_TZ;
+    const long ntiles = (long)nby * nbz * ((g->nn1 + F_CX - 1) / F_CX);
+#pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
+    for (long b = 0; b < ntiles; ++b) {
+        const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), chunk = (int)(b / ((long)nbz * nby));
+        const int ix0 = chunk * F_CX, ix1 = (ix0 + F_CX < n + 1) ? ix0 + F_CX : n + 1;
+        double s[4][F_NT];
+        for (int tid = 0; tid < F_NT; ++tid) {
+            const int ty = tid / F_TZ, tz = tid % F_TZ;
+            const int iy = 1 + by * F_TY + ty, iz = 1 + bz * F_TZ + tz;
+            double a_pap = 0.0, a_s = 0.0, a_c = 0.0, a_xx = 0.0;
+            /* where the tile writes p: window rows 1..8 and columns 1..32 (row and
+             * column 0 too on the first tiles), in plane order, slot order */
+            for (int xp = ix0; xp < ix1; ++xp)
+                for (int k = 0; k < F_XR; ++k) {
+                    const int i = tid + k * F_NT;
+                    if (i >= F_XN) continue;
+                    const int r = i / F_XW, c = i % F_XW;
+                    const int yy = by * F_TY + r, zz = bz * F_TZ + c;
+                    if (yy > n || zz > n || r > F_TY || c > F_TZ) continue;
+                    if ((r < 1 && by != 0) || (c < 1 && bz != 0)) continue;
+                    const double v = p[lnid(g, xp, yy, zz)];
+                    if (w) a_c += w[lnid(g, xp, yy, zz)] * v;
+                    a_xx += v * v;
+                }
+            for (int ix = ix0; ix < ix1; ++ix)
+                if (ix >= 1 && ix <= n - 1 && iy <= n - 1 && iz <= n - 1) {
+                    const long i = lnid(g, ix, iy, iz);
+                    a_pap += p[i] * Ap[i];
+                    a_s += Ap[i];
+                }
+            s[0][tid] = a_pap; s[1][tid] = a_s; s[2][tid] = a_c; s[3][tid] = a_xx;
+        }
+        for (int q = 0; q < 4; ++q) part[q * ntiles + b] = tree_256(s[q]);
+    }
+    for (int q = 0; q < 4; ++q) out[q] = fold_partials(part + q * ntiles, ntiles);
+}
+
+static long fused_part_len(const ho_grid* g) {
+    const int n = g->n;
+    const long ntiles = (long)((n + F_TY - 1) / F_TY) * ((n + F_TZ - 1) / F_TZ) * ((g->nn1 + F_CX - 1) / F_CX);
+    const long nsb = (g->nn + F_SB - 1) / F_SB;
+    return 4 * (ntiles > nsb ? ntiles : nsb);
+}
+
+/* the two orders on their own, for the checks of the restatement */
+double ho_fused_sb_dot(ho_op* op, const double* x, const double* y) {
+    double* part = (double*)malloc(sizeof(double) * fused_part_len(&op->g));
+    const double v = sb_dot(op->g.nn, x, y, part);
+    free(part);
+    return v;
+}
+void ho_fused_tile_sums(ho_op* op, const double* p, const double* Ap, const double* w, double out[4]) {
+    double* part = (double*)malloc(sizeof(double) * fused_part_len(&op->g));
+    tile_sums(&op->g, p, Ap, w, part, out);
+    free(part);
+}
+
+static void apply_projected(ho_op* op, const double* x, double* y) {
+    ho_fi none = op->fi;
+    none.armed = 0;
+    memset(y, 0, sizeof(double) * op->g.nn);
+    operator_sweep(op, x, y, op->dt, &none);
+    dirichlet_value(op, y, 0.0);
+}
+
+int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol, double* hist) {
+    const ho_grid* g = &op->g;
+    const long n = g->nn;
+    g_fused_threads = op->nthreads;
+    double *r = op->r, *z = op->z, *p = op->p, *Ap = op->Ap;
+    double* part = (double*)malloc(sizeof(double) * fused_part_len(g));
+    ho_apply_operator(op, x, Ap, op->dt, 1, 1);          /* level R fires here; D1 in the reference order */
+    for (long i = 0; i < n; ++i) { r[i] = fma(-1.0, Ap[i], b[i]); z[i] = r[i] / op->diag[i]; }
+    double bnorm = sqrt(sb_dot(n, b, b, part));
+    if (bnorm < 1e-30) bnorm = 1.0;
+    const double rr0 = sb_dot(n, r, r, part);
+    double rho = sb_dot(n, r, z, part);
+    if (hist) { hist[0] = sqrt(rr0) / bnorm; hist[1] = NAN; hist[2] = NAN; }
+    int iter = 0;
+    if (max_iter > 0) {
+        memcpy(p, z, sizeof(double) * n);
+        for (;;) {
+            apply_projected(op, p, Ap);
+            double t[4];
+            tile_sums(g, p, Ap, op->detect ? op->wA : NULL, part, t);
+            if (op->detect) {
+                double rel = fabs(t[1] - t[2]) / (op->norm_wA * sqrt(t[3]) + 1e-300);
+                if (!isfinite(rel)) rel = 1e300;
+                op->abft_step_max = fmax(op->abft_step_max, rel);
+                op->abft_step_calls++;
+            }
+            if (fabs(t[0]) < 1e-30) break;
+            const double alpha = rho / t[0];
+            for (long i = 0; i < n; ++i) {
+                x[i] = fma(alpha, p[i], x[i]);
+                r[i] = fma(-alpha, Ap[i], r[i]);
+                z[i] = r[i] / op->diag[i];
+            }
+            const double rr = sb_dot(n, r, r, part), rz = sb_dot(n, r, z, part);
+            ++iter;
+            const double rnorm = sqrt(rr);
+            const int conv = (rnorm / bnorm < tol);
+            const double beta = conv ? 0.0 : rz / rho;
+            if (hist) { double* h = hist + 3 * iter; h[0] = rnorm / bnorm; h[1] = alpha; h[2] = beta; }
+            if (conv || iter >= max_iter) break;
+            rho = rz;
+            for (long i = 0; i < n; ++i) p[i] = fma(beta, p[i], z[i]);
+        }
+    }
+    free(part);
+    return iter;
+}
+
+/* FP32-refined solve in the same order: FP64 outer residual r = b - A x, an
+ * FP32 Jacobi-PCG on A e = r (FP32 vectors and operator; alpha, beta and the
+ * inner products in FP64) to a relative tolerance tol_inner, x += e, until
+ * ||b - A x|| / ||b|| < tol or max_outer corrections. */
+int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_outer, int max_inner,
+                             double tol, double tol_inner, int* outer, int* inner_total) {
+    const ho_grid* g = &op->g;
+    const long n = g->nn;
+    float* f = (float*)malloc(sizeof(float) * 6 * n);
+    float *e = f, *r = f + n, *z = f + 2 * n, *p = f + 3 * n, *ap = f + 4 * n, *d = f + 5 * n;
+    double* dv = (double*)malloc(sizeof(double) * 5 * n);
+    double *r64 = dv, *dr = dv + n, *dz = dv + 2 * n, *dp = dv + 3 * n, *dap = dv + 4 * n;
+    double* part = (double*)malloc(sizeof(double) * fused_part_len(g));
+    for (long i = 0; i < n; ++i) d[i] = (float)op->diag[i];
+    int nout = 0, total = 0;
+    for (;;) {
+        apply_projected(op, x, r64);
+        for (long i = 0; i < n; ++i) {
+            r64[i] = fma(-1.0, r64[i], b[i]);
+            r[i] = (float)r64[i];
+            z[i] = r[i] / d[i];
+            e[i] = 0.0f;
+            dr[i] = r[i]; dz[i] = z[i];
+        }
+        const double rr = sb_dot(n, dr, dr, part), rz = sb_dot(n, dr, dz, part);
+        double bo = sqrt(sb_dot(n, b, b, part));
+        if (bo < 1e-30) bo = 1.0;
+        const int outer_done = sqrt(sb_dot(n, r64, r64, part)) / bo < tol;
+        if (!outer_done && max_inner > 0) {
+            double bnorm = sqrt(rr);
+            if (bnorm < 1e-30) bnorm = 1.0;
+            double rho = rz;
+            int it = 0;
+            memcpy(p, z, sizeof(float) * n);
+            for (;;) {
+                ho_apply_operator_f32(op, p, ap, op->dt, 1);
+                for (long i = 0; i < n; ++i) { dp[i] = p[i]; dap[i] = ap[i]; }
+                double t[4];
+                tile_sums(g, dp, dap, NULL, part, t);
+                if (fabs(t[0]) < 1e-30) break;
+                const double alpha = rho / t[0];
+                for (long i = 0; i < n; ++i) {
+                    e[i] = (float)fma(alpha, (double)p[i], (double)e[i]);
+                    r[i] = (float)fma(-alpha, (double)ap[i], (double)r[i]);
+                    z[i] = r[i] / d[i];
+                    dr[i] = r[i]; dz[i] = z[i];
+                }
+                const double rr2 = sb_dot(n, dr, dr, part), rz2 = sb_dot(n, dr, dz, part);
+                ++it;
+                ++total;
+                const int conv = sqrt(rr2) / bnorm < tol_inner;
+                const double beta = conv ? 0.0 : rz2 / rho;
+                if (conv || it >= max_inner) break;
+                rho = rz2;
+                for (long i = 0; i < n; ++i) p[i] = (float)fma(beta, (double)p[i], (double)z[i]);
+            }
+        }
+        if (outer_done || nout >= max_outer) break;
+        for (long i = 0; i < n; ++i) x[i] += (double)e[i];
+        ++nout;
+    }
+    if (outer) *outer = nout;
+    if (inner_total) *inner_total = total;
+    free(f);
+    free(dv);
+    free(part);
+    return 0;
+}
+
+/* the solve the command line runs, in the order chosen by --pcg-order */
+static int solve_cli(ho_op* op, const double* b, double* x, int max_iter, double tol, FILE* diag_log) {
+    /* the GPU library runs the fused iteration on the colour-ordered fast path only */
+    if (op->pcg_order == 0 || !op->colored || op->kernel == HO_KERNEL_GAUSS)
+        return pcg_impl(op, b, x, max_iter, tol, NULL, diag_log);
+    double* hist = diag_log ? (double*)malloc(sizeof(double) * 3 * (size_t)(max_iter + 1)) : NULL;
+    const int it = ho_pcg_solve_fused(op, b, x, max_iter, tol, hist);
+    if (diag_log) {
+        fprintf(diag_log, "0,%.10e,,\n", hist[0]);
+        for (int i = 1; i <= it; ++i)
+            fprintf(diag_log, "%d,%.10e,%.10e,%.10e\n", i, hist[3 * i], hist[3 * i + 1], hist[3 * i + 2]);
+        free(hist);
+    }
+    return it;
+}
+
 /* src/heat_solver_het.cu:556-584 */
 void ho_initial_condition(ho_op* op, double* v, int mode, double t, double alpha) {
     const ho_grid* g = &op->g;
@@ -737,7 +992,7 @@ static void run_case(ho_op* op, int n_steps, int snap_every, ho_fi fi_cfg, int r
         }
         double t0 = now_ms();
         ho_build_rhs(op, T, b);
-        int it = pcg_impl(op, b, T, ro->max_iter, ro->tol, NULL, NULL);
+        int it = solve_cli(op, b, T, ro->max_iter, ro->tol, NULL);
         if (reproject) dirichlet_value(op, T, 0.0);
         res->solve_ms += now_ms() - t0;
         op->fi.armed = 0;
@@ -855,7 +1110,7 @@ int ho_main(int argc, char** argv) {
     const char* tag = "run";
     ho_fi fi_cfg; memset(&fi_cfg, 0, sizeof fi_cfg); fi_cfg.step = -1;
     int snap_every = 5, verify_mode = 0, kernel = HO_KERNEL_FAST, colored = 0, reproject = 0, detect = 0;
-    int no_source = 0, ic_mode = 0, dump_snaps = 0, nthreads = 1, max_iter = 500;
+    int no_source = 0, ic_mode = 0, dump_snaps = 0, nthreads = 1, max_iter = 500, pcg_order = 0;
     int source_mode = HO_SOURCE_GAUSSIAN; double source_value = 0.0;
     double tol = 1e-8;
     const char* diag_cg_path = NULL;
@@ -884,6 +1139,7 @@ int ho_main(int argc, char** argv) {
         } else if (!strcmp(argv[i], "--source-const") && i + 1 < argc) {
             source_mode = HO_SOURCE_CONST; source_value = atof(argv[++i]);
         } else if (!strcmp(argv[i], "--threads") && i + 1 < argc) { nthreads = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--pcg-order") && i + 1 < argc) { pcg_order = !strcmp(argv[++i], "fused");
         } else if (argv[i][0] != '-') { outdir = argv[i];
         } else { fprintf(stderr, "unknown option %s\n", argv[i]); usage(argv[0]); return 1; }
     }
@@ -911,6 +1167,7 @@ int ho_main(int argc, char** argv) {
     if (!op) { fprintf(stderr, "%s\n", g_err); return g_err[0] == 'r' && strstr(g_err, "self-check") ? 2 : 1; }
     op->no_source = no_source;
     ho_set_threads(op, nthreads);
+    op->pcg_order = pcg_order;
     printf("Material: k in [%g, %g] (contrast %.3g), rho*c in [%g, %g]\n",
            op->kmin, op->kmax, op->kmax / op->kmin, op->rhocmin, op->rhocmax);
 
@@ -933,7 +1190,7 @@ int ho_main(int argc, char** argv) {
         long total_it = 0;
         for (int step = 0; step < n_steps; ++step) {
             ho_build_rhs(op, T, b);
-            total_it += pcg_impl(op, b, T, max_iter, tol, NULL, NULL);
+            total_it += solve_cli(op, b, T, max_iter, tol, NULL);
         }
         double ms = now_ms() - t0;
         double t_end = n_steps * dt;
@@ -961,7 +1218,7 @@ int ho_main(int argc, char** argv) {
         if (ic_mode == 1) ho_initial_condition(op, T, 1, 0.0, 0.0);
         if (ic_mode == 2) ho_initial_condition(op, T, 2, 0.0, op->kmin / op->rhocmin);
         ho_build_rhs(op, T, b);
-        int it = pcg_impl(op, b, T, max_iter, tol, NULL, lf);
+        int it = solve_cli(op, b, T, max_iter, tol, lf);
         fclose(lf);
         printf("diag-cg: %d iterations to %g, contrast k_max/k_min=%.3g -> %s\n", it, tol, op->kmax / op->kmin, diag_cg_path);
     } else {

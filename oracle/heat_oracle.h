@@ -64,6 +64,17 @@ void   ho_abft_take(ho_op* op, double* step_max, long* calls);
  * (max_iter+1) rows of {rel_residual, alpha, beta}. */
 int    ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol,
                     double* hist);
+/* The same solve with the inner products summed in the GPU library's fused
+ * two-launch order (tile / super-block partials, 256-wide trees). */
+int    ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol,
+                          double* hist);
+/* The fused order's sums alone: super-block x.y, and the tile sums
+ * [p.Ap (interior), 1_int.Ap, w.p, p.p]. */
+double ho_fused_sb_dot(ho_op* op, const double* x, const double* y);
+void   ho_fused_tile_sums(ho_op* op, const double* p, const double* Ap, const double* w, double out[4]);
+/* FP32-refined solve (FP64 outer residual, FP32 inner Jacobi-PCG) in that order. */
+int    ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_outer, int max_inner,
+                                double tol, double tol_inner, int* outer, int* inner_total);
 
 /* initial conditions, src/heat_solver_het.cu:556-584.  mode 1 gaussian, 2 sine mode at t */
 void   ho_initial_condition(ho_op* op, double* v, int mode, double t, double alpha);

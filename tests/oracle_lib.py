@@ -34,6 +34,10 @@ _SIG = {
     "ho_flip_bit": (None, [_P, _L, _I]),
     "ho_abft_take": (None, [_P, _P, _P]),
     "ho_pcg_solve": (_I, [_P, _P, _P, _I, _D, _P]),
+    "ho_pcg_solve_fused": (_I, [_P, _P, _P, _I, _D, _P]),
+    "ho_pcg_solve_mixed_fused": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
+    "ho_fused_sb_dot": (_D, [_P, _P, _P]),
+    "ho_fused_tile_sums": (None, [_P, _P, _P, _P, _P]),
     "ho_initial_condition": (None, [_P, _P, _I, _D, _D]),
 }
 _lib = None
@@ -133,6 +137,33 @@ class OracleOp:
         hist = np.full(3 * (max_iter + 1), np.nan)
         it = lib().ho_pcg_solve(self.h, _p(b), _p(x), max_iter, tol, _p(hist))
         return it, x, hist.reshape(-1, 3)[: it + 1]
+
+    def pcg_fused(self, b, x, max_iter=500, tol=1e-8):
+        """The same solve with the GPU library's fused summation order."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        hist = np.full(3 * (max_iter + 1), np.nan)
+        it = lib().ho_pcg_solve_fused(self.h, _p(b), _p(x), max_iter, tol, _p(hist))
+        return it, x, hist.reshape(-1, 3)[: it + 1]
+
+    def pcg_mixed_fused(self, b, x, max_outer=50, max_inner=500, tol=1e-8, tol_inner=1e-4):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        o, i = C.c_int(), C.c_int()
+        lib().ho_pcg_solve_mixed_fused(self.h, _p(b), _p(x), max_outer, max_inner, tol, tol_inner,
+                                       C.byref(o), C.byref(i))
+        return o.value, i.value, x
+
+    def fused_sb_dot(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        return lib().ho_fused_sb_dot(self.h, _p(x), _p(y))
+
+    def fused_tile_sums(self, p, Ap, w):
+        p, Ap, w = (np.ascontiguousarray(v, dtype=np.float64) for v in (p, Ap, w))
+        out = np.empty(4)
+        lib().ho_fused_tile_sums(self.h, _p(p), _p(Ap), _p(w), _p(out))
+        return out
 
     def initial_condition(self, mode, t=0.0, alpha=1.0):
         v = np.empty(self.nn)

@@ -28,17 +28,24 @@ def gold_csv(rel):
         return list(csv.DictReader(f))
 
 
-@pytest.fixture(params=["oracle", pytest.param("hip", marks=pytest.mark.gpu)])
+@pytest.fixture(params=["oracle", "oracle-fused", pytest.param("hip", marks=pytest.mark.gpu),
+                        pytest.param("hip-ref", marks=pytest.mark.gpu)])
 def solver(request):
-    """The binary under test: the CPU oracle, or the HIP driver (GPU)."""
+    """The binary under test: the CPU oracle in the reference's summation
+    order or in the GPU library's fused order, or the HIP driver (GPU) in its
+    default fused order or in the reference's order."""
     return request.param
 
 
 def run_oracle(args, tmp, solver="oracle"):
-    if solver == "oracle":
+    if solver.startswith("oracle"):
         cmd = [ORACLE_CLI, *map(str, args), "--threads", THREADS]
+        if solver == "oracle-fused":
+            cmd += ["--pcg-order", "fused"]
     else:
         cmd = [build.DRIVER, *map(str, args)]
+        if solver == "hip-ref":
+            cmd += ["--pcg-order", "reference"]
         print(f"[case] {' '.join(cmd)}", flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, cwd=tmp)
     assert r.returncode == 0, r.stderr
@@ -222,7 +229,7 @@ def test_v3_injection_runs(tmp_path, ln32, level, bit, solver):
     tr = list(csv.DictReader(open(tmp_path / "fi_trace_fi.csv")))
     E = np.array([float(r["E_fault"]) for r in tr])
     assert (E[:10] == 0).all()
-    if not (solver == "hip" and level == 2 and bit == 0):
+    if not (solver.startswith("hip") and level == 2 and bit == 0):
         assert E[10] > 0 or not np.isfinite(E[10])
 
 
@@ -244,7 +251,7 @@ def test_v3_boundary_control(tmp_path, ln32, solver):
 
 
 def test_level_h_and_qrank_rejected(tmp_path, ln32, solver):
-    binary = ORACLE_CLI if solver == "oracle" else build.DRIVER
+    binary = ORACLE_CLI if solver.startswith("oracle") else build.DRIVER
     r = subprocess.run([binary, "32", "0.02", "5e-4", ln32, str(tmp_path), "--fi", "3", "10", "5", "51"],
                        capture_output=True, text=True)
     assert r.returncode == 1 and "level H needs >= 2 ranks" in r.stderr

@@ -162,15 +162,21 @@ def test_dot_and_minmax_bitwise(nn_n):
     assert op.minmax(xd) == (x.min(), x.max())
 
 
+def oracle_pcg(orc, mode):
+    return orc.pcg_fused if mode == "fused" else orc.pcg
+
+
+@pytest.mark.parametrize("mode", ["fused", "reference"])
 @pytest.mark.parametrize("graph", [True, False])
 @pytest.mark.parametrize("chunk", [1, 8])
-def test_pcg_bitwise_with_oracle(graph, chunk):
+def test_pcg_bitwise_with_oracle(graph, chunk, mode):
     n, dt = 16, 5e-4
-    print(f"[case] pcg graph={graph} chunk={chunk}", flush=True)
+    print(f"[case] pcg graph={graph} chunk={chunk} mode={mode}", flush=True)
     k, rhoc = fields.lognormal(n, sigma=1.0, L=0.125, seed=7)
     k, rhoc = k.ravel(), rhoc.ravel()
     op = heat().HeatOperator(n, dt, k, rhoc)
     op.set_tuning(chunk, graph)
+    op.set_pcg_mode(mode)
     orc = OracleOp(n, dt, k, rhoc, kernel="struct")
     T0 = np.zeros(op.nn)
     b_or = orc.build_rhs(T0)
@@ -179,7 +185,7 @@ def test_pcg_bitwise_with_oracle(graph, chunk):
     assert np.array_equal(op.download("b"), b_or) or relerr(op.download("b"), b_or) < 1e-14
     op.upload("b", b_or)
     it, hist = op.pcg_solve("b", "T", 500, 1e-10, history=True)
-    ito, xo, histo = orc.pcg(b_or, T0, 500, 1e-10)
+    ito, xo, histo = oracle_pcg(orc, mode)(b_or, T0, 500, 1e-10)
     assert it == ito
     x = op.download("T")
     assert np.array_equal(x, xo), f"x differs: {relerr(x, xo):.3e}"
@@ -191,36 +197,76 @@ def test_pcg_bitwise_with_oracle(graph, chunk):
         assert it2 == ito and np.array_equal(op.download("T"), xo)
 
 
-def test_pcg_edge_cases():
+@pytest.mark.parametrize("mode", ["fused", "reference"])
+def test_pcg_edge_cases(mode):
     n, dt = 5, 1e-3
-    print("[case] pcg edges", flush=True)
+    print(f"[case] pcg edges {mode}", flush=True)
     k, rhoc = make_field(n)
     op = heat().HeatOperator(n, dt, k, rhoc)
+    op.set_pcg_mode(mode)
     orc = OracleOp(n, dt, k, rhoc, kernel="struct")
+    solve = oracle_pcg(orc, mode)
     zero = np.zeros(op.nn)
     # b = 0: bnorm clamps to 1, r = 0 -> p.Ap = 0 -> break after 0 iterations
     op.upload("b", zero)
     op.upload("T", zero)
-    assert op.pcg_solve("b", "T", 500, 1e-8) == orc.pcg(zero, zero, 500, 1e-8)[0] == 0
+    assert op.pcg_solve("b", "T", 500, 1e-8) == solve(zero, zero, 500, 1e-8)[0] == 0
     # max_iter reached
     b = orc.build_rhs(np.random.default_rng(0).standard_normal(op.nn))
     op.upload("b", b)
     op.upload("T", zero)
-    assert op.pcg_solve("b", "T", 3, 1e-30) == orc.pcg(b, zero, 3, 1e-30)[0] == 3
-    assert np.array_equal(op.download("T"), orc.pcg(b, zero, 3, 1e-30)[1])
-    # max_iter = 0
+    assert op.pcg_solve("b", "T", 3, 1e-30) == solve(b, zero, 3, 1e-30)[0] == 3
+    assert np.array_equal(op.download("T"), solve(b, zero, 3, 1e-30)[1])
+    # max_iter = 0: no update, history row 0 only
     op.upload("T", zero)
-    assert op.pcg_solve("b", "T", 0, 1e-8) == 0
+    it, hist = op.pcg_solve("b", "T", 0, 1e-8, history=True)
+    it_o, x_o, hist_o = solve(b, zero, 0, 1e-8)
+    assert it == it_o == 0 and np.array_equal(op.download("T"), zero)
+    np.testing.assert_array_equal(hist, hist_o)
 
 
-@pytest.mark.parametrize("impl", ["tdma", "tile", "march", "gather"])
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 31, 33, 40])
+@pytest.mark.parametrize("detect", [False, True])
+def test_fused_pcg_tile_geometry(n, detect):
+    """Mesh sizes around the fused apply's 8 x 32 x 8 tile (partial tiles in
+    every direction, a single-plane last chunk at n = 8, 40): iterates,
+    history and the D1 record bit-identical to the oracle's fused order."""
+    dt = 1e-3
+    print(f"[case] fused pcg n={n} detect={detect}", flush=True)
+    k, rhoc = make_field(n, seed=30 + n)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=detect, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=detect, source="const", source_value=1.0)
+    T0 = orc.initial_condition("sine", 0.0, 1.0)
+    b = orc.build_rhs(T0)
+    op.upload("b", b)
+    op.upload("T", T0)
+    op.abft_take()
+    orc.abft_take()
+    it, hist = op.pcg_solve("b", "T", 300, 1e-11, history=True)
+    ito, xo, histo = orc.pcg_fused(b, T0, 300, 1e-11)
+    print(f"  iterations {it} (oracle {ito})", flush=True)
+    assert it == ito
+    assert np.array_equal(op.download("T"), xo)
+    np.testing.assert_array_equal(hist, histo)
+    d1, calls = op.abft_take()
+    d1o, callso = orc.abft_take()
+    assert calls == callso
+    if detect:
+        # one record per apply: the initial residual, one per update, and one
+        # more when the loop ends on p.Ap = 0 (n = 1: no interior node)
+        assert d1 == d1o and calls == it + (2 if n == 1 else 1)
+
+
+@pytest.mark.parametrize("impl,mode", [("tdma", "fused"), ("tile", "fused"), ("march", "fused"), ("gather", "fused"),
+                                       ("tdma", "reference"), ("gather", "reference")])
 @pytest.mark.parametrize("bit", [30, 55, 62, 63])
-def test_fault_level_R_and_G(bit, impl):
+def test_fault_level_R_and_G(bit, impl, mode):
     n, dt = 8, 5e-4
-    print(f"[case] faults bit={bit}", flush=True)
+    print(f"[case] faults bit={bit} impl={impl} mode={mode}", flush=True)
     k, rhoc = make_field(n, seed=21)
     op = heat().HeatOperator(n, dt, k, rhoc, detect=True)
     op.set_colored_impl(impl)
+    op.set_pcg_mode(mode)
     orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True)
     T0 = orc.initial_condition("gaussian")
     target = (n // 2) * n * n + (n // 2) * n + n // 2
@@ -228,7 +274,7 @@ def test_fault_level_R_and_G(bit, impl):
     orc.abft_take()
     # level R: first matvec of the solve
     orc.arm_fault(2, target, bit)
-    ito, xo, _ = orc.pcg(b, T0, 500, 1e-8)
+    ito, xo, _ = oracle_pcg(orc, mode)(b, T0, 500, 1e-8)
     d1o, _ = orc.abft_take()
     op.upload("T", T0)
     op.upload("b", b)
@@ -243,6 +289,7 @@ def test_fault_level_R_and_G(bit, impl):
     assert np.array_equal(np.isfinite(x), fin)
     assert np.array_equal(x[fin], xo[fin])
     assert (d1 > 1e-10) == (d1o > 1e-10)
+    assert d1 == d1o or not (np.isfinite(d1) and np.isfinite(d1o))
     # level G: XOR of T[target % nn]
     node = target % op.nn
     op.upload("T", T0)
@@ -283,12 +330,34 @@ def test_fp32_operator_bitwise(dt_eff):
     assert relerr(y, y64) < 1e-5
 
 
-def test_mixed_precision_solve():
+@pytest.mark.parametrize("n", [7, 24, 33])
+def test_mixed_fused_bitwise(n):
+    """FP32-refined solve (fused order) bit-identical to the oracle's
+    restatement: outer and inner counts exact, FP64 iterate identical."""
+    dt = 1e-4
+    print(f"[case] mixed fused n={n}", flush=True)
+    k, rhoc = make_field(n, seed=40 + n)
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+    b = orc.build_rhs(np.zeros(op.nn))
+    for tol_inner in (1e-4, 1e-2):
+        op.upload("b", b)
+        op.upload("T", np.zeros(op.nn))
+        outer, inner = op.pcg_solve_mixed("b", "T", 50, 500, 1e-10, tol_inner)
+        oo, io, xo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, tol_inner)
+        print(f"  tol_inner={tol_inner}: outer {outer} inner {inner} (oracle {oo} {io})", flush=True)
+        assert (outer, inner) == (oo, io)
+        assert np.array_equal(op.download("T"), xo)
+
+
+@pytest.mark.parametrize("mode", ["fused", "reference"])
+def test_mixed_precision_solve(mode):
     n, dt = 24, 1e-4
-    print("[case] mixed solve", flush=True)
+    print(f"[case] mixed solve {mode}", flush=True)
     k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=1)
     k, rhoc = k.ravel(), rhoc.ravel()
     op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    op.set_pcg_mode(mode)
     orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
     T0 = np.zeros(op.nn)
     b = orc.build_rhs(T0)

@@ -217,3 +217,122 @@ def test_quadrants_matches_analytic_model():
     k, rhoc = fields.quadrants(4)
     assert k[0, 0, 0] == 100 and k[3, 0, 0] == 200 and k[0, 3, 0] == 300 and k[0, 0, 3] == 400
     assert rhoc[3, 0, 0] == 0.8 * 1.5 and rhoc[3, 3, 3] == 0.5 * 2.5
+
+
+def interior_mask(n):
+    X, Y, Z = node_grid(n)
+    return ((X > 0) & (X < 1) & (Y > 0) & (Y < 1) & (Z > 0) & (Z < 1)).ravel()
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 33, 40])
+def test_fused_order_sums_cover_every_node(n):
+    """The fused order's tile sums against correctly rounded sums: p.p and
+    w.p over every node (boundary included), p.Ap and 1.Ap over the interior.
+    A tile that skips its first row/column, a plane counted twice or a chunk
+    left out moves one of them far beyond the rounding bound."""
+    op = OracleOp(n, 1e-3, np.ones(n ** 3), np.ones(n ** 3), kernel="struct")
+    rng = np.random.default_rng(100 + n)
+    p = rng.standard_normal(op.nn) + 3.0
+    Ap = rng.standard_normal(op.nn) + 2.0
+    w = rng.standard_normal(op.nn) + 1.0
+    t = op.fused_tile_sums(p, Ap, w)
+    inner = interior_mask(n)
+    exact = [math.fsum((p * Ap)[inner]), math.fsum(Ap[inner]), math.fsum(w * p), math.fsum(p * p)]
+    bound = [np.sum(np.abs(p * Ap)), np.sum(np.abs(Ap)), np.sum(np.abs(w * p)), np.sum(p * p)]
+    for q in range(4):
+        assert abs(t[q] - exact[q]) <= 1e-13 * bound[q] + 1e-300, q
+    x = rng.standard_normal(op.nn)
+    y = rng.standard_normal(op.nn)
+    assert abs(op.fused_sb_dot(x, y) - math.fsum((x * y).tolist())) <= 1e-14 * np.sum(np.abs(x * y))
+
+
+def fused_sb_dot_restated(x, y):
+    """Super-block order: 2048-node blocks, thread t over nodes t + 256 s, a
+    256-wide tree per block, then a 256-wide strided fold and tree."""
+    nn = x.size
+    nsb = -(-nn // 2048)
+    part = []
+    for sb in range(nsb):
+        s = [0.0] * 256
+        for t in range(256):
+            a = 0.0
+            for k in range(8):
+                i = sb * 2048 + 256 * k + t
+                if i < nn:
+                    a += x[i] * y[i]
+            s[t] = a
+        st = 128
+        while st:
+            for t in range(st):
+                s[t] += s[t + st]
+            st //= 2
+        part.append(s[0])
+    f = [0.0] * 256
+    for t in range(256):
+        a = 0.0
+        for i in range(t, nsb, 256):
+            a += part[i]
+        f[t] = a
+    st = 128
+    while st:
+        for t in range(st):
+            f[t] += f[t + st]
+        st //= 2
+    return f[0]
+
+
+@pytest.mark.parametrize("n", [3, 20])
+def test_fused_sb_dot_order(n):
+    op = OracleOp(n, 1e-3, np.ones(n ** 3), np.ones(n ** 3), kernel="struct")
+    rng = np.random.default_rng(7 * n)
+    x = rng.standard_normal(op.nn)
+    y = rng.standard_normal(op.nn)
+    assert op.fused_sb_dot(x, y) == fused_sb_dot_restated(x, y)
+
+
+def lognormal_problem(n, seed=5, dt=1e-3):
+    rng = np.random.default_rng(seed)
+    k = np.exp(rng.standard_normal(n ** 3))
+    rhoc = np.exp(0.5 * rng.standard_normal(n ** 3))
+    op = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True, source="const", source_value=1.0)
+    T0 = op.initial_condition("sine", 0.0, 1.0)
+    return op, op.build_rhs(T0)
+
+
+@pytest.mark.parametrize("n", [9, 17])
+def test_fused_pcg_agrees_with_reference_order(n):
+    """The fused solve is the reference's iteration with another summation
+    order: same iteration count, iterates equal to rounding, same D1 scale."""
+    op, b = lognormal_problem(n)
+    op.abft_take()
+    it_r, x_r, h_r = op.pcg(b, np.zeros(op.nn), 500, 1e-10)
+    d1_r, calls_r = op.abft_take()
+    it_f, x_f, h_f = op.pcg_fused(b, np.zeros(op.nn), 500, 1e-10)
+    d1_f, calls_f = op.abft_take()
+    assert abs(it_r - it_f) <= 1
+    assert calls_r == calls_f == min(it_r, it_f) + 1
+    assert np.max(np.abs(x_r - x_f)) <= 1e-9 * np.max(np.abs(x_r))
+    k = min(it_r, it_f)
+    assert np.allclose(h_r[1:k, 1:], h_f[1:k, 1:], rtol=1e-6)
+    assert d1_r < 1e-12 and d1_f < 1e-12
+    # and the solution solves the system
+    r = b - op.apply(x_f, op.dt)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * 1.0001
+
+
+def test_fused_pcg_edge_cases():
+    op, b = lognormal_problem(5)
+    it, x, h = op.pcg_fused(b, np.zeros(op.nn), 0, 1e-8)
+    assert it == 0 and (x == 0).all() and h.shape == (1, 3) and np.isfinite(h[0, 0])
+    it, x, h = op.pcg_fused(np.zeros(op.nn), np.zeros(op.nn), 50, 1e-8)
+    assert it == 0 and (x == 0).all()          # p.Ap = 0: breakdown before any update
+    it1, x1, _ = op.pcg_fused(b, np.zeros(op.nn), 1, 1e-30)
+    assert it1 == 1
+
+
+def test_fused_mixed_reaches_fp64_tolerance():
+    op, b = lognormal_problem(11)
+    outer, inner, x = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    r = b - op.apply(x, op.dt)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+    assert 2 <= outer <= 6 and inner > outer
