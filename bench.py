@@ -31,22 +31,23 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD specification
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "gather_matvec_n128_traffic.txt")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "colored_matvec_n128_traffic.txt")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic():
-    """HBM bytes per launch of the matvec kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+def read_traffic(schedule):
+    """HBM bytes per launch of the matvec kernel of `schedule` ("tdma/p8/s0")
+    from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, each
+    scaled by the factor the same counter reads for a kernel of known bytes)."""
     try:
         with open(PROFILE_TRAFFIC) as f:
             for line in f:
-                m = re.match(r"\s*traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", line)
-                if m:
-                    return float(m.group(1))
+                m = re.match(r"\s*traffic_bytes_per_launch\[([^\]]+)\]\s*[:=]\s*([0-9.eE+]+)", line)
+                if m and m.group(1) == schedule:
+                    return float(m.group(2))
     except OSError:
         pass
     return None
@@ -84,7 +85,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
-                    help="run this many device copies of one node vector first (PMC byte-counter calibration)")
+                    help="run this many axpy passes (2 node vectors read, 1 written) first: the known-bytes "
+                         "kernel that calibrates the PMC byte counters")
+    ap.add_argument("--schedule", default=None,
+                    help="impl/pPLANES/sSHAPE (e.g. tdma/p8/s0) instead of the autotuner's pick")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {a.gpus})",
@@ -106,7 +110,7 @@ def main():
     op.upload("p", x)
 
     for _ in range(a.probe_copy):
-        op.copy("p", "r")
+        op.axpy(0.5, "p", "r")
     op.synchronize()
     if a.sweep:
         table = {}
@@ -120,8 +124,21 @@ def main():
             ms, _ = op.time_matvec("p", "Ap", 100)
             table[f"{impl}/p{planes}/s{shape}"] = round(ms * 1e3, 2)
         log(f"schedules (us/apply): {table}")
-    pick = op.autotune(50)
+    if a.schedule:
+        impl, pl, sh = a.schedule.split("/")
+        op.set_colored_impl(impl)
+        op.set_march_chunk(int(pl[1:]))
+        op.set_tile_shape(int(sh[1:]))
+        op.time_matvec("p", "Ap", 5)
+        ms_pick, _ = op.time_matvec("p", "Ap", 50)
+        from mnhack8_amd.heat import TILE_SHAPES
+        pick = (impl, int(pl[1:]), TILE_SHAPES[int(sh[1:])], ms_pick)
+    else:
+        pick = op.autotune(50)
     picked = f"{pick[0]} planes={pick[1]} tile={pick[2][0]}x{pick[2][1]}"
+    from mnhack8_amd.heat import TILE_SHAPES as _TS
+    shape_id = {v: k for k, v in _TS.items()}.get(tuple(pick[2]), 0)
+    sched_key = f"{pick[0]}/p{pick[1]}/s{shape_id}"
     log(f"autotuner picked {picked} ({pick[3] * 1e3:.2f} us)")
     log(f"warmup {a.warmup}")
     op.time_matvec("p", "Ap", max(1, a.warmup))
@@ -139,7 +156,7 @@ def main():
     alg_bytes = 8 * (2 * nn + 2 * ne)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     achieved_cold = alg_bytes / (cold_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    traffic = read_traffic(sched_key)
 
     result = {
         "metric": "PCG matvec GDOF/s at N=128 hex; time-to-1e-8, FP32-refined vs FP64",
@@ -162,7 +179,7 @@ def main():
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
                        "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
         "algorithmic_bytes_per_apply": alg_bytes,
-        "schedule": {"picked": picked, "us_per_apply": round(pick[3] * 1e3, 2)},
+        "schedule": {"picked": picked, "key": sched_key, "us_per_apply": round(pick[3] * 1e3, 2)},
     }
 
     if not a.no_solve:

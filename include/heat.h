@@ -168,6 +168,25 @@ int heat_flip_bit(heat_op* op, double* v, int64_t idx, int bit);
 int heat_abft_take(heat_op* op, double* step_max, int64_t* calls);
 /* PCG execution mode (HEAT_PCG_*), new. */
 int heat_set_pcg_mode(heat_op* op, int mode);
+
+/* Single-event upset in the PCG state (new; the reference injects only into
+ * T before build_rhs, level G, and into one element contribution of the
+ * first matvec, level R: src/heat_solver_het.cu:181-186, 369-374).  Arms a
+ * one-shot fault for the next heat_pcg_solve or heat_pcg_solve_mixed (fused
+ * order only): at PCG iteration ITERATION (0-based; the refined solve counts
+ * its inner iterations over all corrections), the kernel that produces
+ * vector VEC stores node NODE with bit BIT flipped (FP64 words: 0..63; the
+ * refined solve's FP32 inner state: 0..31).  Sums taken in that same kernel
+ * see the clean value.  VEC: the iterate (x, or the inner correction e),
+ * r, p, z or Ap. */
+#define HEAT_SEU_U  0
+#define HEAT_SEU_R  1
+#define HEAT_SEU_P  2
+#define HEAT_SEU_Z  3
+#define HEAT_SEU_AP 4
+int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteration);
+/* 1 if the fault armed for the last solve fired (its iteration was reached). */
+int heat_pcg_fault_fired(heat_op* op, int* fired);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);

@@ -88,6 +88,8 @@ struct heat_op {
     int fi_level = 0;
     // fused PCG (two launches per iteration)
     int pcg_mode = HEAT_PCG_FUSED;
+    struct { int armed = 0, vec = 0, bit = 0, iter = 0; long long idx = 0; } pcg_fi;   // one-shot PCG-state SEU
+    int pcg_fi_fired = 0;
     FusedState* fst = nullptr;
     FusedState* hfst = nullptr;   // pinned mirror
     double* fsp = nullptr;        // super-block partials, 5 x nsb
@@ -396,8 +398,18 @@ __global__ void k_fused_params(FusedState* fs, double tol, int max_iter, int che
     fs->check_outer = check_outer;
     fs->tol_outer = tol_outer;
     fs->inner_total = 0;
+    fs->inner_base = 0;
     fs->outer_done = 0;
     fs->done = 0;
+}
+
+__global__ void k_fused_fault(FusedState* fs, int armed, int vec, long long idx, int bit, int iter) {
+    fs->fi_armed = armed;
+    fs->fi_vec = vec;
+    fs->fi_idx = idx;
+    fs->fi_bit = bit;
+    fs->fi_iter = iter;
+    fs->fi_fired = 0;
 }
 
 // one fused iteration (K1 + K2) on vectors of type T
@@ -853,6 +865,9 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
     if (gather_path(op) && op->pcg_mode == HEAT_PCG_FUSED) {
         k_fused_params<<<1, 1, 0, s>>>(op->fst, tol, max_iter, 0, 0.0);
         CHECK_LAUNCH();
+        k_fused_fault<<<1, 1, 0, s>>>(op->fst, op->pcg_fi.armed, op->pcg_fi.vec, op->pcg_fi.idx, op->pcg_fi.bit,
+                                      op->pcg_fi.iter);
+        CHECK_LAUNCH();
         // r = b - A x0, level R firing here if armed (src/heat_solver_het.cu:999)
         rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
         if (rc) return rc;
@@ -863,6 +878,8 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
         if (rc) return rc;
         rc = read_fused(op);
         if (rc) return rc;
+        op->pcg_fi_fired = op->pcg_fi.armed ? op->hfst->fi_fired : 0;
+        op->pcg_fi.armed = 0;
         if (iters) *iters = op->hfst->iter;
         if (hist) {
             const int rows = op->hfst->iter + 1;
@@ -871,6 +888,7 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
         }
         return HEAT_OK;
     }
+    if (op->pcg_fi.armed) return fail(HEAT_E_STATE, "PCG-state faults need the fused PCG on the colour-ordered fast path");
     k_pcg_params<<<1, 1, 0, s>>>(op->st, tol, max_iter);
     CHECK_LAUNCH();
     // r = b - A x0, with level R firing here if armed (src/heat_solver_het.cu:999)
@@ -923,6 +941,8 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     if (rc) return rc;
     hipStream_t s = op->stream;
     const Grid& g = op->g;
+    if (op->pcg_fi.armed && (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER || op->pcg_fi.bit > 31))
+        return fail(HEAT_E_STATE, "PCG-state faults of the refined solve need the fused order and BIT in 0..31");
     if (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER) {
         k_mixed_params<<<1, 1, 0, s>>>(op->st, tol, tol_inner, max_inner);
         CHECK_LAUNCH();
@@ -968,9 +988,13 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     }
     k_fused_params<<<1, 1, 0, s>>>(op->fst, tol_inner, max_inner, 1, tol);
     CHECK_LAUNCH();
+    k_fused_fault<<<1, 1, 0, s>>>(op->fst, op->pcg_fi.armed, op->pcg_fi.vec, op->pcg_fi.idx, op->pcg_fi.bit,
+                                  op->pcg_fi.iter);
+    CHECK_LAUNCH();
+    const bool detect = op->cfg.detect != 0;
     int nout = 0;
-    const void* key6[6] = {op->e32, (const void*)(intptr_t)7, (const void*)(intptr_t)op->pcg_chunk, nullptr,
-                           (const void*)(intptr_t)(100 + op->colored_impl), nullptr};
+    const void* key6[6] = {op->e32, (const void*)(intptr_t)(detect ? 9 : 7), (const void*)(intptr_t)op->pcg_chunk,
+                           nullptr, (const void*)(intptr_t)(100 + op->colored_impl), nullptr};
     for (;;) {
         double* Ax = op->vec[HEAT_VEC_AP];
         rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
@@ -981,7 +1005,7 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         rc = run_until_done(op, max_inner, [&]() {
             return launch_chunk_generic(op, key6, [&]() {
                 return launch_fused_iteration<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32, op->d32,
-                                                     op->cr32, op->ck32, false, nullptr);
+                                                     op->cr32, op->ck32, detect, nullptr);
             });
         }, &op->fst->done);
         if (rc) return rc;
@@ -992,8 +1016,30 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         CHECK_LAUNCH();
         ++nout;
     }
+    op->pcg_fi_fired = op->pcg_fi.armed ? op->hfst->fi_fired : 0;
+    op->pcg_fi.armed = 0;
     if (outer) *outer = nout;
     if (inner_iters) *inner_iters = op->hfst->inner_total;
+    return HEAT_OK;
+}
+
+int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteration) {
+    if (!op) return fail(HEAT_E_ARG, "null handle");
+    if (vec < HEAT_SEU_U || vec > HEAT_SEU_AP) return fail(HEAT_E_ARG, "VEC must be one of HEAT_SEU_*");
+    if (node < 0 || node >= op->g.nn) return fail(HEAT_E_ARG, "node %lld out of range", (long long)node);
+    if (bit < 0 || bit > 63) return fail(HEAT_E_ARG, "BIT must be in 0..63");
+    if (iteration < 0) return fail(HEAT_E_ARG, "iteration must be >= 0");
+    op->pcg_fi.armed = 1;
+    op->pcg_fi.vec = vec;
+    op->pcg_fi.idx = node;
+    op->pcg_fi.bit = bit;
+    op->pcg_fi.iter = iteration;
+    return HEAT_OK;
+}
+
+int heat_pcg_fault_fired(heat_op* op, int* fired) {
+    if (!op || !fired) return fail(HEAT_E_ARG, "null argument");
+    *fired = op->pcg_fi_fired;
     return HEAT_OK;
 }
 

@@ -47,7 +47,22 @@ struct FusedState {
     int max_iter, done;
     int outer_done, inner_total, check_outer, pad1;
     double bnorm_outer, rnorm_outer, tol_outer;
+    // single-event upset in the PCG state: at iteration fi_iter (counted over
+    // inner_base + iter), the producer of vector fi_vec stores node fi_idx with
+    // bit fi_bit flipped; sums formed in the same kernel use the clean value
+    int fi_armed, fi_vec, fi_bit, fi_iter;
+    long long fi_idx;
+    int inner_base, fi_fired;
 };
+
+enum { SEU_U = 0, SEU_R = 1, SEU_P = 2, SEU_Z = 3, SEU_AP = 4 };
+
+__device__ __forceinline__ double seu_flip(double v, int bit) {
+    return __longlong_as_double(__double_as_longlong(v) ^ (long long)(1ull << bit));
+}
+__device__ __forceinline__ float seu_flip(float v, int bit) {
+    return __int_as_float(__float_as_int(v) ^ (int)(1u << bit));
+}
 
 // 256-wide tree over one value per thread; result valid in thread 0.
 __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
@@ -204,6 +219,10 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
     const bool first = (iter == 0);
+    const bool fi_now = fs->fi_armed && (fs->inner_base + iter == fs->fi_iter);
+    const int fi_vec = fi_now ? fs->fi_vec : -1;
+    const long long fi_idx = fs->fi_idx;
+    const int fi_bit = fs->fi_bit;
     const T* __restrict__ pold = (iter & 1) ? P0 : P1;
     T* __restrict__ p = (iter & 1) ? P1 : P0;
     auto pnew = [&](size_t i) -> T {
@@ -267,7 +286,13 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
         for (int k = 0; k < XR; ++k) {
             const T v = first ? zf[k] : (T)fma(beta, (double)pf[k], (double)zf[k]);
             if (own_plane && xown[k]) {
-                p[(size_t)fplane * S + xoff[k]] = v;
+                const size_t i = (size_t)fplane * S + xoff[k];
+                if (fi_vec == SEU_P && (long long)i == fi_idx) {
+                    p[i] = seu_flip(v, fi_bit);
+                    fs->fi_fired = 1;
+                } else {
+                    p[i] = v;
+                }
                 if (DETECT) {
                     a_c += wf[k] * (double)v;
                     a_xx += (double)v * (double)v;
@@ -353,10 +378,18 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
             }
             const size_t base = (size_t)ix * S;
             const size_t me = base + (size_t)iy * nn1 + iz;
-            Ap[me] = acc;
-            if (by == 0 && ty == 0) Ap[base + iz] = T(0);
-            if (bz == 0 && tz == 0) Ap[base + (size_t)iy * nn1] = T(0);
-            if (by == 0 && bz == 0 && tid == 0) Ap[base] = T(0);
+            auto store_ap = [&](size_t i, T v) {
+                if (fi_vec == SEU_AP && (long long)i == fi_idx) {
+                    Ap[i] = seu_flip(v, fi_bit);
+                    fs->fi_fired = 1;
+                } else {
+                    Ap[i] = v;
+                }
+            };
+            store_ap(me, acc);
+            if (by == 0 && ty == 0) store_ap(base + iz, T(0));
+            if (bz == 0 && tz == 0) store_ap(base + (size_t)iy * nn1, T(0));
+            if (by == 0 && bz == 0 && tid == 0) store_ap(base, T(0));
             // p.Ap over interior nodes (Ap is 0 elsewhere, and adding a zero
             // product leaves a sum that starts at +0 unchanged)
             const bool int_node = interior && ix >= 1 && ix <= n - 1;
@@ -413,16 +446,28 @@ __global__ __launch_bounds__(NT) void k_fused_update(T* __restrict__ x, const T*
     }
     const double alpha = fs->rho[k1 & 1] / pAp;
     const T* __restrict__ p = (k1 & 1) ? P1 : P0;
+    const bool fi_now = fs->fi_armed && (fs->inner_base + k1 == fs->fi_iter);
+    const int fi_vec = fi_now ? fs->fi_vec : -1;
+    const long long fi_idx = fs->fi_idx;
+    const int fi_bit = fs->fi_bit;
     double a_rr = 0.0, a_rz = 0.0;
     const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
     for (int s = 0; s < SBS; ++s) {
         const size_t i = base + (size_t)s * NT;
         if (i < (size_t)nn) {
-            x[i] = (T)fma(alpha, (double)p[i], (double)x[i]);
+            const T xi = (T)fma(alpha, (double)p[i], (double)x[i]);
             const T ri = (T)fma(-alpha, (double)Ap[i], (double)r[i]);
             const T zi = ri / d[i];
-            r[i] = ri;
-            z[i] = zi;
+            if (fi_vec >= 0 && (long long)i == fi_idx) {
+                x[i] = fi_vec == SEU_U ? seu_flip(xi, fi_bit) : xi;
+                r[i] = fi_vec == SEU_R ? seu_flip(ri, fi_bit) : ri;
+                z[i] = fi_vec == SEU_Z ? seu_flip(zi, fi_bit) : zi;
+                if (fi_vec == SEU_U || fi_vec == SEU_R || fi_vec == SEU_Z) fs->fi_fired = 1;
+            } else {
+                x[i] = xi;
+                r[i] = ri;
+                z[i] = zi;
+            }
             a_rr += (double)ri * (double)ri;
             a_rz += (double)ri * (double)zi;
         }
@@ -475,6 +520,7 @@ __global__ __launch_bounds__(NT) void k_fused_mixed_init(
         if (blockIdx.x == 0) {
             fs->iter = 0;
             fs->done = 0;
+            fs->inner_base = fs->inner_total;
         }
     }
 }

@@ -27,6 +27,7 @@ SCATTER = {"atomic": 0, "colored": 1}
 SOURCE = {"gaussian": 0, "const": 1}
 COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3, "tdma": 4}
 TILE_SHAPES = {0: (8, 32), 1: (4, 64), 2: (16, 16)}
+SEU_VEC = {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4}
 VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "bad field", 4: "self-check failed", 5: "bad state"}
 
@@ -81,6 +82,8 @@ SIGNATURES = {
     "heat_set_tuning": (_I, [_P, _I, _I]),
     "heat_set_march_chunk": (_I, [_P, _I]),
     "heat_set_pcg_mode": (_I, [_P, _I]),
+    "heat_arm_pcg_fault": (_I, [_P, _I, C.c_int64, _I, _I]),
+    "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
@@ -201,6 +204,16 @@ class HeatOperator:
     def set_pcg_mode(self, mode):
         """"fused" (two launches per iteration) or "reference" (the reference's dot order)."""
         _check(self._lib.heat_set_pcg_mode(self.h, {"fused": 0, "reference": 1}[mode]))
+
+    def arm_pcg_fault(self, vec, node, bit, iteration):
+        """One-shot flip of bit `bit` of node `node` of PCG vector `vec`
+        ("u", "r", "p", "z", "Ap") where iteration `iteration` produces it."""
+        _check(self._lib.heat_arm_pcg_fault(self.h, SEU_VEC[vec], int(node), int(bit), int(iteration)))
+
+    def pcg_fault_fired(self):
+        f = C.c_int()
+        _check(self._lib.heat_pcg_fault_fired(self.h, C.byref(f)))
+        return bool(f.value)
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))

@@ -62,6 +62,9 @@ struct ho_op {
     double *r, *z, *p, *Ap;     /* PCG work vectors */
     int nthreads;
     int pcg_order;              /* 0: the reference's sums, 1: the GPU library's fused order */
+    /* one-shot PCG-state upset of the fused solves: vector, node, bit, iteration */
+    int sfi_armed, sfi_vec, sfi_bit, sfi_iter, sfi_fired;
+    long sfi_idx;
 };
 
 static char g_err[512];
@@ -759,6 +762,18 @@ static void apply_projected(ho_op* op, const double* x, double* y) {
     dirichlet_value(op, y, 0.0);
 }
 
+static void flip_f32(float* v, long idx, int bit) {
+    uint32_t u;
+    memcpy(&u, &v[idx], 4);
+    u ^= 1u << bit;
+    memcpy(&v[idx], &u, 4);
+}
+
+void ho_arm_pcg_fault(ho_op* op, int vec, long idx, int bit, int iter) {
+    op->sfi_armed = 1; op->sfi_vec = vec; op->sfi_idx = idx; op->sfi_bit = bit; op->sfi_iter = iter;
+}
+int ho_pcg_fault_fired(const ho_op* op) { return op->sfi_fired; }
+
 int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol, double* hist) {
     const ho_grid* g = &op->g;
     const long n = g->nn;
@@ -773,12 +788,19 @@ int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, doub
     double rho = sb_dot(n, r, z, part);
     if (hist) { hist[0] = sqrt(rr0) / bnorm; hist[1] = NAN; hist[2] = NAN; }
     int iter = 0;
+    const int fi = op->sfi_armed;
+    op->sfi_armed = 0;
+    op->sfi_fired = 0;
     if (max_iter > 0) {
         memcpy(p, z, sizeof(double) * n);
         for (;;) {
             apply_projected(op, p, Ap);
             double t[4];
             tile_sums(g, p, Ap, op->detect ? op->wA : NULL, part, t);
+            /* the apply stores p and Ap; an armed upset hits the stored copy */
+            const int hit = fi && iter == op->sfi_iter;
+            if (hit && op->sfi_vec == HO_SEU_P) { ho_flip_bit(p, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
+            if (hit && op->sfi_vec == HO_SEU_AP) { ho_flip_bit(Ap, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
             if (op->detect) {
                 double rel = fabs(t[1] - t[2]) / (op->norm_wA * sqrt(t[3]) + 1e-300);
                 if (!isfinite(rel)) rel = 1e300;
@@ -793,6 +815,11 @@ int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, doub
                 z[i] = r[i] / op->diag[i];
             }
             const double rr = sb_dot(n, r, r, part), rz = sb_dot(n, r, z, part);
+            if (hit && (op->sfi_vec == HO_SEU_U || op->sfi_vec == HO_SEU_R || op->sfi_vec == HO_SEU_Z)) {
+                double* v = op->sfi_vec == HO_SEU_U ? x : op->sfi_vec == HO_SEU_R ? r : z;
+                ho_flip_bit(v, op->sfi_idx, op->sfi_bit);
+                op->sfi_fired = 1;
+            }
             ++iter;
             const double rnorm = sqrt(rr);
             const int conv = (rnorm / bnorm < tol);
@@ -822,6 +849,9 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
     double* part = (double*)malloc(sizeof(double) * fused_part_len(g));
     for (long i = 0; i < n; ++i) d[i] = (float)op->diag[i];
     int nout = 0, total = 0;
+    const int fi = op->sfi_armed;
+    op->sfi_armed = 0;
+    op->sfi_fired = 0;
     for (;;) {
         apply_projected(op, x, r64);
         for (long i = 0; i < n; ++i) {
@@ -845,7 +875,16 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
                 ho_apply_operator_f32(op, p, ap, op->dt, 1);
                 for (long i = 0; i < n; ++i) { dp[i] = p[i]; dap[i] = ap[i]; }
                 double t[4];
-                tile_sums(g, dp, dap, NULL, part, t);
+                tile_sums(g, dp, dap, op->detect ? op->wA : NULL, part, t);
+                if (op->detect) {
+                    double rel = fabs(t[1] - t[2]) / (op->norm_wA * sqrt(t[3]) + 1e-300);
+                    if (!isfinite(rel)) rel = 1e300;
+                    op->abft_step_max = fmax(op->abft_step_max, rel);
+                    op->abft_step_calls++;
+                }
+                const int hit = fi && total == op->sfi_iter;
+                if (hit && op->sfi_vec == HO_SEU_P) { flip_f32(p, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
+                if (hit && op->sfi_vec == HO_SEU_AP) { flip_f32(ap, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
                 if (fabs(t[0]) < 1e-30) break;
                 const double alpha = rho / t[0];
                 for (long i = 0; i < n; ++i) {
@@ -855,6 +894,11 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
                     dr[i] = r[i]; dz[i] = z[i];
                 }
                 const double rr2 = sb_dot(n, dr, dr, part), rz2 = sb_dot(n, dr, dz, part);
+                if (hit && (op->sfi_vec == HO_SEU_U || op->sfi_vec == HO_SEU_R || op->sfi_vec == HO_SEU_Z)) {
+                    float* v = op->sfi_vec == HO_SEU_U ? e : op->sfi_vec == HO_SEU_R ? r : z;
+                    flip_f32(v, op->sfi_idx, op->sfi_bit);
+                    op->sfi_fired = 1;
+                }
                 ++it;
                 ++total;
                 const int conv = sqrt(rr2) / bnorm < tol_inner;

@@ -68,6 +68,17 @@ int    ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double 
  * two-launch order (tile / super-block partials, 256-wide trees). */
 int    ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol,
                           double* hist);
+/* One-shot upset of the PCG state for the next fused solve: at iteration
+ * ITER the stored copy of VEC (HO_SEU_*) at IDX gets bit BIT flipped after
+ * the sums of the kernel that produced it (the refined solve: its FP32 inner
+ * state, iterations counted over all corrections). */
+#define HO_SEU_U  0
+#define HO_SEU_R  1
+#define HO_SEU_P  2
+#define HO_SEU_Z  3
+#define HO_SEU_AP 4
+void   ho_arm_pcg_fault(ho_op* op, int vec, long idx, int bit, int iter);
+int    ho_pcg_fault_fired(const ho_op* op);
 /* The fused order's sums alone: super-block x.y, and the tile sums
  * [p.Ap (interior), 1_int.Ap, w.p, p.p]. */
 double ho_fused_sb_dot(ho_op* op, const double* x, const double* y);

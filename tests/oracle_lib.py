@@ -37,6 +37,8 @@ _SIG = {
     "ho_pcg_solve_fused": (_I, [_P, _P, _P, _I, _D, _P]),
     "ho_pcg_solve_mixed_fused": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
     "ho_fused_sb_dot": (_D, [_P, _P, _P]),
+    "ho_arm_pcg_fault": (None, [_P, _I, _L, _I, _I]),
+    "ho_pcg_fault_fired": (_I, [_P]),
     "ho_fused_tile_sums": (None, [_P, _P, _P, _P, _P]),
     "ho_initial_condition": (None, [_P, _P, _I, _D, _D]),
 }
@@ -153,6 +155,13 @@ class OracleOp:
         lib().ho_pcg_solve_mixed_fused(self.h, _p(b), _p(x), max_outer, max_inner, tol, tol_inner,
                                        C.byref(o), C.byref(i))
         return o.value, i.value, x
+
+    def arm_pcg_fault(self, vec, node, bit, iteration):
+        lib().ho_arm_pcg_fault(self.h, {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4}[vec], int(node), int(bit),
+                               int(iteration))
+
+    def pcg_fault_fired(self):
+        return bool(lib().ho_pcg_fault_fired(self.h))
 
     def fused_sb_dot(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float64)

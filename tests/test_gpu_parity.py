@@ -406,3 +406,87 @@ def test_autotune_keeps_bits():
     op.upload("p", x)
     op.apply_operator("p", "Ap", dt)
     assert np.array_equal(op.download("Ap"), orc.apply(x, dt))
+
+
+SEU_VECS = ["u", "r", "p", "z", "Ap"]
+
+
+@pytest.mark.parametrize("vec", SEU_VECS)
+def test_pcg_state_upset_fp64_bitwise(vec):
+    """Upsets of the FP64 PCG state (fused order) bit-identical to the
+    oracle's restatement: iterations, iterate (NaN/Inf included), whether the
+    fault fired, and the D1 record."""
+    n, dt = 12, 5e-4
+    k, rhoc = make_field(n, seed=50)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=True, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True, source="const", source_value=1.0)
+    T0 = orc.initial_condition("sine", 0.0, 1.0)
+    b = orc.build_rhs(T0)
+    rng = np.random.default_rng(7)
+    for bit, it_site in [(3, 0), (51, 5), (62, 2), (63, 7), (55, 30)]:
+        node = int(rng.integers(op.nn))
+        print(f"[case] seu fp64 vec={vec} node={node} bit={bit} iter={it_site}", flush=True)
+        op.upload("b", b)
+        op.upload("T", T0)
+        op.abft_take()
+        orc.abft_take()
+        op.arm_pcg_fault(vec, node, bit, it_site)
+        it = op.pcg_solve("b", "T", 200, 1e-10)
+        fired = op.pcg_fault_fired()
+        orc.arm_pcg_fault(vec, node, bit, it_site)
+        ito, xo, _ = orc.pcg_fused(b, T0, 200, 1e-10)
+        assert (it, fired) == (ito, orc.pcg_fault_fired())
+        assert np.array_equal(op.download("T"), xo, equal_nan=True)
+        d1, calls = op.abft_take()
+        d1o, callso = orc.abft_take()
+        assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
+
+
+@pytest.mark.parametrize("vec", SEU_VECS)
+def test_pcg_state_upset_refined_bitwise(vec):
+    """Upsets of the FP32 inner state of the refined solve, with the FP32 D1
+    check on: outer/inner counts, iterate and D1 record bit-identical to the
+    oracle."""
+    n, dt = 10, 1e-4
+    k, rhoc = make_field(n, seed=51)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=True, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True, source="const", source_value=1.0)
+    b = orc.build_rhs(np.zeros(op.nn))
+    rng = np.random.default_rng(8)
+    for bit, it_site in [(5, 0), (22, 4), (29, 9), (30, 2), (31, 6)]:
+        node = int(rng.integers(op.nn))
+        print(f"[case] seu fp32 vec={vec} node={node} bit={bit} iter={it_site}", flush=True)
+        op.upload("b", b)
+        op.upload("T", np.zeros(op.nn))
+        op.abft_take()
+        orc.abft_take()
+        op.arm_pcg_fault(vec, node, bit, it_site)
+        outer, inner = op.pcg_solve_mixed("b", "T", 8, 60, 1e-10, 1e-4)
+        fired = op.pcg_fault_fired()
+        orc.arm_pcg_fault(vec, node, bit, it_site)
+        oo, io, xo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 8, 60, 1e-10, 1e-4)
+        assert (outer, inner, fired) == (oo, io, orc.pcg_fault_fired())
+        assert np.array_equal(op.download("T"), xo, equal_nan=True)
+        d1, calls = op.abft_take()
+        d1o, callso = orc.abft_take()
+        assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
+
+
+def test_pcg_state_upset_argument_checks():
+    n = 4
+    k, rhoc = make_field(n)
+    h = heat()
+    op = h.HeatOperator(n, 1e-3, k, rhoc)
+    with pytest.raises(h.HeatError):
+        op.arm_pcg_fault("r", op.nn, 3, 0)
+    with pytest.raises(h.HeatError):
+        op.arm_pcg_fault("r", 0, 64, 0)
+    op.arm_pcg_fault("r", 0, 40, 0)
+    op.upload("T", np.zeros(op.nn))
+    op.build_rhs("T", "b")
+    with pytest.raises(h.HeatError):
+        op.pcg_solve_mixed("b", "T", 5, 50, 1e-10, 1e-4)     # bit 40 on FP32 words
+    op.set_pcg_mode("reference")
+    op.arm_pcg_fault("r", 0, 3, 0)
+    with pytest.raises(h.HeatError):
+        op.pcg_solve("b", "T", 50, 1e-8)

@@ -336,3 +336,105 @@ def test_fused_mixed_reaches_fp64_tolerance():
     r = b - op.apply(x, op.dt)
     assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
     assert 2 <= outer <= 6 and inner > outer
+
+
+def fused_solve_with_fault(op, b, vec, node, bit, it, tol=1e-12):
+    op.arm_pcg_fault(vec, node, bit, it)
+    out = op.pcg_fused(b, np.zeros(op.nn), 500, tol)
+    assert op.pcg_fault_fired()
+    return out
+
+
+@pytest.mark.parametrize("vec", ["u", "r", "p", "z", "Ap"])
+def test_pcg_state_upset_follows_cg_algebra(vec):
+    """An upset of the stored PCG state breaks (or keeps) the invariant
+    r = b - A x in the way CG's algebra fixes, independently of how the
+    oracle is written:
+      u:  x never feeds back, so the fault moves x at that node only;
+      r:  the recursive residual converges, the true one tends to -delta e_i;
+      p:  x moves along p' while r follows A p: true residual -> -alpha delta A e_i;
+      Ap: r follows Ap': true residual -> alpha delta e_i;
+      z:  x and r stay consistent: the solve still reaches the solution."""
+    n = 9
+    op, b = lognormal_problem(n, seed=11)
+    op = OracleOp(n, op.dt, op.k, op.rhoc, kernel="struct", source="const", source_value=1.0)
+    tol = 1e-12
+    it0, x0, h0 = op.pcg_fused(b, np.zeros(op.nn), 500, tol)
+    j = 4
+    i = int(np.flatnonzero(interior_mask(n))[len(np.flatnonzero(interior_mask(n))) // 2])
+    # the stored value the fault hits, from a clean run stopped at the same point
+    itj, xj, hj = op.pcg_fused(b, np.zeros(op.nn), j + 1, 1e-300)
+    it, x, h = fused_solve_with_fault(op, b, vec, i, 51, j, tol)
+    bn = np.linalg.norm(b)
+    res = b - op.apply(x, op.dt)
+    if vec == "u":
+        d = x - x0
+        others = np.delete(d, i)
+        assert it == it0 and np.max(np.abs(others)) <= 1e-12 * np.max(np.abs(x0))
+        v = xj[i]          # x after update j
+        assert abs(d[i] - (flip_value(v, 51) - v)) <= 1e-12 * abs(v) + 1e-300
+    elif vec == "z":
+        assert np.linalg.norm(res) <= 10 * tol * bn
+        assert np.max(np.abs(x - x0)) <= 1e-8 * np.max(np.abs(x0))
+    else:
+        e_i = np.zeros(op.nn)
+        e_i[i] = 1.0
+        alpha = h[j + 1, 1]
+        # delta of the flipped stored value, from the clean state at iteration j
+        state = clean_state_at(op, b, j)
+        v = state[vec][i]
+        delta = flip_value(v, 51) - v
+        if vec == "r":
+            expect = -delta * e_i
+        elif vec == "Ap":
+            expect = alpha * delta * e_i
+        else:  # p
+            expect = -alpha * delta * op.apply(e_i, op.dt)
+        assert np.linalg.norm(res - expect) <= 1e-6 * np.linalg.norm(expect) + 10 * tol * bn
+
+
+def flip_value(v, bit):
+    u = np.array([v], dtype=np.float64).view(np.uint64)
+    u ^= np.uint64(1) << np.uint64(bit)
+    return float(u.view(np.float64)[0])
+
+
+def clean_state_at(op, b, j):
+    """Clean r, p, Ap of iteration j (p_j, A p_j; r after update j), from the
+    textbook recurrences in numpy (a different summation order: agreement to
+    rounding is all that is needed for the delta of bit 51)."""
+    d = op.vector("diag")
+    x = np.zeros(op.nn)
+    r = b - op.apply(x, op.dt)
+    z = r / d
+    p = z.copy()
+    rho = r @ z
+    for k in range(j + 1):
+        Ap = op.apply(p, op.dt)
+        alpha = rho / (p @ Ap)
+        if k == j:
+            pj, Apj = p.copy(), Ap.copy()
+        x = x + alpha * p
+        r = r - alpha * Ap
+        z = r / d
+        rho_new = r @ z
+        if k < j:
+            p = z + rho_new / rho * p
+        rho = rho_new
+    return {"r": r, "p": pj, "Ap": Apj}
+
+
+def test_pcg_state_upset_refined_solve_self_corrects():
+    """The refined solve's FP64 outer residual sees an upset of the FP32 inner
+    state and corrects it: the final true residual still meets the outer
+    tolerance."""
+    op, b = lognormal_problem(11)
+    o0, i0, x0 = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    node = int(np.flatnonzero(interior_mask(11))[100])
+    for vec in ("r", "p", "Ap", "u"):
+        op.arm_pcg_fault(vec, node, 29, 3)
+        o, i, x = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+        assert op.pcg_fault_fired()
+        r = b - op.apply(x, op.dt)
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10, vec
+        assert o >= 1 and i > 0
