@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="run this many axpy passes (2 node vectors read, 1 written) first: the known-bytes "
@@ -218,6 +219,11 @@ def main():
         res["speedup_fp32_refined"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
         res["speedup_fused_vs_reference_order"] = round(res["fp64_reference_order"]["ms"] / res["fp64"]["ms"], 3)
         result["time_to_1e-8"] = res
+
+    if not a.no_ensemble:
+        from mnhack8_amd import ensemble
+        log("ensemble: 96 members x 20 steps, batched and one after the other")
+        result["ensemble"] = ensemble.run(96, 20, 1e-8, log=log)
 
     if not a.no_cpu:
         try:

@@ -150,8 +150,10 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
 /* FP32-refined solve (new; no reference counterpart): FP64 defect correction
  * x += e, r = b - A x (FP64 operator) around an FP32 Jacobi-PCG on A e = r
  * (FP32 operator and vectors, FP64 alpha/beta inner products) to the relative
- * inner tolerance tol_inner, until ||b - A x|| / ||b|| < tol.  *outer and
- * *inner_iters receive the outer and the total inner iteration counts. */
+ * inner tolerance tol_inner, until ||b - A x|| / ||b|| < tol, max_outer
+ * corrections, or (fused order) an FP64 residual that is not finite.
+ * *outer and *inner_iters receive the outer and the total inner iteration
+ * counts. */
 int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
                          int max_inner, double tol, double tol_inner,
                          int* outer, int* inner_iters);
@@ -187,6 +189,28 @@ int heat_set_pcg_mode(heat_op* op, int mode);
 int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteration);
 /* 1 if the fault armed for the last solve fired (its iteration was reached). */
 int heat_pcg_fault_fired(heat_op* op, int* fired);
+
+/* Batched ensembles (new; replaces the reference's one process per ensemble
+ * member, scripts/ensemble/run_ensemble.py:53-104, each running
+ * src/heat_solver_het.cu's main).  COUNT independent solves, each with its own
+ * configuration and fields (fast kernel, colour-ordered scatter, no
+ * detectors), advanced together: every PCG iteration of all solves is one
+ * pair of launches over all their tiles, and a converged or masked solve's
+ * blocks return at once.  Each solve's iterates are bit-identical to the
+ * same solve run alone in the fused order. */
+typedef struct heat_batch heat_batch;
+int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
+                     const double* const* rhoc, heat_batch** out);
+int heat_batch_cleanup(heat_batch* b);
+/* The handle of solve I (owned by the batch): upload T^0, download T, etc. */
+int heat_batch_instance(heat_batch* b, int i, heat_op** op);
+/* Per-solve mask (nonzero = active); masked solves keep their T. */
+int heat_batch_set_active(heat_batch* b, const int* active);
+int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph);
+/* NSTEPS backward-Euler steps of every active solve: b = M T + dt F, then
+ * Jacobi-PCG from x0 = T (src/heat_solver_het.cu:1308-1311).  ITERS
+ * (optional) receives NSTEPS x COUNT iteration counts (-1: masked). */
+int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* iters);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);

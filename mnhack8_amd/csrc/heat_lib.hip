@@ -46,6 +46,17 @@ int fail(int code, const char* fmt, ...) {
 
 #define CHECK_LAUNCH() HIP_TRY(hipGetLastError())
 
+// HIP_TRY with a clean-up action in place of the plain return
+#define HIP_TRY_OR(call, action)                                                        \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fail(HEAT_E_HIP, "%s:%d: %s: %s", __FILE__, __LINE__, #call,                \
+                 hipGetErrorString(e_));                                                \
+            action;                                                                     \
+        }                                                                               \
+    } while (0)
+
 inline unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -412,26 +423,37 @@ __global__ void k_fused_fault(FusedState* fs, int armed, int vec, long long idx,
     fs->fi_fired = 0;
 }
 
-// one fused iteration (K1 + K2) on vectors of type T
+// The fused-kernel view of a handle's solve on vectors of type T.
 template <typename T>
-int launch_fused_iteration(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, const T* d, const T* cr,
-                           const T* ck, bool detect, double* hist) {
+FusedView<T> make_view(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, T* b, const T* d, const T* cr,
+                       const T* ck, double* hist) {
+    FusedView<T> v;
+    v.x = x; v.P0 = p0; v.P1 = p1; v.r = r; v.z = z; v.Ap = Ap; v.b = b;
+    v.d = d; v.cr = cr; v.ck = ck;
+    v.w = op->vec[HEAT_VEC_WA];
+    v.F = op->vec[HEAT_VEC_F];
+    v.g = op->g;
+    v.sp = op->fsp; v.sp_stride = op->nsb; v.nsb = op->nsb;
+    v.tp = op->ftp; v.tp_stride = op->fntiles; v.ntiles = op->fntiles;
+    v.nby = op->fnby; v.nbz = op->fnbz;
+    v.fs = op->fst; v.st = op->st;
+    v.nwA = op->nwA; v.dt = op->cfg.dt;
+    v.hist = hist;
+    v.add_source = !(op->cfg.verify || op->cfg.no_source);
+    v.active = 1;
+    return v;
+}
+
+// one fused iteration (K1 + K2) of a view
+template <typename T>
+int launch_fused_iteration(heat_op* op, const FusedView<T>& v, bool detect) {
     hipStream_t s = op->stream;
-    const unsigned grid1 = (unsigned)(((op->fntiles + NXCD - 1) / NXCD) * NXCD);
-    const double* w = op->vec[HEAT_VEC_WA];
-    if (detect)
-        k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(p0, p1, z, Ap, cr, ck, op->g, w, op->fsp, op->nsb, op->ftp,
-                                                          op->fnby, op->fnbz, op->fntiles, op->fst, hist);
-    else
-        k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(p0, p1, z, Ap, cr, ck, op->g, w, op->fsp, op->nsb, op->ftp,
-                                                           op->fnby, op->fnbz, op->fntiles, op->fst, hist);
+    const unsigned grid1 = (unsigned)(((v.ntiles + NXCD - 1) / NXCD) * NXCD);
+    if (detect) k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
+    else k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
     CHECK_LAUNCH();
-    if (detect)
-        k_fused_update<T, true><<<op->nsb, NT, 0, s>>>(x, p0, p1, r, Ap, d, z, op->g.nn, op->ftp, op->fntiles, op->fsp,
-                                                      op->nsb, op->fst, op->st, op->nwA);
-    else
-        k_fused_update<T, false><<<op->nsb, NT, 0, s>>>(x, p0, p1, r, Ap, d, z, op->g.nn, op->ftp, op->fntiles, op->fsp,
-                                                       op->nsb, op->fst, op->st, op->nwA);
+    if (detect) k_fused_update<T, true><<<v.nsb, NT, 0, s>>>(v);
+    else k_fused_update<T, false><<<v.nsb, NT, 0, s>>>(v);
     CHECK_LAUNCH();
     return HEAT_OK;
 }
@@ -470,9 +492,10 @@ int launch_chunk(heat_op* op, double* x, bool detect) {
                            (const void*)(intptr_t)(op->march_cx + 100 * op->tile_shape)};
     if (op->pcg_mode == HEAT_PCG_FUSED) {
         return launch_chunk_generic(op, key6, [&]() {
-            return launch_fused_iteration<double>(op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R],
-                                                  op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
-                                                  op->vec[HEAT_VEC_DIAG], op->cr, op->ck, detect, op->cur_hist);
+            const FusedView<double> v = make_view<double>(op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R],
+                                                          op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP], nullptr,
+                                                          op->vec[HEAT_VEC_DIAG], op->cr, op->ck, op->cur_hist);
+            return launch_fused_iteration<double>(op, v, detect);
         });
     }
     return launch_chunk_generic(op, key6, [&]() { return launch_iteration(op, x, detect); });
@@ -871,8 +894,9 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
         // r = b - A x0, level R firing here if armed (src/heat_solver_het.cu:999)
         rc = apply_impl(op, x, Ap, op->cfg.dt, true, true, detect ? EPI_D1 : EPI_NONE, 0);
         if (rc) return rc;
-        k_fused_init<double><<<op->nsb, NT, 0, s>>>(b, Ap, op->vec[HEAT_VEC_DIAG], op->vec[HEAT_VEC_R],
-                                                     op->vec[HEAT_VEC_Z], op->g.nn, op->fsp, op->nsb, op->fst);
+        k_fused_init<double><<<op->nsb, NT, 0, s>>>(make_view<double>(
+            op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], Ap, const_cast<double*>(b),
+            op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr));
         CHECK_LAUNCH();
         rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); }, &op->fst->done);
         if (rc) return rc;
@@ -1004,8 +1028,9 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         CHECK_LAUNCH();
         rc = run_until_done(op, max_inner, [&]() {
             return launch_chunk_generic(op, key6, [&]() {
-                return launch_fused_iteration<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32, op->d32,
-                                                     op->cr32, op->ck32, detect, nullptr);
+                const FusedView<float> v = make_view<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32,
+                                                            op->ap32, nullptr, op->d32, op->cr32, op->ck32, nullptr);
+                return launch_fused_iteration<float>(op, v, detect);
             });
         }, &op->fst->done);
         if (rc) return rc;
@@ -1196,6 +1221,233 @@ int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph) {
     if (!op || pcg_chunk < 1 || pcg_chunk > 256) return fail(HEAT_E_ARG, "pcg_chunk must be in [1, 256]");
     op->pcg_chunk = pcg_chunk;
     op->use_graph = use_graph ? 1 : 0;
+    return HEAT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched ensembles: many independent solves advanced together.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_batch_iters(const FusedView<double>* __restrict__ views, int count, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = views[i].active ? views[i].fs->iter : -1;
+}
+
+}  // namespace
+
+struct heat_batch {
+    std::vector<heat_op*> ops;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int* hflag = nullptr;                    // pinned mailboxes, one per chunk in flight
+    int* dflag = nullptr;                    // device: all solves done
+    int* diters = nullptr;
+    int* hiters = nullptr;                   // pinned
+    std::vector<FusedView<double>> views;
+    FusedView<double>* dviews = nullptr;
+    std::vector<int> tile_off, sb_off, node_off;
+    int *d_tile_off = nullptr, *d_sb_off = nullptr, *d_node_off = nullptr;
+    double *sp = nullptr, *tp = nullptr;
+    int pcg_chunk = 8, use_graph = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_chunk = 0;
+};
+
+namespace {
+
+void batch_free(heat_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    if (b->hflag) (void)hipHostFree(b->hflag);
+    if (b->hiters) (void)hipHostFree(b->hiters);
+    (void)hipFree(b->dflag); (void)hipFree(b->diters); (void)hipFree(b->dviews);
+    (void)hipFree(b->d_tile_off); (void)hipFree(b->d_sb_off); (void)hipFree(b->d_node_off);
+    (void)hipFree(b->sp); (void)hipFree(b->tp);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (heat_op* op : b->ops) heat_cleanup(op);
+}
+
+int batch_upload_views(heat_batch* b) {
+    HIP_TRY(hipMemcpyAsync(b->dviews, b->views.data(), sizeof(FusedView<double>) * b->views.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return HEAT_OK;
+}
+
+int batch_chunk(heat_batch* b) {
+    const int count = (int)b->ops.size();
+    hipStream_t s = b->stream;
+    const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
+    const unsigned grid1 = (unsigned)(((total_tiles + NXCD - 1) / NXCD) * NXCD);
+    auto body = [&]() -> int {
+        for (int i = 0; i < b->pcg_chunk; ++i) {
+            k_batch_apply<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
+            CHECK_LAUNCH();
+            k_batch_update<false><<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
+            CHECK_LAUNCH();
+        }
+        k_batch_all_done<<<1, NT, 0, s>>>(b->dviews, count, b->dflag);
+        CHECK_LAUNCH();
+        return HEAT_OK;
+    };
+    if (!b->use_graph) return body();
+    if (!b->gexec || b->graph_chunk != b->pcg_chunk) {
+        if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        b->graph = graph;
+        HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        b->graph_chunk = b->pcg_chunk;
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, s));
+    return HEAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k, const double* const* rhoc,
+                     heat_batch** out) {
+    if (!out || !cfgs || !k || !rhoc || count < 1) return fail(HEAT_E_ARG, "bad batch arguments");
+    *out = nullptr;
+    heat_batch* b = new heat_batch();
+    auto bail = [&](int rc) { batch_free(b); delete b; return rc; };
+    for (int i = 0; i < count; ++i) {
+        const heat_config& c = cfgs[i];
+        if (c.kernel != HEAT_KERNEL_FAST || c.scatter != HEAT_SCATTER_COLORED)
+            return bail(fail(HEAT_E_ARG, "batch solve %d: the batch runs the colour-ordered fast path only", i));
+        if (c.detect) return bail(fail(HEAT_E_ARG, "batch solve %d: the batch runs without detectors", i));
+        heat_op* op = nullptr;
+        const int rc = heat_setup(&c, k[i], rhoc[i], &op);
+        if (rc) {
+            char msg[512];
+            snprintf(msg, sizeof msg, "%s", heat_last_error());
+            return bail(fail(rc, "batch solve %d: %s", i, msg));
+        }
+        b->ops.push_back(op);
+        HIP_TRY_OR(hipStreamSynchronize(op->stream), return bail(HEAT_E_HIP));
+    }
+    b->tile_off.assign(count + 1, 0);
+    b->sb_off.assign(count + 1, 0);
+    b->node_off.assign(count + 1, 0);
+    long long nodes = 0;
+    for (int i = 0; i < count; ++i) {
+        b->tile_off[i + 1] = b->tile_off[i] + b->ops[i]->fntiles;
+        b->sb_off[i + 1] = b->sb_off[i] + b->ops[i]->nsb;
+        nodes += b->ops[i]->g.nn;
+        if (nodes > (1LL << 30)) return bail(fail(HEAT_E_ARG, "batch too large: %lld nodes", nodes));
+        b->node_off[i + 1] = (int)nodes;
+    }
+    const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
+    HIP_TRY_OR(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipEventCreateWithFlags(&b->ev[0], hipEventDisableTiming), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipEventCreateWithFlags(&b->ev[1], hipEventDisableTiming), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipHostMalloc(&b->hflag, 2 * sizeof(int), hipHostMallocDefault), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipHostMalloc(&b->hiters, count * sizeof(int), hipHostMallocDefault), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->dflag, sizeof(int)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->diters, count * sizeof(int)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->dviews, count * sizeof(FusedView<double>)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->d_tile_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->d_sb_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->d_node_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->sp, 3 * sizeof(double) * (size_t)total_sb), return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMalloc(&b->tp, 4 * sizeof(double) * (size_t)total_tiles), return bail(HEAT_E_HIP));
+    for (int i = 0; i < count; ++i) {
+        heat_op* op = b->ops[i];
+        FusedView<double> v = make_view<double>(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_P], op->p2,
+                                                op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
+                                                op->vec[HEAT_VEC_B], op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr);
+        v.sp = b->sp + b->sb_off[i];
+        v.sp_stride = total_sb;
+        v.tp = b->tp + b->tile_off[i];
+        v.tp_stride = total_tiles;
+        b->views.push_back(v);
+    }
+    HIP_TRY_OR(hipMemcpy(b->d_tile_off, b->tile_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
+               return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMemcpy(b->d_sb_off, b->sb_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
+               return bail(HEAT_E_HIP));
+    HIP_TRY_OR(hipMemcpy(b->d_node_off, b->node_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
+               return bail(HEAT_E_HIP));
+    const int rc = batch_upload_views(b);
+    if (rc) return bail(rc);
+    *out = b;
+    return HEAT_OK;
+}
+
+int heat_batch_cleanup(heat_batch* b) {
+    if (!b) return HEAT_OK;
+    batch_free(b);
+    delete b;
+    return HEAT_OK;
+}
+
+int heat_batch_instance(heat_batch* b, int i, heat_op** op) {
+    if (!b || !op || i < 0 || i >= (int)b->ops.size()) return fail(HEAT_E_ARG, "bad batch instance");
+    *op = b->ops[i];
+    return HEAT_OK;
+}
+
+int heat_batch_set_active(heat_batch* b, const int* active) {
+    if (!b || !active) return fail(HEAT_E_ARG, "null argument");
+    for (size_t i = 0; i < b->views.size(); ++i) b->views[i].active = active[i] ? 1 : 0;
+    return batch_upload_views(b);
+}
+
+int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph) {
+    if (!b || pcg_chunk < 1 || pcg_chunk > 64) return fail(HEAT_E_ARG, "pcg_chunk must be in 1..64");
+    b->pcg_chunk = pcg_chunk;
+    b->use_graph = use_graph ? 1 : 0;
+    return HEAT_OK;
+}
+
+int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* iters) {
+    if (!b || nsteps < 0 || max_iter < 0) return fail(HEAT_E_ARG, "bad batch step arguments");
+    const int count = (int)b->ops.size();
+    hipStream_t s = b->stream;
+    for (heat_op* op : b->ops) HIP_TRY(hipStreamSynchronize(op->stream));
+    const int total_sb = b->sb_off[count];
+    const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
+    for (int step = 0; step < nsteps; ++step) {
+        k_batch_rhs<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->d_node_off, count);
+        CHECK_LAUNCH();
+        k_batch_params<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, tol, max_iter);
+        CHECK_LAUNCH();
+        k_batch_residual<<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
+        CHECK_LAUNCH();
+        // chunks until every solve is done; the flag of chunk k is read while chunk k+1 runs
+        const int max_chunks = max_iter / b->pcg_chunk + 3;
+        int rc = batch_chunk(b);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(&b->hflag[0], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(b->ev[0], s));
+        for (int kk = 1;; ++kk) {
+            if (kk > max_chunks) return fail(HEAT_E_STATE, "batch PCG did not report completion");
+            rc = batch_chunk(b);
+            if (rc) return rc;
+            HIP_TRY(hipMemcpyAsync(&b->hflag[kk & 1], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipEventRecord(b->ev[kk & 1], s));
+            HIP_TRY(hipEventSynchronize(b->ev[(kk - 1) & 1]));
+            if (b->hflag[(kk - 1) & 1]) break;
+        }
+        k_batch_iters<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, b->diters);
+        CHECK_LAUNCH();
+        HIP_TRY(hipMemcpyAsync(b->hiters, b->diters, count * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (iters) memcpy(iters + (size_t)step * count, b->hiters, count * sizeof(int));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
     return HEAT_OK;
 }
 

@@ -64,6 +64,27 @@ __device__ __forceinline__ float seu_flip(float v, int bit) {
     return __int_as_float(__float_as_int(v) ^ (int)(1u << bit));
 }
 
+// One solve's vectors, mesh and reduction slots.  A single solve launches
+// with its view by value; a batch of solves passes a device table of views,
+// and each block finds its solve from its tile or super-block index.  The
+// partial arrays of a batch are shared: row q of solve i lives at
+// sp[q * sp_stride + j] with sp pointing at the solve's first super-block.
+template <typename T>
+struct FusedView {
+    T *x, *P0, *P1, *r, *z, *Ap, *b;
+    const T *d, *cr, *ck;
+    const double *w, *F;
+    Grid g;
+    double* sp;                 // super-block partials (rows of sp_stride)
+    double* tp;                 // tile partials (rows of tp_stride)
+    int sp_stride, nsb, tp_stride, ntiles, nby, nbz;
+    FusedState* fs;
+    DevState* st;
+    double nwA, dt;             // ||w_A||; time step of the right-hand side
+    double* hist;
+    int add_source, active;     // b = M T (+ dt F); a masked solve does no work
+};
+
 // 256-wide tree over one value per thread; result valid in thread 0.
 __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
     const int t = threadIdx.x;
@@ -80,12 +101,12 @@ __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
 // final stage over `count` partials of K sums (stride count), all threads get the results
 template <int K>
 __device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* __restrict__ part, int count,
-                                                double out[K]) {
+                                                int stride, double out[K]) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double a = 0.0;
-        for (int i = t; i < count; i += NT) a += part[(size_t)k * count + i];
+        for (int i = t; i < count; i += NT) a += part[(size_t)k * stride + i];
         sd[k][t] = a;
     }
     __syncthreads();
@@ -102,16 +123,18 @@ __device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* 
     __syncthreads();
 }
 
-// r = b - Ap; z = r/d; super-block partials of b.b, r.r, r.z.
+// r = b - Ap; z = r/d; super-block partials of b.b, r.r, r.z (super-block j).
 template <typename T>
-__global__ __launch_bounds__(NT) void k_fused_init(const T* __restrict__ b, const T* __restrict__ Ap,
-                                                   const T* __restrict__ d, T* __restrict__ r,
-                                                   T* __restrict__ z, int nn,
-                                                   double* __restrict__ sp, int nsb,
-                                                   FusedState* fs) {
+__device__ __forceinline__ void fused_init_body(const FusedView<T>& v, int j) {
     __shared__ double sd[3][NT];
+    const T* __restrict__ b = v.b;
+    const T* __restrict__ Ap = v.Ap;
+    const T* __restrict__ d = v.d;
+    T* __restrict__ r = v.r;
+    T* __restrict__ z = v.z;
+    const int nn = v.g.nn;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-    const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
+    const size_t base = (size_t)j * SB + threadIdx.x;
     for (int s = 0; s < SBS; ++s) {
         const size_t i = base + (size_t)s * NT;
         if (i < (size_t)nn) {
@@ -127,24 +150,40 @@ __global__ __launch_bounds__(NT) void k_fused_init(const T* __restrict__ b, cons
     }
     const double s0 = tree256(sd, 0, a0), s1 = tree256(sd, 1, a1), s2 = tree256(sd, 2, a2);
     if (threadIdx.x == 0) {
-        sp[blockIdx.x] = s0;
-        sp[nsb + blockIdx.x] = s1;
-        sp[2 * nsb + blockIdx.x] = s2;
-        if (blockIdx.x == 0) {
-            fs->iter = 0;
-            fs->done = 0;
+        v.sp[j] = s0;
+        v.sp[v.sp_stride + j] = s1;
+        v.sp[2 * v.sp_stride + j] = s2;
+        if (j == 0) {
+            v.fs->iter = 0;
+            v.fs->done = v.active ? 0 : 1;
         }
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fused_init(FusedView<T> v) {
+    fused_init_body<T>(v, blockIdx.x);
+}
+
 // K1.  sp: K2 partials (r.r, r.z) or, at iteration 0, init partials (b.b, r.r, r.z).
 // tp: output partials of this launch [p.Ap | 1.Ap | w.p | p.p] x ntiles.
+// (tile b of the solve's ntiles)
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
-    T* __restrict__ P0, T* __restrict__ P1, const T* __restrict__ z, T* __restrict__ Ap, const T* __restrict__ cr,
-    const T* __restrict__ ck, Grid g, const double* __restrict__ w, const double* __restrict__ sp,
-    int nsb, double* __restrict__ tp, int nby, int nbz, int ntiles, FusedState* fs,
-    double* __restrict__ hist) {
+__device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
+    T* __restrict__ P0 = v.P0;
+    T* __restrict__ P1 = v.P1;
+    const T* __restrict__ z = v.z;
+    T* __restrict__ Ap = v.Ap;
+    const T* __restrict__ cr = v.cr;
+    const T* __restrict__ ck = v.ck;
+    const Grid g = v.g;
+    const double* __restrict__ w = v.w;
+    const double* __restrict__ sp = v.sp;
+    const int nsb = v.nsb, sp_stride = v.sp_stride;
+    double* __restrict__ tp = v.tp;
+    const int nby = v.nby, nbz = v.nbz, tp_stride = v.tp_stride;
+    FusedState* fs = v.fs;
+    double* __restrict__ hist = v.hist;
     constexpr int TY = FTY, TZ = FTZ, CX = FCX, NTH = TY * TZ;
     constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
     constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
@@ -160,23 +199,26 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
     if (iter == 0) {
         double o[5];
         if (fs->check_outer) {
-            reduce_partials<5>(sd, sp, nsb, o);
+            reduce_partials<5>(sd, sp, nsb, sp_stride, o);
             double bo = sqrt(o[3]);
             if (bo < 1e-30) bo = 1.0;
-            const bool outer_done = sqrt(o[4]) / bo < fs->tol_outer;
-            if (threadIdx.x == 0 && blockIdx.x == 0) {
+            // 1: converged; 2: the FP64 residual is not finite (stop: no
+            // correction can recover it); 0: run the inner solve
+            const double rn = sqrt(o[4]);
+            const int outer_done = (rn / bo < fs->tol_outer) ? 1 : (isfinite(rn) ? 0 : 2);
+            if (threadIdx.x == 0 && b == 0) {
                 fs->bnorm_outer = bo;
-                fs->rnorm_outer = sqrt(o[4]);
+                fs->rnorm_outer = rn;
                 fs->outer_done = outer_done;
                 if (outer_done) fs->done = 1;
             }
             if (outer_done) return;
         } else {
-            reduce_partials<3>(sd, sp, nsb, o);
+            reduce_partials<3>(sd, sp, nsb, sp_stride, o);
         }
         double bnorm = sqrt(o[0]);
         if (bnorm < 1e-30) bnorm = 1.0;
-        if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (threadIdx.x == 0 && b == 0) {
             fs->bnorm = bnorm;
             fs->rho[0] = o[2];
             fs->k1_iter = 0;
@@ -186,12 +228,12 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
         if (fs->max_iter <= 0) return;
     } else {
         double o[2];
-        reduce_partials<2>(sd, sp, nsb, o);
+        reduce_partials<2>(sd, sp, nsb, sp_stride, o);
         const double bnorm = fs->bnorm;
         const double rnorm = sqrt(o[0]);
         const bool conv = (rnorm / bnorm < fs->tol);
         if (!conv) beta = o[1] / fs->rho[(iter - 1) & 1];
-        if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (threadIdx.x == 0 && b == 0) {
             if (hist) {
                 double* h = hist + 3 * iter;
                 h[0] = rnorm / bnorm; h[1] = fs->alpha; h[2] = conv ? 0.0 : beta;
@@ -202,8 +244,6 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
         if (conv || iter >= fs->max_iter) return;      // uniform over the grid
     }
     // ---- tile apply on p = z + beta p (iteration 0: p = z)
-    const int b = xcd_slab_block(ntiles);
-    if (b >= ntiles) return;
     const int bz = b % nbz;
     const int by = (b / nbz) % nby;
     const int chunk = b / (nbz * nby);
@@ -417,31 +457,48 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(
     if (DETECT) { t1 = tree256(sd, 1, a_s); t2 = tree256(sd, 2, a_c); t3 = tree256(sd, 3, a_xx); }
     if (tid == 0) {
         tp[b] = t0;
-        if (DETECT) { tp[ntiles + b] = t1; tp[2 * ntiles + b] = t2; tp[3 * ntiles + b] = t3; }
+        if (DETECT) { tp[tp_stride + b] = t1; tp[2 * tp_stride + b] = t2; tp[3 * tp_stride + b] = t3; }
     }
 }
 
-// K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(NT) void k_fused_update(T* __restrict__ x, const T* __restrict__ P0,
-                                                     const T* __restrict__ P1,
-                                                     T* __restrict__ r, const T* __restrict__ Ap,
-                                                     const T* __restrict__ d, T* __restrict__ z,
-                                                     int nn, const double* __restrict__ tp,
-                                                     int ntiles, double* __restrict__ sp, int nsb,
-                                                     FusedState* fs, DevState* st, double nwA) {
+__global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(FusedView<T> v) {
+    const int b = xcd_slab_block(v.ntiles);
+    if (b >= v.ntiles) return;
+    fused_apply_body<T, DETECT>(v, b);
+}
+
+// K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
+// (super-block j of the solve's nsb)
+template <typename T, bool DETECT>
+__device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) {
+    T* __restrict__ x = v.x;
+    const T* __restrict__ P0 = v.P0;
+    const T* __restrict__ P1 = v.P1;
+    T* __restrict__ r = v.r;
+    const T* __restrict__ Ap = v.Ap;
+    const T* __restrict__ d = v.d;
+    T* __restrict__ z = v.z;
+    const int nn = v.g.nn;
+    const double* __restrict__ tp = v.tp;
+    const int ntiles = v.ntiles, tp_stride = v.tp_stride;
+    double* __restrict__ sp = v.sp;
+    const int sp_stride = v.sp_stride;
+    FusedState* fs = v.fs;
+    DevState* st = v.st;
+    const double nwA = v.nwA;
     __shared__ double sd[4][NT];
     if (fs->done) return;
     const int k1 = fs->k1_iter;
     double o[4];
-    if (DETECT) reduce_partials<4>(sd, tp, ntiles, o);
-    else reduce_partials<1>(sd, tp, ntiles, o);
+    if (DETECT) reduce_partials<4>(sd, tp, ntiles, tp_stride, o);
+    else reduce_partials<1>(sd, tp, ntiles, tp_stride, o);
     const double pAp = o[0];
     // D1 of this iteration's apply is recorded before the breakdown check, as
     // the reference's apply_operator does (:908-919 run before :1013)
-    if (DETECT && threadIdx.x == 0 && blockIdx.x == 0) abft_record(st, o[1], o[2], o[3], nwA);
+    if (DETECT && threadIdx.x == 0 && j == 0) abft_record(st, o[1], o[2], o[3], nwA);
     if (fabs(pAp) < 1e-30) {                         // :1013
-        if (threadIdx.x == 0 && blockIdx.x == 0) fs->done = 1;
+        if (threadIdx.x == 0 && j == 0) fs->done = 1;
         return;
     }
     const double alpha = fs->rho[k1 & 1] / pAp;
@@ -451,7 +508,7 @@ __global__ __launch_bounds__(NT) void k_fused_update(T* __restrict__ x, const T*
     const long long fi_idx = fs->fi_idx;
     const int fi_bit = fs->fi_bit;
     double a_rr = 0.0, a_rz = 0.0;
-    const size_t base = (size_t)blockIdx.x * SB + threadIdx.x;
+    const size_t base = (size_t)j * SB + threadIdx.x;
     for (int s = 0; s < SBS; ++s) {
         const size_t i = base + (size_t)s * NT;
         if (i < (size_t)nn) {
@@ -474,14 +531,19 @@ __global__ __launch_bounds__(NT) void k_fused_update(T* __restrict__ x, const T*
     }
     const double s0 = tree256(sd, 0, a_rr), s1 = tree256(sd, 1, a_rz);
     if (threadIdx.x == 0) {
-        sp[blockIdx.x] = s0;
-        sp[nsb + blockIdx.x] = s1;
-        if (blockIdx.x == 0) {
+        sp[j] = s0;
+        sp[sp_stride + j] = s1;
+        if (j == 0) {
             fs->alpha = alpha;
             fs->iter = k1 + 1;
             fs->inner_total += 1;
         }
     }
+}
+
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(NT) void k_fused_update(FusedView<T> v) {
+    fused_update_body<T, DETECT>(v, blockIdx.x);
 }
 
 // FP64 outer residual of the refined solve from Ax (the projected FP64 apply):
@@ -523,6 +585,169 @@ __global__ __launch_bounds__(NT) void k_fused_mixed_init(
             fs->inner_base = fs->inner_total;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent solves (an ensemble of meshes and fields).  Each
+// solve keeps the tile and super-block decomposition it has on its own, so
+// its sums, and therefore its iterates, are bit-identical to a single solve;
+// the batch only packs all solves' tiles (super-blocks) into one grid.  A
+// solve that has converged, or is masked, sets its done flag and its blocks
+// return at once.
+// ---------------------------------------------------------------------------
+
+// i with off[i] <= gidx < off[i + 1] (off ascending, off[0] = 0)
+__device__ __forceinline__ int find_solve(const int* __restrict__ off, int count, int gidx) {
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= gidx) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Colour-ordered row sum at any node, boundary included: the contributions of
+// the elements that exist around node (ix, iy, iz), in colour order from 0
+// (the colour sweep's sum, src/heat_solver_het.cu:896-899).  MASS: dt_eff = 0.
+template <typename T, bool MASS>
+__device__ __forceinline__ T node_apply_any(const T* __restrict__ x, const T* __restrict__ cr,
+                                           const T* __restrict__ ck, const Grid& g, int ix, int iy, int iz) {
+    const int n = g.n, nsq = n * n;
+    const int px = ix & 1, py = iy & 1, pz = iz & 1;
+    const int sx = px ? -1 : 1, sy = py ? -1 : 1, sz = pz ? -1 : 1;
+    T m[3][3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int X = ix + (a - 1) * sx, Y = iy + (b - 1) * sy, Z = iz + (c - 1) * sz;
+                const bool ok = X >= 0 && X <= n && Y >= 0 && Y <= n && Z >= 0 && Z <= n;
+                m[a][b][c] = ok ? x[(size_t)X * g.sx + (size_t)Y * g.sy + Z] : T(0);
+            }
+    T acc = T(0);
+#pragma unroll
+    for (int col = 0; col < 8; ++col) {
+        const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
+        const int ex = ix - (cx ^ px), ey = iy - (cy ^ py), ez = iz - (cz ^ pz);
+        if (ex < 0 || ex >= n || ey < 0 || ey >= n || ez < 0 || ez >= n) continue;
+        const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
+        const size_t e = (size_t)ex * nsq + (size_t)ey * n + ez;
+        const T S0 = m[1][1][1];
+        const T S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
+        const T S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
+        const T S3 = m[ox][oy][oz];
+        acc += struct_row(S0, S1, S2, S3, cr[e], MASS ? T(0) : ck[e]);
+    }
+    return acc;
+}
+
+// b = P(M T) [+ dt F] for every node of every active solve: build_rhs's
+// apply_operator(T, b, 0, false) projects by default
+// (src/heat_solver_het.cu:885, 963-971); one thread per node of the batch.
+__global__ __launch_bounds__(NT) void k_batch_rhs(const FusedView<double>* __restrict__ views,
+                                                  const int* __restrict__ node_off, int count) {
+    const int gid = blockIdx.x * NT + threadIdx.x;
+    if (gid >= node_off[count]) return;
+    const int i = find_solve(node_off, count, gid);
+    const FusedView<double>& v = views[i];
+    if (!v.active) return;
+    const int nid = gid - node_off[i];
+    int ix, iy, iz;
+    node_coords(v.g, nid, ix, iy, iz);
+    const double acc = node_is_boundary(v.g, ix, iy, iz) ? 0.0
+                                                         : node_apply_any<double, true>(v.x, v.cr, v.ck, v.g, ix, iy, iz);
+    v.b[nid] = v.add_source ? fma(v.dt, v.F[nid], acc) : acc;
+}
+
+// PCG start of every solve: r = b - P A x, z = r/d, super-block partials of
+// b.b, r.r, r.z (the order of k_fused_init); grid = all super-blocks.
+__global__ __launch_bounds__(NT) void k_batch_residual(const FusedView<double>* __restrict__ views,
+                                                       const int* __restrict__ sb_off, int count) {
+    __shared__ double sd[3][NT];
+    if ((int)blockIdx.x >= sb_off[count]) return;
+    const int i = find_solve(sb_off, count, blockIdx.x);
+    const FusedView<double>& v = views[i];
+    const int j = blockIdx.x - sb_off[i];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    const size_t base = (size_t)j * SB + threadIdx.x;
+    if (v.active) {
+        for (int s = 0; s < SBS; ++s) {
+            const size_t n = base + (size_t)s * NT;
+            if (n < (size_t)v.g.nn) {
+                int ix, iy, iz;
+                node_coords(v.g, (int)n, ix, iy, iz);
+                const double ax = node_is_boundary(v.g, ix, iy, iz)
+                                      ? 0.0 : node_apply_any<double, false>(v.x, v.cr, v.ck, v.g, ix, iy, iz);
+                const double bi = v.b[n];
+                const double ri = fma(-1.0, ax, bi);
+                const double zi = ri / v.d[n];
+                v.r[n] = ri;
+                v.z[n] = zi;
+                a0 += bi * bi;
+                a1 += ri * ri;
+                a2 += ri * zi;
+            }
+        }
+    }
+    const double s0 = tree256(sd, 0, a0), s1 = tree256(sd, 1, a1), s2 = tree256(sd, 2, a2);
+    if (threadIdx.x == 0) {
+        v.sp[j] = s0;
+        v.sp[v.sp_stride + j] = s1;
+        v.sp[2 * v.sp_stride + j] = s2;
+        if (j == 0) {
+            v.fs->iter = 0;
+            v.fs->done = v.active ? 0 : 1;
+        }
+    }
+}
+
+template <bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ) void k_batch_apply(const FusedView<double>* __restrict__ views,
+                                                          const int* __restrict__ tile_off, int count) {
+    const int total = tile_off[count];
+    const int gb = xcd_slab_block(total);
+    if (gb >= total) return;
+    const int i = find_solve(tile_off, count, gb);
+    fused_apply_body<double, DETECT>(views[i], gb - tile_off[i]);
+}
+
+template <bool DETECT>
+__global__ __launch_bounds__(NT) void k_batch_update(const FusedView<double>* __restrict__ views,
+                                                     const int* __restrict__ sb_off, int count) {
+    if ((int)blockIdx.x >= sb_off[count]) return;
+    const int i = find_solve(sb_off, count, blockIdx.x);
+    fused_update_body<double, DETECT>(views[i], blockIdx.x - sb_off[i]);
+}
+
+// per-solve PCG parameters of a batch step (one thread per solve)
+__global__ void k_batch_params(const FusedView<double>* __restrict__ views, int count, double tol,
+                               int max_iter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    FusedState* fs = views[i].fs;
+    fs->tol = tol;
+    fs->max_iter = max_iter;
+    fs->check_outer = 0;
+    fs->inner_total = 0;
+    fs->inner_base = 0;
+    fs->outer_done = 0;
+    fs->fi_armed = 0;
+    fs->fi_fired = 0;
+    fs->done = views[i].active ? 0 : 1;
+}
+
+// *flag = 1 when every solve of the batch is done
+__global__ void k_batch_all_done(const FusedView<double>* __restrict__ views, int count, int* flag) {
+    __shared__ int open;
+    if (threadIdx.x == 0) open = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        if (!views[i].fs->done) atomicOr(&open, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *flag = open ? 0 : 1;
 }
 
 }  // namespace heat

@@ -83,6 +83,12 @@ SIGNATURES = {
     "heat_set_march_chunk": (_I, [_P, _I]),
     "heat_set_pcg_mode": (_I, [_P, _I]),
     "heat_arm_pcg_fault": (_I, [_P, _I, C.c_int64, _I, _I]),
+    "heat_batch_setup": (_I, [_I, _P, _P, _P, C.POINTER(_P)]),
+    "heat_batch_cleanup": (_I, [_P]),
+    "heat_batch_instance": (_I, [_P, _I, C.POINTER(_P)]),
+    "heat_batch_set_active": (_I, [_P, _P]),
+    "heat_batch_set_tuning": (_I, [_P, _I, _I]),
+    "heat_batch_step": (_I, [_P, _I, _I, _D, _P]),
     "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
@@ -117,9 +123,17 @@ class HeatOperator:
     """One mesh on the current HIP device (HeatOperator::setup)."""
 
     def __init__(self, n, dt, k, rhoc, kernel="fast", scatter="colored", verify=False,
-                 no_source=False, source="gaussian", source_value=0.0, detect=False):
+                 no_source=False, source="gaussian", source_value=0.0, detect=False, _borrow=None):
         lib = load_library()
         self._lib = lib
+        if _borrow is not None:                 # a handle owned by a HeatBatch
+            self.h, self._owned = _borrow, False
+            self.n, self.dt, self.detect = n, float(dt), bool(detect)
+            nn, ne = _I64(), _I64()
+            _check(lib.heat_sizes(self.h, C.byref(nn), C.byref(ne)))
+            self.nn, self.ne = nn.value, ne.value
+            return
+        self._owned = True
         k = np.ascontiguousarray(k, dtype=np.float64).ravel()
         rhoc = np.ascontiguousarray(rhoc, dtype=np.float64).ravel()
         if k.size != n ** 3 or rhoc.size != n ** 3:
@@ -137,7 +151,8 @@ class HeatOperator:
     # -- lifetime ----------------------------------------------------------
     def cleanup(self):
         if getattr(self, "h", None):
-            _check(self._lib.heat_cleanup(self.h))
+            if getattr(self, "_owned", True):
+                _check(self._lib.heat_cleanup(self.h))
             self.h = None
 
     close = cleanup
@@ -273,6 +288,12 @@ class HeatOperator:
     def flip_bit(self, v, idx, bit):
         _check(self._lib.heat_flip_bit(self.h, self.ptr(v), int(idx), int(bit)))
 
+    def abft_threshold(self):
+        """The D1 threshold chk_tol of the FP64 operator checksum."""
+        t = C.c_double()
+        _check(self._lib.heat_abft_threshold(self.h, C.byref(t)))
+        return t.value
+
     def abft_take(self):
         m, c = C.c_double(), _I64()
         _check(self._lib.heat_abft_take(self.h, C.byref(m), C.byref(c)))
@@ -305,6 +326,71 @@ class HeatOperator:
         _check(self._lib.heat_pcg_solve_mixed(self.h, self.ptr(b), self.ptr(x), int(max_outer), int(max_inner),
                                               float(tol), float(tol_inner), C.byref(outer), C.byref(inner)))
         return outer.value, inner.value
+
+
+class HeatBatch:
+    """An ensemble of independent solves advanced together (heat_batch_*).
+
+    members: list of dicts with keys n, dt, k, rhoc and optionally source,
+    source_value, no_source, verify.  ops[i] is a HeatOperator view of solve
+    i (upload T^0, download T)."""
+
+    def __init__(self, members):
+        lib = load_library()
+        self._lib = lib
+        count = len(members)
+        cfgs = (HeatConfig * count)()
+        self._keep = []
+        kp = (C.c_void_p * count)()
+        rp = (C.c_void_p * count)()
+        for i, m in enumerate(members):
+            n = int(m["n"])
+            k = np.ascontiguousarray(m["k"], dtype=np.float64).ravel()
+            r = np.ascontiguousarray(m["rhoc"], dtype=np.float64).ravel()
+            if k.size != n ** 3 or r.size != n ** 3:
+                raise ValueError(f"member {i}: expected {n ** 3} element values")
+            self._keep += [k, r]
+            kp[i], rp[i] = k.ctypes.data, r.ctypes.data
+            cfgs[i] = HeatConfig(n, float(m["dt"]), KERNEL["fast"], SCATTER["colored"], int(m.get("verify", 0)),
+                                 int(m.get("no_source", 0)), SOURCE[m.get("source", "gaussian")],
+                                 float(m.get("source_value", 0.0)), 0)
+        h = C.c_void_p()
+        _check(lib.heat_batch_setup(count, cfgs, kp, rp, C.byref(h)))
+        self.h = h
+        self.count = count
+        self.ops = []
+        for i, m in enumerate(members):
+            oh = C.c_void_p()
+            _check(lib.heat_batch_instance(h, i, C.byref(oh)))
+            self.ops.append(HeatOperator(int(m["n"]), m["dt"], None, None, _borrow=oh))
+        self._keep = []
+
+    def set_active(self, mask):
+        a = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))
+        _check(self._lib.heat_batch_set_active(self.h, a.ctypes.data))
+
+    def set_tuning(self, pcg_chunk=8, use_graph=True):
+        _check(self._lib.heat_batch_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))
+
+    def step(self, nsteps=1, max_iter=500, tol=1e-8):
+        """nsteps backward-Euler steps of every active solve; returns the
+        (nsteps, count) iteration counts (-1: masked)."""
+        it = np.zeros((max(nsteps, 1), self.count), dtype=np.int32)
+        _check(self._lib.heat_batch_step(self.h, int(nsteps), int(max_iter), float(tol), it.ctypes.data))
+        return it[:nsteps]
+
+    def cleanup(self):
+        if getattr(self, "h", None):
+            for op in self.ops:
+                op.h = None
+            _check(self._lib.heat_batch_cleanup(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
 
 
 def pcg_solve(op: HeatOperator, b, x, max_iter=500, tol=1e-8, history=False):

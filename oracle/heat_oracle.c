@@ -576,6 +576,14 @@ int ho_get_vector(const ho_op* op, int which, double* out) {
     memcpy(out, src, sizeof(double) * op->g.nn);
     return 0;
 }
+/* Replace the source vector F (which = 1) or the diagonal (which = 0): lets a
+ * check feed the oracle the GPU's own setup vectors. */
+int ho_set_vector(ho_op* op, int which, const double* in) {
+    double* dst = which == 0 ? op->diag : which == 1 ? op->F : NULL;
+    if (!dst) return 1;
+    memcpy(dst, in, sizeof(double) * op->g.nn);
+    return 0;
+}
 void ho_get_ref_matrices(const ho_op* op, double* Mh, double* Kh) {
     memcpy(Mh, op->Mh, sizeof(op->Mh)); memcpy(Kh, op->Kh, sizeof(op->Kh));
 }
@@ -837,7 +845,8 @@ int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, doub
 /* FP32-refined solve in the same order: FP64 outer residual r = b - A x, an
  * FP32 Jacobi-PCG on A e = r (FP32 vectors and operator; alpha, beta and the
  * inner products in FP64) to a relative tolerance tol_inner, x += e, until
- * ||b - A x|| / ||b|| < tol or max_outer corrections. */
+ * ||b - A x|| / ||b|| < tol, the residual is not finite, or max_outer
+ * corrections. */
 int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_outer, int max_inner,
                              double tol, double tol_inner, int* outer, int* inner_total) {
     const ho_grid* g = &op->g;
@@ -864,7 +873,8 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
         const double rr = sb_dot(n, dr, dr, part), rz = sb_dot(n, dr, dz, part);
         double bo = sqrt(sb_dot(n, b, b, part));
         if (bo < 1e-30) bo = 1.0;
-        const int outer_done = sqrt(sb_dot(n, r64, r64, part)) / bo < tol;
+        const double rn = sqrt(sb_dot(n, r64, r64, part));
+        const int outer_done = (rn / bo < tol) ? 1 : (isfinite(rn) ? 0 : 2);   /* 2: not finite, stop */
         if (!outer_done && max_inner > 0) {
             double bnorm = sqrt(rr);
             if (bnorm < 1e-30) bnorm = 1.0;

@@ -40,6 +40,7 @@ long   ho_nn(const ho_op* op);
 long   ho_ne(const ho_op* op);
 /* copies of internal vectors: 0 diag, 1 F, 2 wM, 3 wA, 4 ones_int */
 int    ho_get_vector(const ho_op* op, int which, double* out);
+int    ho_set_vector(ho_op* op, int which, const double* in);   /* 0 diag, 1 F */
 void   ho_get_ref_matrices(const ho_op* op, double* Mh, double* Kh);
 void   ho_get_stats(const ho_op* op, double* out4); /* kmin kmax rhocmin rhocmax */
 

@@ -37,6 +37,7 @@ _SIG = {
     "ho_pcg_solve_fused": (_I, [_P, _P, _P, _I, _D, _P]),
     "ho_pcg_solve_mixed_fused": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
     "ho_fused_sb_dot": (_D, [_P, _P, _P]),
+    "ho_set_vector": (_I, [_P, _I, _P]),
     "ho_arm_pcg_fault": (None, [_P, _I, _L, _I, _I]),
     "ho_pcg_fault_fired": (_I, [_P]),
     "ho_fused_tile_sums": (None, [_P, _P, _P, _P, _P]),
@@ -83,6 +84,12 @@ class OracleOp:
         if getattr(self, "h", None):
             lib().ho_destroy(self.h)
             self.h = None
+
+    def set_vector(self, which, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.size == self.nn
+        if lib().ho_set_vector(self.h, {"diag": 0, "F": 1}[which], _p(v)):
+            raise KeyError(which)
 
     def vector(self, which):
         out = np.empty(self.nn)

@@ -103,3 +103,20 @@ def test_gen_field_cli(tmp_path):
     subprocess.run(["python", os.path.join(ROOT, "mnhack8_amd", "gen_field.py"), "--n", "4", "--model",
                     "homogeneous", "--k", "2.5", "--out", out], check=True, capture_output=True)
     assert np.all(np.fromfile(out + "_k.bin") == 2.5) and np.fromfile(out + "_rhoc.bin").size == 64
+
+
+def test_seu_campaign_summary_logic():
+    """Classification bookkeeping of the upset campaign (host logic only)."""
+    from mnhack8_amd import seu_campaign as sc
+    rows = []
+    for prec in ("fp64", "fp32_refined"):
+        for i, cls in enumerate(["benign"] * 6 + ["detected"] * 3 + ["silent"]):
+            rows.append({"precision": prec, "vec": sc.VECS[i % 5], "class": cls, "fired": 1,
+                         "d1_fired": int(cls == "detected"), "d2_fired": 0, "nonfinite": 0,
+                         "stuck": 0, "detected_without_S": int(cls == "detected")})
+    s = sc.summarize(rows, {})
+    assert s["fp64"]["benign"] == 6 and s["fp64"]["detected"] == 3 and s["fp64"]["silent"] == 1
+    assert s["fp64"]["coverage"] == 0.75 and s["coverage_difference_pp"] == 0.0
+    lo, hi = s["fp64"]["coverage_ci95"]
+    assert 0.2 < lo < 0.75 < hi <= 1.0
+    assert sc.wilson(0, 0)[0] != sc.wilson(0, 0)[0]       # NaN for no harmful sites

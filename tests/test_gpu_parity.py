@@ -490,3 +490,108 @@ def test_pcg_state_upset_argument_checks():
     op.arm_pcg_fault("r", 0, 3, 0)
     with pytest.raises(h.HeatError):
         op.pcg_solve("b", "T", 50, 1e-8)
+
+
+def test_seu_campaign_small():
+    """The campaign harness end to end on a small mesh: every armed fault
+    fires, the clean solves meet their tolerance, the FP32 D1 threshold sits
+    above the FP32 floor and below the signal of a gross upset."""
+    from mnhack8_amd import seu_campaign as sc
+    print("[case] seu campaign n=16", flush=True)
+    c = sc.Campaign(n=16, seed=3)
+    clean = c.clean()
+    assert clean["fp64"]["true_residual"] < 1e-9 and clean["fp32_refined"]["true_residual"] < 1e-9
+    assert clean["thresholds"]["fp64"] == 1e-10
+    assert 0 < clean["thresholds"]["fp32_refined"] < 1e-3
+    rows = c.run(sites=40, seed=3)
+    assert all(r["fired"] for r in rows)
+    s = sc.summarize(rows, clean)
+    for prec in ("fp64", "fp32_refined"):
+        assert s[prec]["sites"] == 40
+        assert s[prec]["benign"] + s[prec]["detected"] + s[prec]["silent"] == 40
+
+
+BATCH_MEMBERS = [
+    # (n, dt, seed, source, source_value, ic)
+    (3, 1e-3, 60, "const", 1.0, "zero"),
+    (8, 5e-4, 61, "gaussian", 0.0, "sine"),
+    (9, 1e-3, 62, "const", 2.0, "sine"),
+    (16, 1e-3, 63, "const", 1.0, "zero"),
+    (5, 2e-3, 64, "gaussian", 0.0, "gaussian"),
+    (33, 1e-3, 65, "const", 1.0, "sine"),
+    (12, 1e-3, 66, "const", 1.0, "zero"),
+]
+
+
+def _batch_and_oracles(members):
+    h = heat()
+    ms, orcs, T0s = [], [], []
+    for n, dt, seed, src, sv, ic in members:
+        k, rhoc = make_field(n, seed=seed)
+        ms.append({"n": n, "dt": dt, "k": k, "rhoc": rhoc, "source": src, "source_value": sv})
+        orc = OracleOp(n, dt, k, rhoc, kernel="struct", source=src, source_value=sv)
+        orcs.append(orc)
+        T0s.append(orc.initial_condition(ic, 0.0, 1.0))
+    batch = h.HeatBatch(ms)
+    for op, orc, T0 in zip(batch.ops, orcs, T0s):
+        op.upload("T", T0)
+        orc.set_vector("F", op.download("F"))      # the same source vector on both sides
+        assert np.array_equal(op.download("diag"), orc.vector("diag"))
+    return batch, orcs, T0s
+
+
+def _oracle_steps(orc, T, nsteps, max_iter, tol):
+    its = []
+    for _ in range(nsteps):
+        b = orc.build_rhs(T)
+        it, T, _ = orc.pcg_fused(b, T, max_iter, tol)
+        its.append(it)
+    return its, T
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_bitwise_with_oracle(graph):
+    """An ensemble of differently sized and driven solves advanced together:
+    per-member iteration counts and T after each step bit-identical to the
+    oracle's fused solve of that member alone."""
+    print(f"[case] batch graph={graph}", flush=True)
+    batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS)
+    batch.set_tuning(4, graph)
+    nsteps, max_iter, tol = 3, 300, 1e-10
+    its = batch.step(nsteps, max_iter, tol)
+    for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
+        ito, To = _oracle_steps(orc, T0, nsteps, max_iter, tol)
+        print(f"  member {i} n={op.n}: iterations {list(its[:, i])} (oracle {ito})", flush=True)
+        assert list(its[:, i]) == ito
+        assert np.array_equal(op.download("T"), To)
+    batch.cleanup()
+
+
+def test_batch_mask():
+    """A masked member keeps its T and reports -1; the others go on exactly
+    as without it."""
+    print("[case] batch mask", flush=True)
+    batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS[:4])
+    mask = [1, 0, 1, 1]
+    batch.set_active(mask)
+    its = batch.step(2, 300, 1e-10)
+    for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
+        if not mask[i]:
+            assert (its[:, i] == -1).all()
+            assert np.array_equal(op.download("T"), T0)
+            continue
+        ito, To = _oracle_steps(orc, T0, 2, 300, 1e-10)
+        assert list(its[:, i]) == ito and np.array_equal(op.download("T"), To)
+    # unmask: member 1 now advances from its untouched T0
+    batch.set_active([1, 1, 1, 1])
+    its = batch.step(1, 300, 1e-10)
+    ito, To = _oracle_steps(orcs[1], T0s[1], 1, 300, 1e-10)
+    assert its[0, 1] == ito[0] and np.array_equal(batch.ops[1].download("T"), To)
+    batch.cleanup()
+
+
+def test_batch_rejects_unsupported_members():
+    h = heat()
+    k, rhoc = make_field(4)
+    with pytest.raises(h.HeatError):
+        h.HeatBatch([{"n": 4, "dt": 1e-3, "k": k, "rhoc": rhoc}, {"n": 4, "dt": 1e-3, "k": -k, "rhoc": rhoc}])
