@@ -214,7 +214,9 @@ def test_v3_injection_runs(tmp_path, ln32, level, bit, solver):
     the element arithmetic's rounding.  With the reference's 8x8 form it does
     (E_max 2.7e-16 recorded); with the closed-form rows of the HIP solver it
     is absorbed at this site (E = 0, the oracle's closed-form restatement
-    gives the same), so for that case only the class is compared."""
+    gives the same), and in the fused summation order it is absorbed with
+    either form, so for that case only the reference-order oracle checks
+    that E moves."""
     N = 32
     target = validations.node_index(16, 16, 16, N) if level == 1 else validations.elem_index(16, 16, 16, N)
     run_oracle([N, 0.02, 5e-4, ln32, tmp_path, "--tag", "fi", "--snap", "1", "--kernel", "fast", "--scatter",
@@ -229,7 +231,7 @@ def test_v3_injection_runs(tmp_path, ln32, level, bit, solver):
     tr = list(csv.DictReader(open(tmp_path / "fi_trace_fi.csv")))
     E = np.array([float(r["E_fault"]) for r in tr])
     assert (E[:10] == 0).all()
-    if not (solver.startswith("hip") and level == 2 and bit == 0):
+    if not (solver != "oracle" and level == 2 and bit == 0):
         assert E[10] > 0 or not np.isfinite(E[10])
 
 
