@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-mrhs", action="store_true", help="skip the 16-RHS N=320 apply measurement")
     ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="run this many axpy passes (2 node vectors read, 1 written) first: the known-bytes "
@@ -224,6 +225,11 @@ def main():
         from mnhack8_amd import ensemble
         log("ensemble: 96 members x 20 steps, batched and one after the other")
         result["ensemble"] = ensemble.run(96, 20, 1e-8, log=log)
+
+    if not a.no_mrhs:
+        from mnhack8_amd import mrhs
+        log("16-RHS apply at N=320 (matrix-core contraction)")
+        result["multi_rhs"] = mrhs.bench_apply(320, log=log)
 
     if not a.no_cpu:
         try:

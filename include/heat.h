@@ -190,6 +190,18 @@ int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteratio
 /* 1 if the fault armed for the last solve fired (its iteration was reached). */
 int heat_pcg_fault_fired(heat_op* op, int* fired);
 
+/* Sixteen right-hand sides at once (new; the reference applies its operator
+ * to one vector, src/heat_solver_het.cu:885-921).  X and Y are FP32 device
+ * arrays of nn x 16, node-major (X[node * 16 + r]).  Y_r = P (rho*c M +
+ * dt_eff kappa K) X_r with FP32 coefficients (dt_eff = 0 or the handle's
+ * dt); the element contraction runs on the matrix cores.  Each node's eight
+ * element contributions are summed in a fixed order (element plane below,
+ * then above, each in (y, z) parity order), not the colour order, so results
+ * agree with the single-vector FP32 apply to FP32 rounding, not bitwise. */
+int heat_apply_mrhs(heat_op* op, const float* X, float* Y, double dt_eff);
+/* Mean time of REPS back-to-back multi-RHS applies (A, not M) on the handle's stream. */
+int heat_time_mrhs(heat_op* op, const float* X, float* Y, int reps, double* avg_ms);
+
 /* Batched ensembles (new; replaces the reference's one process per ensemble
  * member, scripts/ensemble/run_ensemble.py:53-104, each running
  * src/heat_solver_het.cu's main).  COUNT independent solves, each with its own

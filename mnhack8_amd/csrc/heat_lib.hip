@@ -19,6 +19,7 @@
 #include "heat.h"
 #include "heat_kernels.h"
 #include "heat_pcg_fused.h"
+#include "heat_mrhs.h"
 
 using namespace heat;
 
@@ -1066,6 +1067,45 @@ int heat_pcg_fault_fired(heat_op* op, int* fired) {
     if (!op || !fired) return fail(HEAT_E_ARG, "null argument");
     *fired = op->pcg_fi_fired;
     return HEAT_OK;
+}
+
+int heat_apply_mrhs(heat_op* op, const float* X, float* Y, double dt_eff) {
+    if (!op || !X || !Y) return fail(HEAT_E_ARG, "null argument");
+    if (!gather_path(op)) return fail(HEAT_E_STATE, "the multi-RHS apply needs --kernel fast --scatter colored");
+    const bool mass = (dt_eff == 0.0);
+    if (!mass && dt_eff != op->cfg.dt)
+        return fail(HEAT_E_ARG, "dt_eff must be 0 or the handle's dt (%g), got %g", op->cfg.dt, dt_eff);
+    int rc = ensure_mixed(op);
+    if (rc) return rc;
+    const int n = op->g.n;
+    const int nby = (n + 1 + MTY - 1) / MTY, nbz = (n + 1 + MTZ - 1) / MTZ, nch = (n + 1 + MCX - 1) / MCX;
+    const long long nt = (long long)nby * nbz * nch;
+    if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
+    const int ntiles = (int)nt;
+    const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
+    if (mass) k_mrhs_apply<true><<<grid, 256, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    else k_mrhs_apply<false><<<grid, 256, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+int heat_time_mrhs(heat_op* op, const float* X, float* Y, int reps, double* avg_ms) {
+    if (!op || !X || !Y || reps < 1 || !avg_ms) return fail(HEAT_E_ARG, "bad timing request");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = heat_apply_mrhs(op, X, Y, op->cfg.dt);          // warm-up
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(e0, op->stream));
+    for (int i = 0; i < reps && rc == HEAT_OK; ++i) rc = heat_apply_mrhs(op, X, Y, op->cfg.dt);
+    HIP_TRY(hipEventRecord(e1, op->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    return rc;
 }
 
 int heat_set_pcg_mode(heat_op* op, int mode) {

@@ -1,0 +1,199 @@
+// heat_mrhs.h -- sixteen right-hand sides at once: Y_r = P (rho*c M + dt kappa K) X_r
+// in FP32, with the element contraction on the matrix cores.
+//
+// Per element e the sixteen columns of X_e (8 nodes x 16 right-hand sides)
+// are contracted with one constant 16 x 8 integer matrix whose rows are, for
+// each local node i, the Hamming-distance weights of the closed-form element
+// rows (src/heat_solver_het.cu:677-705):
+//   P_i = sum_j {8,4,2,1}[d(i,j)] x_j      (rho*c M row = m3 P_i)
+//   Q_i = sum_j {-4,0,1,1}[d(i,j)] x_j     (kappa K row = k2 Q_i)
+// so y_i = cr_e P_i + ck_e Q_i with cr = rho*c m3 and ck = dt kappa k2, the
+// same coefficients and final fused multiply-add as the single-vector
+// kernels.  The 16 x 8 by 8 x 16 product is two v_mfma_f32_16x16x4f32.
+// Rows are ordered (P_a, P_b, Q_a, Q_b) per pair of nodes a = (0, y, z),
+// b = (1, y, z), so each lane holds both terms of its two outputs and no
+// lane exchange is needed.
+//
+// Blocks own a TY x TZ tile of node columns and walk CX node planes in x.
+// Element plane ex is contracted in four classes of (ey, ez) parity; elements
+// of one class share no node, so the class's contributions are added to the
+// LDS accumulators without conflicts, and every node receives its eight
+// contributions in a fixed order: element plane below, then above, each in
+// class order.  Node plane ex is complete after element plane ex and is
+// written out (boundary rows 0: the Dirichlet projection).
+//
+// Layout: node-major, right-hand-side-minor: X[node * 16 + r].
+#pragma once
+#include "heat_kernels.h"
+
+namespace heat {
+
+constexpr int MR = 16;                                   // right-hand sides
+constexpr int MTY = 8, MTZ = 16, MCX = 16;               // node tile and planes per block
+
+typedef float mr_v4f __attribute__((ext_vector_type(4)));
+
+// entry (row, j) of the contraction matrix: row = 4 g + t, g = (y | z << 1) of
+// the node pair, t = 0: P_a, 1: P_b, 2: Q_a, 3: Q_b; column j = local node
+// (j & 1, (j >> 1) & 1, j >> 2) in (x, y, z)
+__device__ __forceinline__ float mrhs_coef(int row, int j) {
+    const int g = row >> 2, t = row & 3;
+    const int ix = t & 1, iy = g & 1, iz = g >> 1;
+    const int jx = j & 1, jy = (j >> 1) & 1, jz = j >> 2;
+    const int d = (ix != jx) + (iy != jy) + (iz != jz);
+    if (t < 2) return (float)(8 >> d);
+    return d == 0 ? -4.0f : (d == 1 ? 0.0f : 1.0f);
+}
+
+template <bool MASS>
+__global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X, float* __restrict__ Y,
+                                                    const float* __restrict__ cr, const float* __restrict__ ck,
+                                                    Grid g, int nby, int nbz, int ntiles) {
+    constexpr int TY = MTY, TZ = MTZ, CX = MCX;
+    constexpr int WY = TY + 2, WZ = TZ + 2, WN = WY * WZ;          // node window with halo
+    constexpr int EY = TY + 1, EZ = TZ + 1, EN = EY * EZ;          // elements per plane
+    constexpr int XSTRIDE = WN * MR + 16;                          // slot stride: odd multiple of 16 floats
+    constexpr int YROW = TZ * MR + 16;                              // accumulator row stride
+    constexpr int YSTRIDE = TY * YROW;
+    constexpr int V4 = WN * (MR / 4);                               // float4 per node window
+    constexpr int XR = (V4 + 255) / 256;
+    constexpr int ER = (EN + 255) / 256;
+    __shared__ float xs[2 * XSTRIDE];
+    __shared__ float ys[2 * YSTRIDE];
+    __shared__ float ec[2][2][EN];
+    const int b = xcd_slab_block(ntiles);
+    if (b >= ntiles) return;
+    const int bz = b % nbz, by = (b / nbz) % nby, chunk = b / (nbz * nby);
+    const int n = g.n, nn1 = g.nn1, nsq = n * n;
+    const size_t S = (size_t)g.sx;
+    const int y0 = by * TY, z0 = bz * TZ;
+    const int ix0 = chunk * CX, ix1 = min(ix0 + CX, n + 1);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 15, q = lane >> 4;                        // RHS column; k / row group
+    const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
+    const int gy = q & 1, gz = q >> 1;                               // output node pair (y, z) of group q
+    const float a0 = mrhs_coef(col, q), a1 = mrhs_coef(col, 4 + q);  // A: row = lane % 16, k = lane / 16
+
+    // ---- staging: node window of plane p and element coefficients of plane e
+    float4 xf[XR];
+    float ef[2][ER];
+    auto fetch = [&](int p, int e) {
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            const int v = tid + k * 256;
+            float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (v < V4) {
+                const int w = v >> 2, part = v & 3;
+                const int wy = w / WZ, wz = w - (w / WZ) * WZ;
+                const int yy = y0 - 1 + wy, zz = z0 - 1 + wz;
+                if (p >= 0 && p <= n && yy >= 0 && yy <= n && zz >= 0 && zz <= n)
+                    val = *reinterpret_cast<const float4*>(X + ((size_t)p * S + (size_t)yy * nn1 + zz) * MR + part * 4);
+            }
+            xf[k] = val;
+        }
+#pragma unroll
+        for (int k = 0; k < ER; ++k) {
+            const int v = tid + k * 256;
+            float c0 = 0.f, c1 = 0.f;
+            if (v < EN) {
+                const int ky = v / EZ, kz = v - (v / EZ) * EZ;
+                const int ey = y0 - 1 + ky, ez = z0 - 1 + kz;
+                if (e >= 0 && e < n && ey >= 0 && ey < n && ez >= 0 && ez < n) {
+                    const size_t ei = (size_t)e * nsq + (size_t)ey * n + ez;
+                    c0 = cr[ei];
+                    c1 = MASS ? 0.f : ck[ei];
+                }
+            }
+            ef[0][k] = c0;
+            ef[1][k] = c1;
+        }
+    };
+    auto park_x = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            const int v = tid + k * 256;
+            if (v < V4) *reinterpret_cast<float4*>(&xs[slot * XSTRIDE + v * 4]) = xf[k];
+        }
+    };
+    auto park_e = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < ER; ++k) {
+            const int v = tid + k * 256;
+            if (v < EN) { ec[slot][0][v] = ef[0][k]; ec[slot][1][v] = ef[1][k]; }
+        }
+    };
+    // accumulators of both node planes start at 0
+    for (int i = tid; i < 2 * YSTRIDE; i += 256) ys[i] = 0.f;
+    // prologue: node planes ix0-1 and ix0, element plane ix0-1
+    fetch(ix0 - 1, ix0 - 1);
+    park_x((ix0 - 1) & 1);
+    park_e((ix0 - 1) & 1);
+    fetch(ix0, ix0);
+    park_x(ix0 & 1);
+    __syncthreads();
+
+    for (int ex = ix0 - 1; ex < ix1; ++ex) {
+        fetch(ex + 2, ex + 1);                                   // in flight during the contraction
+        if (ex >= 0 && ex < n) {
+            const float* xlo = xs + (ex & 1) * XSTRIDE;          // node plane ex
+            const float* xhi = xs + ((ex + 1) & 1) * XSTRIDE;    // node plane ex + 1
+            const float* xlane = (qx ? xhi : xlo) + qy * WZ * MR + col;      // B operand base of this lane
+            const bool own_lo = ex >= ix0, own_hi = ex + 1 < ix1;
+            float* ylo = ys + (ex & 1) * YSTRIDE;
+            float* yhi = ys + ((ex + 1) & 1) * YSTRIDE;
+#pragma unroll 1
+            for (int cls = 0; cls < 4; ++cls) {
+                const int cy = cls >> 1, cz = cls & 1;
+                const int ky0 = (cy - (y0 - 1)) & 1, kz0 = (cz - (z0 - 1)) & 1;
+                const int nyc = (EY - ky0 + 1) >> 1, nzc = (EZ - kz0 + 1) >> 1;
+                const int nc = nyc * nzc;
+#pragma unroll 2
+                for (int idx = wave; idx < nc; idx += 4) {
+                    const int kyi = idx / nzc, kzi = idx - (idx / nzc) * nzc;
+                    const int ky = ky0 + 2 * kyi, kz = kz0 + 2 * kzi;
+                    const int ey = y0 - 1 + ky, ez = z0 - 1 + kz;
+                    if (ey < 0 || ey >= n || ez < 0 || ez >= n) continue;      // outside the mesh
+                    const float ce_r = ec[ex & 1][0][ky * EZ + kz];
+                    const float ce_k = ec[ex & 1][1][ky * EZ + kz];
+                    const int w0 = ky * WZ + kz;
+                    const float b0 = xlane[w0 * MR];                              // node (qx, qy, 0)
+                    const float b1 = xlane[(w0 + 1) * MR];                        // node (qx, qy, 1)
+                    mr_v4f c = {0.f, 0.f, 0.f, 0.f};
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c, 0, 0, 0);
+                    const float ya = MASS ? ce_r * c[0] : fmaf(ce_r, c[0], ce_k * c[2]);
+                    const float yb = MASS ? ce_r * c[1] : fmaf(ce_r, c[1], ce_k * c[3]);
+                    const int ty = ky - 1 + gy, tz = kz - 1 + gz;
+                    if (ty >= 0 && ty < TY && tz >= 0 && tz < TZ) {
+                        const int o = ty * YROW + tz * MR + col;
+                        if (own_lo) ylo[o] += ya;
+                        if (own_hi) yhi[o] += yb;
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+            __syncthreads();
+        }
+        if (ex >= ix0) {                                         // node plane ex is complete
+            float* yp = ys + (ex & 1) * YSTRIDE;
+            for (int i = tid; i < TY * TZ * (MR / 4); i += 256) {
+                const int node = i >> 2, part = i & 3;
+                const int ty = node / TZ, tz = node - (node / TZ) * TZ;
+                const int iy = y0 + ty, iz = z0 + tz;
+                float4* src = reinterpret_cast<float4*>(yp + ty * YROW + tz * MR + part * 4);
+                if (iy <= n && iz <= n) {
+                    const bool bnd = ex == 0 || ex == n || iy == 0 || iy == n || iz == 0 || iz == n;
+                    const float4 v = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : *src;
+                    *reinterpret_cast<float4*>(Y + ((size_t)ex * S + (size_t)iy * nn1 + iz) * MR + part * 4) = v;
+                }
+                *src = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+        park_x(ex & 1);                                          // plane ex + 2 replaces plane ex
+        park_e((ex + 1) & 1);                                    // element plane ex + 1
+        __syncthreads();
+    }
+}
+
+}  // namespace heat

@@ -83,6 +83,8 @@ SIGNATURES = {
     "heat_set_march_chunk": (_I, [_P, _I]),
     "heat_set_pcg_mode": (_I, [_P, _I]),
     "heat_arm_pcg_fault": (_I, [_P, _I, C.c_int64, _I, _I]),
+    "heat_apply_mrhs": (_I, [_P, _P, _P, _D]),
+    "heat_time_mrhs": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
     "heat_batch_setup": (_I, [_I, _P, _P, _P, C.POINTER(_P)]),
     "heat_batch_cleanup": (_I, [_P]),
     "heat_batch_instance": (_I, [_P, _I, C.POINTER(_P)]),
@@ -219,6 +221,16 @@ class HeatOperator:
     def set_pcg_mode(self, mode):
         """"fused" (two launches per iteration) or "reference" (the reference's dot order)."""
         _check(self._lib.heat_set_pcg_mode(self.h, {"fused": 0, "reference": 1}[mode]))
+
+    def apply_mrhs(self, X, Y, dt_eff):
+        """Y[:, r] = P(rho c M + dt_eff kappa K) X[:, r] for 16 columns (FP32
+        device arrays of shape (nn, 16), node-major)."""
+        _check(self._lib.heat_apply_mrhs(self.h, self.ptr(X), self.ptr(Y), float(dt_eff)))
+
+    def time_mrhs(self, X, Y, reps=20):
+        ms = C.c_double()
+        _check(self._lib.heat_time_mrhs(self.h, self.ptr(X), self.ptr(Y), int(reps), C.byref(ms)))
+        return ms.value
 
     def arm_pcg_fault(self, vec, node, bit, iteration):
         """One-shot flip of bit `bit` of node `node` of PCG vector `vec`

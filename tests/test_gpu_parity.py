@@ -595,3 +595,38 @@ def test_batch_rejects_unsupported_members():
     k, rhoc = make_field(4)
     with pytest.raises(h.HeatError):
         h.HeatBatch([{"n": 4, "dt": 1e-3, "k": k, "rhoc": rhoc}, {"n": 4, "dt": 1e-3, "k": -k, "rhoc": rhoc}])
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 9, 16, 17, 33])
+@pytest.mark.parametrize("dt_eff", [0.0, 1e-3])
+def test_mrhs_apply(n, dt_eff):
+    """Sixteen right-hand sides through the matrix-core contraction: every
+    column within FP32 rounding of the oracle's FP32 apply of that column
+    (and of the FP64 apply), columns independent of each other bit for bit,
+    and exact scaling by 2."""
+    print(f"[case] mrhs n={n} dt_eff={dt_eff}", flush=True)
+    dt = 1e-3
+    k, rhoc = make_field(n, seed=70 + n)
+    op = heat().HeatOperator(n, dt, k, rhoc)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct")
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((op.nn, 16)).astype(np.float32)
+    Xd = dev(X)
+    Yd = dev(np.full((op.nn, 16), np.nan, dtype=np.float32))
+    op.apply_mrhs(Xd, Yd, dt_eff)
+    Y = host(op, Yd)
+    assert np.isfinite(Y).all()
+    for r in range(16):
+        y32 = orc.apply_f32(X[:, r].copy(), dt_eff)
+        y64 = orc.apply(X[:, r].astype(np.float64), dt_eff)
+        scale = max(np.max(np.abs(y64)), 1e-30)
+        assert np.max(np.abs(Y[:, r] - y32)) <= 4e-6 * scale, r
+        assert np.max(np.abs(Y[:, r] - y64)) <= 4e-6 * scale, r
+    # a column's result does not depend on the other columns
+    perm = rng.permutation(16)
+    Yp = dev(np.zeros((op.nn, 16), dtype=np.float32))
+    op.apply_mrhs(dev(X[:, perm]), Yp, dt_eff)
+    assert np.array_equal(host(op, Yp), Y[:, perm])
+    Y2 = dev(np.zeros((op.nn, 16), dtype=np.float32))
+    op.apply_mrhs(dev(2 * X), Y2, dt_eff)
+    assert np.array_equal(host(op, Y2), 2 * Y)
