@@ -229,7 +229,8 @@ def main():
     if not a.no_mrhs:
         from mnhack8_amd import mrhs
         log("16-RHS apply at N=320 (matrix-core contraction)")
-        result["multi_rhs"] = mrhs.bench_apply(320, log=log)
+        result["multi_rhs"] = {"apply": mrhs.bench_apply(320, log=log),
+                               "solve": mrhs.bench_solve(320, steps=1, log=log)}
 
     if not a.no_cpu:
         try:

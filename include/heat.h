@@ -219,6 +219,14 @@ int heat_batch_instance(heat_batch* b, int i, heat_op** op);
 /* Per-solve mask (nonzero = active); masked solves keep their T. */
 int heat_batch_set_active(heat_batch* b, const int* active);
 int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph);
+/* A batch of COUNT right-hand sides on one mesh (new): the members share
+ * OP's coefficients and diagonal (OP stays owned by the caller and must
+ * outlive the batch) and own their T, b, PCG vectors and source vector F,
+ * copied from the device array F (COUNT x nn doubles, member-major; NULL:
+ * zero sources).  Steps and masks as for heat_batch_setup. */
+int heat_batch_setup_rhs(heat_op* op, int count, const double* F, heat_batch** out);
+/* Device pointer of vector WHICH (HEAT_VEC_T, _B, _R, _Z, _P, _AP or _F) of member I. */
+int heat_batch_vector(heat_batch* b, int i, int which, double** ptr);
 /* NSTEPS backward-Euler steps of every active solve: b = M T + dt F, then
  * Jacobi-PCG from x0 = T (src/heat_solver_het.cu:1308-1311).  ITERS
  * (optional) receives NSTEPS x COUNT iteration counts (-1: masked). */

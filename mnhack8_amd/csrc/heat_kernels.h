@@ -273,6 +273,35 @@ struct Coef {
     T cr, ck;
 };
 
+// march_node with the window's planes rotated by ROT (plane q of the step
+// lives in slot (q + ROT) % 3) and the element planes in E[EROT] (below)
+// and E[EROT ^ 1] (above): a walk that renames instead of moving registers.
+template <typename T, bool MASS, int PX, int ROT, int EROT>
+__device__ __forceinline__ T march_node_rot(const T (&W)[3][3][3], const Coef<T> (&E)[2][2][2], int par_yz,
+                                            int ix, const int (&eoff)[2][2], int nsq, const FiArm& fi) {
+    constexpr int Q0 = ROT % 3, Q1 = (ROT + 1) % 3, Q2 = (ROT + 2) % 3;
+    T acc = T(0);
+#pragma unroll
+    for (int col = 0; col < 8; ++col) {
+        const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
+        const int sa = cx ^ PX;                 // 1: the element lies below the node in x
+        const int ax = sa ? Q0 : Q2;            // window slot of the element's other x-node
+        const int oy = cy ? 0 : 2, oz = cz ? 0 : 2;
+        const T S0 = W[Q1][1][1];
+        const T S1 = (W[ax][1][1] + W[Q1][oy][1]) + W[Q1][1][oz];
+        const T S2 = (W[ax][oy][1] + W[ax][1][oz]) + W[Q1][oy][oz];
+        const T S3 = W[ax][oy][oz];
+        const Coef<T>& Ee = sa ? E[EROT][cy][cz] : E[EROT ^ 1][cy][cz];
+        T v = struct_row(S0, S1, S2, S3, Ee.cr, MASS ? T(0) : Ee.ck);
+        if (fi.armed && col == ((PX << 2) | par_yz)) {
+            const long long e = (long long)(ix - sa) * nsq + eoff[cy][cz];
+            if (e == fi.target) v = (T)flip_bit((double)v, fi.bit);
+        }
+        acc += v;
+    }
+    return acc;
+}
+
 template <typename T, bool MASS, int PX>
 __device__ __forceinline__ T march_node(const T (&W)[3][3][3], const Coef<T> (&Eprev)[2][2],
                                         const Coef<T> (&Enew)[2][2], int par_yz, int ix,
@@ -675,8 +704,12 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
         for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
+    // window planes rotate through the slots of W (period 3) and element
+    // planes through E (period 2): step k of a 6-plane group finds plane
+    // ix-1 in W[k % 3], ix in W[(k+1) % 3], loads ix+1 into W[(k+2) % 3],
+    // and finds element plane ix-1 in E[k & 1], loads ix into E[(k & 1) ^ 1]
     T W[3][3][3];
-    Coef<T> Eprev[2][2], Enew[2][2];
+    Coef<T> E[2][2][2];
     if (interior) {
         const size_t q = (size_t)iy * nn1 + iz;
         if (ix0 >= 1) {
@@ -690,8 +723,8 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
                     const size_t e = (size_t)(ix0 - 1) * nsq + eoff[cy][cz];
-                    Eprev[cy][cz].cr = cr[e];
-                    Eprev[cy][cz].ck = MASS ? T(0) : ck[e];
+                    E[0][cy][cz].cr = cr[e];
+                    E[0][cy][cz].ck = MASS ? T(0) : ck[e];
                 }
         }
 #pragma unroll
@@ -700,56 +733,64 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
             for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
     }
     const int par_yz = (py << 1) | pz;
-    // one node plane: p = position in the batch (ix parity, since batches start even)
-    auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto pc, int ix) {
-        constexpr int p = decltype(pc)::value;
+    // one node plane, step k of the group (k & 1 = position in the batch = ix parity)
+    auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) {
+        constexpr int k = decltype(kc)::value;
+        constexpr int p = k & 1;
         if (interior && ix <= n - 1) {
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[2][bb][c] = Hh.xs[p][xidx[bb][c]];
+                for (int c = 0; c < 3; ++c) W[(k + 2) % 3][bb][c] = Hh.xs[p][xidx[bb][c]];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
-                    Enew[cy][cz].cr = Hh.er[p][eidx[cy][cz]];
-                    Enew[cy][cz].ck = MASS ? T(0) : Hh.ek[p][eidx[cy][cz]];
+                    E[p ^ 1][cy][cz].cr = Hh.er[p][eidx[cy][cz]];
+                    E[p ^ 1][cy][cz].ck = MASS ? T(0) : Hh.ek[p][eidx[cy][cz]];
                 }
         }
         if (inmesh) {
             T acc = T(0);
             if (interior && ix >= 1 && ix <= n - 1)
-                acc = march_node<T, MASS, p>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+                acc = march_node_rot<T, MASS, p, k % 3, p>(W, E, par_yz, ix, eoff, nsq, fi);
             const size_t base = (size_t)ix * S;
             y[base + (size_t)iy * nn1 + iz] = acc;
             if (by == 0 && ty == 0) y[base + iz] = T(0);
             if (bz == 0 && tz == 0) y[base + (size_t)iy * nn1] = T(0);
             if (by == 0 && bz == 0 && tid == 0) y[base] = T(0);
         }
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                W[0][bb][c] = W[1][bb][c];
-                W[1][bb][c] = W[2][bb][c];
-            }
-#pragma unroll
-        for (int cy = 0; cy < 2; ++cy)
-#pragma unroll
-            for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
+        // keep the next step's loads below this step: hoisting them would
+        // keep two windows live at once
+        __builtin_amdgcn_sched_barrier(0);
     };
-    using P0 = std::integral_constant<int, 0>;
-    using P1 = std::integral_constant<int, 1>;
-    issue(H0, ix0);
-    int h = 0;
-    for (int ixb = ix0; ixb < ix1; ixb += 2, h ^= 1) {
+    // one batch of two planes from half h
+    auto batch = [&](int ixb, int h, auto k0, auto k1) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         TdmaHalf<T, MASS, TY, TZ>& cur = h ? H1 : H0;
         TdmaHalf<T, MASS, TY, TZ>& nxt = h ? H0 : H1;
         if (ixb + 2 < ix1) issue(nxt, ixb + 2);
-        step(cur, P0{}, ixb);
-        if (ixb + 1 < ix1) step(cur, P1{}, ixb + 1);
+        step(cur, k0, ixb);
+        if (ixb + 1 < ix1) step(cur, k1, ixb + 1);
+    };
+    using K0 = std::integral_constant<int, 0>;
+    using K1 = std::integral_constant<int, 1>;
+    using K2 = std::integral_constant<int, 2>;
+    using K3 = std::integral_constant<int, 3>;
+    using K4 = std::integral_constant<int, 4>;
+    using K5 = std::integral_constant<int, 5>;
+    issue(H0, ix0);
+    int h = 0;
+    for (int ixg = ix0; ixg < ix1; ixg += 6) {
+        batch(ixg, h, K0{}, K1{});
+        h ^= 1;
+        if (ixg + 2 >= ix1) break;
+        batch(ixg + 2, h, K2{}, K3{});
+        h ^= 1;
+        if (ixg + 4 >= ix1) break;
+        batch(ixg + 4, h, K4{}, K5{});
+        h ^= 1;
     }
     __builtin_amdgcn_s_waitcnt(0);     // no copy outlives the block
 }

@@ -442,6 +442,8 @@ FusedView<T> make_view(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, T* b,
     v.hist = hist;
     v.add_source = !(op->cfg.verify || op->cfg.no_source);
     v.active = 1;
+    v.pre_sp = v.nsb > FUSED_INBLOCK_MAX;
+    v.pre_tp = v.ntiles > FUSED_INBLOCK_MAX;
     return v;
 }
 
@@ -450,9 +452,18 @@ template <typename T>
 int launch_fused_iteration(heat_op* op, const FusedView<T>& v, bool detect) {
     hipStream_t s = op->stream;
     const unsigned grid1 = (unsigned)(((v.ntiles + NXCD - 1) / NXCD) * NXCD);
+    if (v.pre_sp) {
+        k_fused_fin_sp<T><<<1, NT, 0, s>>>(v);
+        CHECK_LAUNCH();
+    }
     if (detect) k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
     else k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
     CHECK_LAUNCH();
+    if (v.pre_tp) {
+        if (detect) k_fused_fin_tp<T, true><<<1, NT, 0, s>>>(v);
+        else k_fused_fin_tp<T, false><<<1, NT, 0, s>>>(v);
+        CHECK_LAUNCH();
+    }
     if (detect) k_fused_update<T, true><<<v.nsb, NT, 0, s>>>(v);
     else k_fused_update<T, false><<<v.nsb, NT, 0, s>>>(v);
     CHECK_LAUNCH();
@@ -1295,9 +1306,13 @@ struct heat_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_chunk = 0;
+    heat_op* mesh = nullptr;                 // shared-mesh batch: the borrowed handle
+    std::vector<void*> owned;                // shared-mesh batch: member vectors and states
 };
 
 namespace {
+
+int batch_upload_views(heat_batch* b);
 
 void batch_free(heat_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1310,6 +1325,37 @@ void batch_free(heat_batch* b) {
     (void)hipFree(b->sp); (void)hipFree(b->tp);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     for (heat_op* op : b->ops) heat_cleanup(op);
+    for (void* p : b->owned) (void)hipFree(p);
+}
+
+// device tables and partial arrays of a batch whose views are filled in
+int batch_finish(heat_batch* b) {
+    const int count = (int)b->views.size();
+    const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b->ev[0], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&b->ev[1], hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc(&b->hflag, 2 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&b->hiters, count * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&b->dflag, sizeof(int)));
+    HIP_TRY(hipMalloc(&b->diters, count * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->dviews, count * sizeof(FusedView<double>)));
+    HIP_TRY(hipMalloc(&b->d_tile_off, (count + 1) * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->d_sb_off, (count + 1) * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->d_node_off, (count + 1) * sizeof(int)));
+    HIP_TRY(hipMalloc(&b->sp, 3 * sizeof(double) * (size_t)total_sb));
+    HIP_TRY(hipMalloc(&b->tp, 4 * sizeof(double) * (size_t)total_tiles));
+    for (int i = 0; i < count; ++i) {
+        FusedView<double>& v = b->views[i];
+        v.sp = b->sp + b->sb_off[i];
+        v.sp_stride = total_sb;
+        v.tp = b->tp + b->tile_off[i];
+        v.tp_stride = total_tiles;
+    }
+    HIP_TRY(hipMemcpy(b->d_tile_off, b->tile_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_sb_off, b->sb_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_node_off, b->node_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    return batch_upload_views(b);
 }
 
 int batch_upload_views(heat_batch* b) {
@@ -1320,14 +1366,24 @@ int batch_upload_views(heat_batch* b) {
 }
 
 int batch_chunk(heat_batch* b) {
-    const int count = (int)b->ops.size();
+    const int count = (int)b->views.size();
     hipStream_t s = b->stream;
     const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
     const unsigned grid1 = (unsigned)(((total_tiles + NXCD - 1) / NXCD) * NXCD);
+    bool pre_sp = false, pre_tp = false;
+    for (const auto& v : b->views) { pre_sp = pre_sp || v.pre_sp; pre_tp = pre_tp || v.pre_tp; }
     auto body = [&]() -> int {
         for (int i = 0; i < b->pcg_chunk; ++i) {
+            if (pre_sp) {
+                k_batch_fin_sp<<<count, NT, 0, s>>>(b->dviews);
+                CHECK_LAUNCH();
+            }
             k_batch_apply<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
             CHECK_LAUNCH();
+            if (pre_tp) {
+                k_batch_fin_tp<<<count, NT, 0, s>>>(b->dviews);
+                CHECK_LAUNCH();
+            }
             k_batch_update<false><<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
             CHECK_LAUNCH();
         }
@@ -1389,40 +1445,75 @@ int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
         if (nodes > (1LL << 30)) return bail(fail(HEAT_E_ARG, "batch too large: %lld nodes", nodes));
         b->node_off[i + 1] = (int)nodes;
     }
-    const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
-    HIP_TRY_OR(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipEventCreateWithFlags(&b->ev[0], hipEventDisableTiming), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipEventCreateWithFlags(&b->ev[1], hipEventDisableTiming), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipHostMalloc(&b->hflag, 2 * sizeof(int), hipHostMallocDefault), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipHostMalloc(&b->hiters, count * sizeof(int), hipHostMallocDefault), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->dflag, sizeof(int)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->diters, count * sizeof(int)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->dviews, count * sizeof(FusedView<double>)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->d_tile_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->d_sb_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->d_node_off, (count + 1) * sizeof(int)), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->sp, 3 * sizeof(double) * (size_t)total_sb), return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMalloc(&b->tp, 4 * sizeof(double) * (size_t)total_tiles), return bail(HEAT_E_HIP));
     for (int i = 0; i < count; ++i) {
         heat_op* op = b->ops[i];
-        FusedView<double> v = make_view<double>(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_P], op->p2,
-                                                op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
-                                                op->vec[HEAT_VEC_B], op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr);
-        v.sp = b->sp + b->sb_off[i];
-        v.sp_stride = total_sb;
-        v.tp = b->tp + b->tile_off[i];
-        v.tp_stride = total_tiles;
-        b->views.push_back(v);
+        b->views.push_back(make_view<double>(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_P], op->p2,
+                                             op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
+                                             op->vec[HEAT_VEC_B], op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr));
     }
-    HIP_TRY_OR(hipMemcpy(b->d_tile_off, b->tile_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
-               return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMemcpy(b->d_sb_off, b->sb_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
-               return bail(HEAT_E_HIP));
-    HIP_TRY_OR(hipMemcpy(b->d_node_off, b->node_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice),
-               return bail(HEAT_E_HIP));
-    const int rc = batch_upload_views(b);
+    const int rc = batch_finish(b);
     if (rc) return bail(rc);
     *out = b;
+    return HEAT_OK;
+}
+
+int heat_batch_setup_rhs(heat_op* op, int count, const double* F, heat_batch** out) {
+    if (!op || !out || count < 1) return fail(HEAT_E_ARG, "bad batch arguments");
+    *out = nullptr;
+    if (!gather_path(op)) return fail(HEAT_E_STATE, "the batch runs the colour-ordered fast path only");
+    if (op->cfg.detect) return fail(HEAT_E_STATE, "the batch runs without detectors");
+    heat_batch* b = new heat_batch();
+    b->mesh = op;
+    auto bail = [&](int rc) { batch_free(b); delete b; return rc; };
+    const size_t nn = (size_t)op->g.nn;
+    if ((long long)nn * count > (1LL << 30)) return bail(fail(HEAT_E_ARG, "batch too large"));
+    b->tile_off.assign(count + 1, 0);
+    b->sb_off.assign(count + 1, 0);
+    b->node_off.assign(count + 1, 0);
+    HIP_TRY_OR(hipStreamSynchronize(op->stream), return bail(HEAT_E_HIP));
+    for (int i = 0; i < count; ++i) {
+        // T, b, r, z, p, p2, Ap, F of this right-hand side; the mesh arrays are shared
+        double* v = nullptr;
+        HIP_TRY_OR(hipMalloc(&v, 8 * nn * sizeof(double)), return bail(HEAT_E_HIP));
+        b->owned.push_back(v);
+        FusedState* fs = nullptr;
+        HIP_TRY_OR(hipMalloc(&fs, sizeof(FusedState)), return bail(HEAT_E_HIP));
+        b->owned.push_back(fs);
+        HIP_TRY_OR(hipMemset(v, 0, 8 * nn * sizeof(double)), return bail(HEAT_E_HIP));
+        HIP_TRY_OR(hipMemset(fs, 0, sizeof(FusedState)), return bail(HEAT_E_HIP));
+        if (F)
+            HIP_TRY_OR(hipMemcpy(v + 7 * nn, F + (size_t)i * nn, nn * sizeof(double), hipMemcpyDeviceToDevice),
+                       return bail(HEAT_E_HIP));
+        FusedView<double> w = make_view<double>(op, v, v + 4 * nn, v + 5 * nn, v + 2 * nn, v + 3 * nn, v + 6 * nn,
+                                                v + nn, op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr);
+        w.F = v + 7 * nn;
+        w.fs = fs;
+        b->views.push_back(w);
+        b->tile_off[i + 1] = b->tile_off[i] + op->fntiles;
+        b->sb_off[i + 1] = b->sb_off[i] + op->nsb;
+        b->node_off[i + 1] = b->node_off[i] + (int)nn;
+    }
+    const int rc = batch_finish(b);
+    if (rc) return bail(rc);
+    *out = b;
+    return HEAT_OK;
+}
+
+int heat_batch_vector(heat_batch* b, int i, int which, double** ptr) {
+    if (!b || !ptr || i < 0 || i >= (int)b->views.size()) return fail(HEAT_E_ARG, "bad batch member");
+    const FusedView<double>& v = b->views[i];
+    double* p = nullptr;
+    switch (which) {
+        case HEAT_VEC_T: p = v.x; break;
+        case HEAT_VEC_B: p = v.b; break;
+        case HEAT_VEC_R: p = v.r; break;
+        case HEAT_VEC_Z: p = v.z; break;
+        case HEAT_VEC_P: p = v.P0; break;
+        case HEAT_VEC_AP: p = v.Ap; break;
+        case HEAT_VEC_F: p = const_cast<double*>(v.F); break;
+        default: return fail(HEAT_E_ARG, "vector %d is not held per batch member", which);
+    }
+    *ptr = p;
     return HEAT_OK;
 }
 
@@ -1434,7 +1525,8 @@ int heat_batch_cleanup(heat_batch* b) {
 }
 
 int heat_batch_instance(heat_batch* b, int i, heat_op** op) {
-    if (!b || !op || i < 0 || i >= (int)b->ops.size()) return fail(HEAT_E_ARG, "bad batch instance");
+    if (!b || !op || i < 0 || i >= (int)b->ops.size())
+        return fail(HEAT_E_ARG, "bad batch instance (a shared-mesh batch has no member handles)");
     *op = b->ops[i];
     return HEAT_OK;
 }
@@ -1454,9 +1546,10 @@ int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph) {
 
 int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* iters) {
     if (!b || nsteps < 0 || max_iter < 0) return fail(HEAT_E_ARG, "bad batch step arguments");
-    const int count = (int)b->ops.size();
+    const int count = (int)b->views.size();
     hipStream_t s = b->stream;
     for (heat_op* op : b->ops) HIP_TRY(hipStreamSynchronize(op->stream));
+    if (b->mesh) HIP_TRY(hipStreamSynchronize(b->mesh->stream));
     const int total_sb = b->sb_off[count];
     const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
     for (int step = 0; step < nsteps; ++step) {

@@ -53,6 +53,9 @@ struct FusedState {
     int fi_armed, fi_vec, fi_bit, fi_iter;
     long long fi_idx;
     int inner_base, fi_fired;
+    // sums of the other kernel's partials, when a one-block finalize launch
+    // forms them (large meshes) instead of every block of K1 / K2
+    double red_sp[5], red_tp[4];
 };
 
 enum { SEU_U = 0, SEU_R = 1, SEU_P = 2, SEU_Z = 3, SEU_AP = 4 };
@@ -83,7 +86,12 @@ struct FusedView {
     double nwA, dt;             // ||w_A||; time step of the right-hand side
     double* hist;
     int add_source, active;     // b = M T (+ dt F); a masked solve does no work
+    int pre_sp, pre_tp;         // partial sums come finalized in fs->red_sp / red_tp
 };
+
+// Partial counts up to this are reduced by every block that needs them;
+// beyond it a one-block launch reduces them once (same order, same bits).
+constexpr int FUSED_INBLOCK_MAX = 2048;
 
 // 256-wide tree over one value per thread; result valid in thread 0.
 __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
@@ -199,7 +207,8 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     if (iter == 0) {
         double o[5];
         if (fs->check_outer) {
-            reduce_partials<5>(sd, sp, nsb, sp_stride, o);
+            if (v.pre_sp) { for (int k = 0; k < 5; ++k) o[k] = fs->red_sp[k]; }
+            else reduce_partials<5>(sd, sp, nsb, sp_stride, o);
             double bo = sqrt(o[3]);
             if (bo < 1e-30) bo = 1.0;
             // 1: converged; 2: the FP64 residual is not finite (stop: no
@@ -214,7 +223,8 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
             }
             if (outer_done) return;
         } else {
-            reduce_partials<3>(sd, sp, nsb, sp_stride, o);
+            if (v.pre_sp) { for (int k = 0; k < 3; ++k) o[k] = fs->red_sp[k]; }
+            else reduce_partials<3>(sd, sp, nsb, sp_stride, o);
         }
         double bnorm = sqrt(o[0]);
         if (bnorm < 1e-30) bnorm = 1.0;
@@ -228,7 +238,8 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
         if (fs->max_iter <= 0) return;
     } else {
         double o[2];
-        reduce_partials<2>(sd, sp, nsb, sp_stride, o);
+        if (v.pre_sp) { o[0] = fs->red_sp[0]; o[1] = fs->red_sp[1]; }
+        else reduce_partials<2>(sd, sp, nsb, sp_stride, o);
         const double bnorm = fs->bnorm;
         const double rnorm = sqrt(o[0]);
         const bool conv = (rnorm / bnorm < fs->tol);
@@ -491,7 +502,8 @@ __device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) 
     if (fs->done) return;
     const int k1 = fs->k1_iter;
     double o[4];
-    if (DETECT) reduce_partials<4>(sd, tp, ntiles, tp_stride, o);
+    if (v.pre_tp) { for (int k = 0; k < 4; ++k) o[k] = fs->red_tp[k]; }
+    else if (DETECT) reduce_partials<4>(sd, tp, ntiles, tp_stride, o);
     else reduce_partials<1>(sd, tp, ntiles, tp_stride, o);
     const double pAp = o[0];
     // D1 of this iteration's apply is recorded before the breakdown check, as
@@ -545,6 +557,44 @@ template <typename T, bool DETECT>
 __global__ __launch_bounds__(NT) void k_fused_update(FusedView<T> v) {
     fused_update_body<T, DETECT>(v, blockIdx.x);
 }
+
+// One-block finalize of the super-block partials K1 will read (the rows its
+// iteration needs) and of the tile partials K2 will read.
+template <typename T>
+__device__ __forceinline__ void fused_fin_sp_body(const FusedView<T>& v) {
+    __shared__ double sd[5][NT];
+    FusedState* fs = v.fs;
+    if (fs->done) return;
+    double o[5];
+    if (fs->iter == 0) {
+        if (fs->check_outer) reduce_partials<5>(sd, v.sp, v.nsb, v.sp_stride, o);
+        else reduce_partials<3>(sd, v.sp, v.nsb, v.sp_stride, o);
+    } else {
+        reduce_partials<2>(sd, v.sp, v.nsb, v.sp_stride, o);
+    }
+    if (threadIdx.x == 0) {
+        const int K = fs->iter == 0 ? (fs->check_outer ? 5 : 3) : 2;
+        for (int k = 0; k < K; ++k) fs->red_sp[k] = o[k];
+    }
+}
+
+template <typename T, bool DETECT>
+__device__ __forceinline__ void fused_fin_tp_body(const FusedView<T>& v) {
+    __shared__ double sd[4][NT];
+    FusedState* fs = v.fs;
+    if (fs->done) return;
+    double o[4];
+    if (DETECT) reduce_partials<4>(sd, v.tp, v.ntiles, v.tp_stride, o);
+    else reduce_partials<1>(sd, v.tp, v.ntiles, v.tp_stride, o);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (DETECT ? 4 : 1); ++k) fs->red_tp[k] = o[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_fused_fin_sp(FusedView<T> v) { fused_fin_sp_body<T>(v); }
+
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(NT) void k_fused_fin_tp(FusedView<T> v) { fused_fin_tp_body<T, DETECT>(v); }
 
 // FP64 outer residual of the refined solve from Ax (the projected FP64 apply):
 // r64 = b - Ax, inner start e = 0, r32 = (float) r64, z32 = r32 / d32;
@@ -720,6 +770,17 @@ __global__ __launch_bounds__(NT) void k_batch_update(const FusedView<double>* __
     if ((int)blockIdx.x >= sb_off[count]) return;
     const int i = find_solve(sb_off, count, blockIdx.x);
     fused_update_body<double, DETECT>(views[i], blockIdx.x - sb_off[i]);
+}
+
+// one block per solve (solves with few partials skip: their blocks reduce in place)
+__global__ __launch_bounds__(NT) void k_batch_fin_sp(const FusedView<double>* __restrict__ views) {
+    const FusedView<double>& v = views[blockIdx.x];
+    if (v.pre_sp) fused_fin_sp_body<double>(v);
+}
+
+__global__ __launch_bounds__(NT) void k_batch_fin_tp(const FusedView<double>* __restrict__ views) {
+    const FusedView<double>& v = views[blockIdx.x];
+    if (v.pre_tp) fused_fin_tp_body<double, false>(v);
 }
 
 // per-solve PCG parameters of a batch step (one thread per solve)

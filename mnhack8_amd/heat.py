@@ -87,6 +87,8 @@ SIGNATURES = {
     "heat_time_mrhs": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
     "heat_batch_setup": (_I, [_I, _P, _P, _P, C.POINTER(_P)]),
     "heat_batch_cleanup": (_I, [_P]),
+    "heat_batch_setup_rhs": (_I, [_P, _I, _P, C.POINTER(_P)]),
+    "heat_batch_vector": (_I, [_P, _I, _I, C.POINTER(_P)]),
     "heat_batch_instance": (_I, [_P, _I, C.POINTER(_P)]),
     "heat_batch_set_active": (_I, [_P, _P]),
     "heat_batch_set_tuning": (_I, [_P, _I, _I]),
@@ -376,6 +378,35 @@ class HeatBatch:
             _check(lib.heat_batch_instance(h, i, C.byref(oh)))
             self.ops.append(HeatOperator(int(m["n"]), m["dt"], None, None, _borrow=oh))
         self._keep = []
+
+    @classmethod
+    def shared(cls, op, F):
+        """COUNT right-hand sides on the mesh of HeatOperator `op` (kept by the
+        caller).  F: (count, nn) source vectors, host array or device tensor."""
+        self = cls.__new__(cls)
+        self._lib = op._lib
+        self.mesh = op
+        F = np.ascontiguousarray(F, dtype=np.float64) if not hasattr(F, "data_ptr") else F
+        count = F.shape[0]
+        if hasattr(F, "data_ptr"):
+            fptr = op.ptr(F)
+        else:
+            import torch
+            Fd = torch.from_numpy(F).cuda()
+            torch.cuda.synchronize()
+            fptr = Fd.data_ptr()
+        h = C.c_void_p()
+        _check(self._lib.heat_batch_setup_rhs(op.h, count, fptr, C.byref(h)))
+        self.h = h
+        self.count = count
+        self.ops = []
+        return self
+
+    def vector(self, i, which="T"):
+        """Device pointer (int) of vector `which` of member i."""
+        p = C.c_void_p()
+        _check(self._lib.heat_batch_vector(self.h, int(i), VEC[which], C.byref(p)))
+        return p.value
 
     def set_active(self, mask):
         a = np.ascontiguousarray(np.asarray(mask, dtype=np.int32))

@@ -83,9 +83,81 @@ def bench_apply(n=320, reps=20, log=None):
     return out
 
 
-def main():
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--n", type=int, default=320)
+    ap.add_argument("--solve-steps", type=int, default=0, help="also time this many steps of the 16-RHS solve")
+    a = ap.parse_args(argv)
     log = lambda m: print(f"[mrhs] {m}", file=sys.stderr, flush=True)
-    print(json.dumps(bench_apply(log=log)))
+    out = {"apply": bench_apply(a.n, log=log)}
+    if a.solve_steps:
+        out["solve"] = bench_solve(a.n, a.solve_steps, log=log)
+    print(json.dumps(out))
+
+
+
+def load_vectors(n, count=NRHS, seed=1):
+    """(count, nn) nodal loads h^3 f_r(x_i) of the Gaussian bumps, zero on the boundary."""
+    X = bumps(n, count, seed).astype(np.float64).T.copy()
+    h3 = (1.0 / n) ** 3
+    t = np.arange(n + 1)
+    ix, iy, iz = np.meshgrid(t, t, t, indexing="ij")
+    interior = ((ix > 0) & (ix < n) & (iy > 0) & (iy < n) & (iz > 0) & (iz < n)).ravel()
+    X *= h3
+    X[:, ~interior] = 0.0
+    return X
+
+
+def bench_solve(n=320, steps=20, tol=1e-8, max_iter=500, log=None):
+    """Sixteen right-hand sides on one mesh advanced together (shared-mesh
+    batch, fused FP64 PCG), against the same sixteen one after another."""
+    from .heat import HeatBatch, HeatOperator
+    k, rhoc = workload_fields(n)
+    dt = 1e-4
+    op = HeatOperator(n, dt, k.ravel(), rhoc.ravel(), kernel="fast", scatter="colored", source="const",
+                      source_value=1.0)
+    F = load_vectors(n)
+    t0 = time.perf_counter()
+    batch = HeatBatch.shared(op, F)
+    setup_s = time.perf_counter() - t0
+    zero = np.zeros(op.nn)
+    its = []
+    batch_s = 0.0
+    for st in range(steps):                                # one step per call: progress lines
+        t0 = time.perf_counter()
+        its.append(batch.step(1, max_iter, tol)[0])
+        batch_s += time.perf_counter() - t0
+        if log:
+            log(f"batched step {st + 1}/{steps}: {(time.perf_counter() - t0) * 1e3:.0f} ms, "
+                f"iterations max {int(its[-1].max())} min {int(its[-1].min())}")
+    its = np.array(its)
+    finals = [op.download(batch.vector(i, "T")) for i in range(NRHS)]
+    batch.cleanup()
+    # the same right-hand sides one after another (one-member batches on the same mesh)
+    seq_s, same = 0.0, True
+    for i in range(NRHS):
+        one = HeatBatch.shared(op, F[i:i + 1])
+        op.upload(one.vector(0, "T"), zero)
+        op.synchronize()
+        t0 = time.perf_counter()
+        for st in range(steps):
+            one.step(1, max_iter, tol)
+        seq_s += time.perf_counter() - t0
+        same = same and np.array_equal(op.download(one.vector(0, "T")), finals[i])
+        one.cleanup()
+        if log:
+            log(f"one after another: right-hand side {i + 1}/{NRHS} done ({seq_s * 1e3:.0f} ms so far)")
+    out = {"n": n, "rhs": NRHS, "steps": steps, "tol": tol, "nodes": op.nn,
+           "batch_ms": round(batch_s * 1e3, 1), "sequential_ms": round(seq_s * 1e3, 1),
+           "speedup_vs_sequential": round(seq_s / batch_s, 2),
+           "rhs_steps_per_s": round(NRHS * steps / batch_s, 2),
+           "iterations_total": int(its.sum()), "iterations_max_per_step": [int(x) for x in its.max(axis=1)],
+           "sequential_matches_batch_bitwise": bool(same), "setup_s": round(setup_s, 2)}
+    if log:
+        log(f"16 RHS x {steps} steps at N={n}: batched {batch_s * 1e3:.0f} ms, one after another {seq_s * 1e3:.0f} ms")
+    op.cleanup()
+    return out
 
 
 if __name__ == "__main__":

@@ -630,3 +630,60 @@ def test_mrhs_apply(n, dt_eff):
     Y2 = dev(np.zeros((op.nn, 16), dtype=np.float32))
     op.apply_mrhs(dev(2 * X), Y2, dt_eff)
     assert np.array_equal(host(op, Y2), 2 * Y)
+
+
+@pytest.mark.parametrize("n", [9, 16])
+def test_batch_shared_mesh_rhs(n):
+    """Several right-hand sides on one mesh (shared coefficients, own vectors
+    and sources): every member's iterations and T after each step
+    bit-identical to the oracle's fused solve with that source."""
+    dt = 1e-3
+    print(f"[case] shared-mesh batch n={n}", flush=True)
+    k, rhoc = make_field(n, seed=80 + n)
+    h = heat()
+    op = h.HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    rng = np.random.default_rng(n)
+    R = 5
+    F = rng.standard_normal((R, op.nn)) * 0.1
+    F[:, op.download("F") == 0] = 0.0          # sources vanish where the handle's own does (the boundary)
+    batch = h.HeatBatch.shared(op, F)
+    T0 = [rng.standard_normal(op.nn) * (op.download("F") != 0) for _ in range(R)]
+    for i in range(R):
+        op.upload(batch.vector(i, "T"), T0[i])
+    op.synchronize()
+    its = batch.step(3, 300, 1e-10)
+    for i in range(R):
+        orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+        orc.set_vector("F", F[i])
+        ito, To = _oracle_steps(orc, T0[i], 3, 300, 1e-10)
+        print(f"  member {i}: iterations {list(its[:, i])} (oracle {ito})", flush=True)
+        assert list(its[:, i]) == ito
+        assert np.array_equal(op.download(batch.vector(i, "T")), To)
+    batch.cleanup()
+
+
+def test_fused_pcg_large_mesh_finalized_sums():
+    """At N=170 (2,427 super-blocks, 2,904 tiles) the partial sums are formed
+    by one-block finalize launches instead of in every block: the same order,
+    so iterates and history stay bit-identical to the oracle."""
+    n, dt = 170, 1e-4
+    print(f"[case] fused pcg n={n} (finalized partial sums)", flush=True)
+    k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=9)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0, threads=16)
+    b = orc.build_rhs(np.zeros(op.nn))
+    op.upload("b", b)
+    op.upload("T", np.zeros(op.nn))
+    it, hist = op.pcg_solve("b", "T", 60, 1e-6, history=True)
+    ito, xo, histo = orc.pcg_fused(b, np.zeros(op.nn), 60, 1e-6)
+    print(f"  iterations {it} (oracle {ito})", flush=True)
+    assert it == ito
+    assert np.array_equal(op.download("T"), xo)
+    np.testing.assert_array_equal(hist, histo)
+    # FP32-refined at the same size: outer/inner counts and iterate bit-identical
+    op.upload("T", np.zeros(op.nn))
+    outer, inner = op.pcg_solve_mixed("b", "T", 10, 200, 1e-8, 1e-3)
+    oo, io, xmo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 10, 200, 1e-8, 1e-3)
+    assert (outer, inner) == (oo, io)
+    assert np.array_equal(op.download("T"), xmo)
