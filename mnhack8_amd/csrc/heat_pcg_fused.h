@@ -473,7 +473,7 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
 }
 
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_fused_apply(FusedView<T> v) {
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_apply(FusedView<T> v) {
     const int b = xcd_slab_block(v.ntiles);
     if (b >= v.ntiles) return;
     fused_apply_body<T, DETECT>(v, b);
@@ -755,7 +755,7 @@ __global__ __launch_bounds__(NT) void k_batch_residual(const FusedView<double>* 
 }
 
 template <bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_batch_apply(const FusedView<double>* __restrict__ views,
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_batch_apply(const FusedView<double>* __restrict__ views,
                                                           const int* __restrict__ tile_off, int count) {
     const int total = tile_off[count];
     const int gb = xcd_slab_block(total);
