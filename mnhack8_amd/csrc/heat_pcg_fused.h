@@ -755,7 +755,7 @@ __global__ __launch_bounds__(NT) void k_batch_residual(const FusedView<double>* 
 }
 
 template <bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ, 3) void k_batch_apply(const FusedView<double>* __restrict__ views,
+__global__ __launch_bounds__(FTY* FTZ) void k_batch_apply(const FusedView<double>* __restrict__ views,
                                                           const int* __restrict__ tile_off, int count) {
     const int total = tile_off[count];
     const int gb = xcd_slab_block(total);
