@@ -733,6 +733,10 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
             for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
     }
     const int par_yz = (py << 1) | pz;
+    // an armed level-R fault is applied by k_apply_gather (the host routes that
+    // one apply there); this kernel never flips
+    const FiArm nofi{0, 0, 0};
+    (void)fi;
     // one node plane, step k of the group (k & 1 = position in the batch = ix parity)
     auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) {
         constexpr int k = decltype(kc)::value;
@@ -753,12 +757,12 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
         if (inmesh) {
             T acc = T(0);
             if (interior && ix >= 1 && ix <= n - 1)
-                acc = march_node_rot<T, MASS, p, k % 3, p>(W, E, par_yz, ix, eoff, nsq, fi);
-            const size_t base = (size_t)ix * S;
-            y[base + (size_t)iy * nn1 + iz] = acc;
-            if (by == 0 && ty == 0) y[base + iz] = T(0);
-            if (bz == 0 && tz == 0) y[base + (size_t)iy * nn1] = T(0);
-            if (by == 0 && bz == 0 && tid == 0) y[base] = T(0);
+                acc = march_node_rot<T, MASS, p, k % 3, p>(W, E, par_yz, ix, eoff, nsq, nofi);
+            T* yp = y + (size_t)ix * S;                            // plane base; 32-bit offsets in it
+            yp[iy * nn1 + iz] = acc;
+            if (by == 0 && ty == 0) yp[iz] = T(0);
+            if (bz == 0 && tz == 0) yp[iy * nn1] = T(0);
+            if (by == 0 && bz == 0 && tid == 0) yp[0] = T(0);
         }
         // keep the next step's loads below this step: hoisting them would
         // keep two windows live at once

@@ -225,7 +225,9 @@ int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool
                    int check_done) {
     const Grid& g = op->g;
     hipStream_t s = op->stream;
-    if (op->colored_impl == HEAT_COLORED_GATHER) {
+    // an armed level-R fault goes through the gather kernel (same sums, same
+    // bits), so the tiled kernels carry no fault code
+    if (op->colored_impl == HEAT_COLORED_GATHER || f.armed) {
         if (mass)
             k_apply_gather<T, true, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, cr, ck, g, f, nullptr, 0.0, op->partials, op->st, check_done);
         else
