@@ -587,7 +587,7 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
 // LDS-tiled plane walk with asynchronous copies (k_apply_tile's sums again).
 //
 // Same tiles, window and arithmetic as k_apply_tile, but the block's x-plane
-// and element-plane tiles reach LDS through global_load_lds (the hardware
+// and element-plane tiles reach LDS through buffer loads to LDS (the hardware
 // copies global memory straight into LDS; no register holds the data).  The
 // tiles of two consecutive planes form a batch; while a batch is computed,
 // the copies of the next batch are in flight into the other half of a
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
     const int chunk = b / (nbz * nby);
     const int n = g.n, S = g.sx, nn1 = g.nn1, nsq = n * n;
     const int tid = threadIdx.x;
-    const int lane_base = tid & ~63;                     // first thread of this wave
+    const int lane_base = __builtin_amdgcn_readfirstlane(tid & ~63);   // first thread of this wave (uniform)
     const int ty = tid / TZ, tz = tid - (tid / TZ) * TZ;
     const int y0 = 1 + by * TY, z0 = 1 + bz * TZ;
     const int iy = y0 + ty, iz = z0 + tz;
@@ -661,32 +661,39 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
     }
     // copies of the batch starting at node plane ixb into half Hh:
     // x planes ixb+1, ixb+2 and element planes ixb, ixb+1
+    // Buffer-addressed copies: one scalar resource per plane (base, size) and
+    // 32-bit per-lane byte offsets, so no 64-bit address arithmetic per copy.
+    // Word 3 of the resource is the gfx9-family raw-buffer configuration.
+    constexpr int RSRC3 = 0x00020000;
     auto issue = [&](TdmaHalf<T, MASS, TY, TZ>& Hh, int ixb) {
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             const int xp = ixb + 1 + p, ep = ixb + p;
             if (xp <= n) {
-                const char* base = reinterpret_cast<const char*>(x + (size_t)xp * S);
+                const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<T*>(x + (size_t)xp * S), 0, (int)(S * sizeof(T)), RSRC3);
 #pragma unroll
                 for (int j = 0; j < XJ; ++j)
                     if (xo[j] >= 0)
-                        __builtin_amdgcn_global_load_lds(base + xo[j],
-                                                         (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 4 * (j * NTH + lane_base)),
-                                                         4, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rx, (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 4 * (j * NTH + lane_base)), 4,
+                            xo[j], 0, 0, 0);
             }
             if (ep < n) {
-                const char* bcr = reinterpret_cast<const char*>(cr + (size_t)ep * nsq);
-                const char* bck = reinterpret_cast<const char*>(ck + (size_t)ep * nsq);
+                const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<T*>(cr + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
+                const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<T*>(ck + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
 #pragma unroll
                 for (int j = 0; j < EJ; ++j)
                     if (eo[j] >= 0) {
-                        __builtin_amdgcn_global_load_lds(bcr + eo[j],
-                                                         (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 4 * (j * NTH + lane_base)),
-                                                         4, 0, 0);
+                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                            rr, (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 4 * (j * NTH + lane_base)), 4,
+                            eo[j], 0, 0, 0);
                         if (!MASS)
-                            __builtin_amdgcn_global_load_lds(bck + eo[j],
-                                                             (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 4 * (j * NTH + lane_base)),
-                                                             4, 0, 0);
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rk, (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 4 * (j * NTH + lane_base)), 4,
+                                eo[j], 0, 0, 0);
                     }
             }
         }
