@@ -613,11 +613,11 @@ struct TdmaHalf {
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-template <typename T, bool MASS, int TY, int TZ, int CX>
-__global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
+template <typename T, bool MASS, int TY, int TZ>
+__global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
                                                        const T* __restrict__ cr,
                                                        const T* __restrict__ ck, Grid g, FiArm fi,
-                                                       int nby, int nbz, int nblocks,
+                                                       int nby, int nbz, int nch, int nblocks,
                                                        const DevState* st, int check_done) {
     using G = TdmaGeom<T, TY, TZ>;
     constexpr int NTH = G::NTH, XW = G::XW, XN = G::XN, EW = G::EW, EN = G::EN;
@@ -637,8 +637,12 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
     const int iy = y0 + ty, iz = z0 + tz;
     const bool inmesh = (iy <= n) && (iz <= n);
     const bool interior = (iy < n) && (iz < n);
-    const int ix0 = chunk * CX;
-    const int ix1 = min(ix0 + CX, n + 1);
+    // nch chunks of whole plane pairs per (y,z) column, lengths differing by
+    // at most one pair; every chunk starts on an even plane (step parity =
+    // plane parity) and below nn1 (nch <= number of pairs)
+    const int npair = (nn1 + 1) >> 1;
+    const int ix0 = 2 * (int)(((long long)chunk * npair) / nch);
+    const int ix1 = min(2 * (int)(((long long)(chunk + 1) * npair) / nch), nn1);
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
     // byte offsets (within a plane) of this thread's copy slots; -1 = no copy
@@ -712,7 +716,7 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
 #pragma unroll
         for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
     // window planes rotate through the slots of W (period 3) and element
-    // planes through E (period 2): step k of a 6-plane group finds plane
+    // planes through E (period 2): step k of a 12-plane group finds plane
     // ix-1 in W[k % 3], ix in W[(k+1) % 3], loads ix+1 into W[(k+2) % 3],
     // and finds element plane ix-1 in E[k & 1], loads ix into E[(k & 1) ^ 1]
     T W[3][3][3];
@@ -775,33 +779,34 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tdma(const T* __restrict__ x, 
         // keep two windows live at once
         __builtin_amdgcn_sched_barrier(0);
     };
-    // one batch of two planes from half h
-    auto batch = [&](int ixb, int h, auto k0, auto k1) {
+    // one batch of two planes (steps k0, k1) from ring half HC (compile time:
+    // with 12-plane groups every batch's half is fixed, so LDS offsets fold
+    // into the reads' immediates)
+    auto batch = [&](int ixb, auto hc, auto k0, auto k1) {
+        constexpr int HC = decltype(hc)::value;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        TdmaHalf<T, MASS, TY, TZ>& cur = h ? H1 : H0;
-        TdmaHalf<T, MASS, TY, TZ>& nxt = h ? H0 : H1;
+        TdmaHalf<T, MASS, TY, TZ>& cur = HC ? H1 : H0;
+        TdmaHalf<T, MASS, TY, TZ>& nxt = HC ? H0 : H1;
         if (ixb + 2 < ix1) issue(nxt, ixb + 2);
         step(cur, k0, ixb);
         if (ixb + 1 < ix1) step(cur, k1, ixb + 1);
     };
-    using K0 = std::integral_constant<int, 0>;
-    using K1 = std::integral_constant<int, 1>;
-    using K2 = std::integral_constant<int, 2>;
-    using K3 = std::integral_constant<int, 3>;
-    using K4 = std::integral_constant<int, 4>;
-    using K5 = std::integral_constant<int, 5>;
+    using H_0 = std::integral_constant<int, 0>;
+    using H_1 = std::integral_constant<int, 1>;
     issue(H0, ix0);
-    int h = 0;
-    for (int ixg = ix0; ixg < ix1; ixg += 6) {
-        batch(ixg, h, K0{}, K1{});
-        h ^= 1;
+    for (int ixg = ix0; ixg < ix1; ixg += 12) {
+        batch(ixg, H_0{}, std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
         if (ixg + 2 >= ix1) break;
-        batch(ixg + 2, h, K2{}, K3{});
-        h ^= 1;
+        batch(ixg + 2, H_1{}, std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{});
         if (ixg + 4 >= ix1) break;
-        batch(ixg + 4, h, K4{}, K5{});
-        h ^= 1;
+        batch(ixg + 4, H_0{}, std::integral_constant<int, 4>{}, std::integral_constant<int, 5>{});
+        if (ixg + 6 >= ix1) break;
+        batch(ixg + 6, H_1{}, std::integral_constant<int, 6>{}, std::integral_constant<int, 7>{});
+        if (ixg + 8 >= ix1) break;
+        batch(ixg + 8, H_0{}, std::integral_constant<int, 8>{}, std::integral_constant<int, 9>{});
+        if (ixg + 10 >= ix1) break;
+        batch(ixg + 10, H_1{}, std::integral_constant<int, 10>{}, std::integral_constant<int, 11>{});
     }
     __builtin_amdgcn_s_waitcnt(0);     // no copy outlives the block
 }

@@ -68,6 +68,7 @@ struct heat_op {
     hipStream_t stream = nullptr;
     int colored_impl = HEAT_COLORED_TDMA;
     int march_cx = 8;
+    int tdma_res[2][2][3] = {};      // resident blocks of k_apply_tdma [float][mass][tile shape]
     int tile_shape = 0;
     int pcg_chunk = 8;
     int use_graph = 1;
@@ -176,6 +177,48 @@ bool gather_path(const heat_op* op) {
            op->colored_impl != HEAT_COLORED_SWEEP;
 }
 
+// Resident blocks of one kernel on the whole device (CUs x blocks per CU).
+int resident_blocks(const void* kernel, int threads) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess)
+        return 0;
+    return ncu * per;
+}
+
+// Chunks per (y,z) column for k_apply_tdma: about `planes` node planes per
+// chunk, then more (shorter) chunks until the last wave of resident blocks
+// is as full as the column count allows.  At most one chunk per plane pair.
+int tdma_chunks(int nn1, int ncol, int planes, int resident) {
+    const int npair = (nn1 + 1) / 2;
+    int nch = std::min(npair, std::max(1, (nn1 + planes - 1) / planes));
+    if (resident > 0) {
+        const long long waves = ((long long)ncol * nch + resident - 1) / resident;
+        nch = (int)std::min<long long>(npair, std::max<long long>(nch, waves * resident / ncol));
+    }
+    return nch;
+}
+
+template <typename T, bool MASS>
+void tdma_resident(int (&r)[3]) {
+    r[0] = resident_blocks((const void*)k_apply_tdma<T, MASS, 8, 32>, 256);
+    r[1] = resident_blocks((const void*)k_apply_tdma<T, MASS, 4, 64>, 256);
+    r[2] = resident_blocks((const void*)k_apply_tdma<T, MASS, 16, 16>, 256);
+}
+
+template <typename T, bool MASS, int TY, int TZ>
+void tdma_launch(heat_op* op, const T* x, T* y, const T* cr, const T* ck, FiArm f, int check_done) {
+    const int resident = op->tdma_res[sizeof(T) == 4][MASS][op->tile_shape];
+    const Grid& g = op->g;
+    const int nby = (g.n + TY - 1) / TY, nbz = (g.n + TZ - 1) / TZ;
+    const int nch = tdma_chunks(g.nn1, nby * nbz, op->march_cx, resident);
+    const int nblocks = nby * nbz * nch;
+    const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
+    k_apply_tdma<T, MASS, TY, TZ><<<grid, TY * TZ, 0, op->stream>>>(x, y, cr, ck, g, f, nby, nbz, nch, nblocks, op->st,
+                                                                   check_done);
+}
+
 template <typename T, int TY, int TZ, int CX>
 void tile_launch1(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
                   int check_done) {
@@ -184,13 +227,6 @@ void tile_launch1(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool 
     const int nblocks = nby * nbz * nch;
     const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
     hipStream_t s = op->stream;
-    if (op->colored_impl == HEAT_COLORED_TDMA) {
-        if (mass)
-            k_apply_tdma<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
-        else
-            k_apply_tdma<T, false, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
-        return;
-    }
     if (mass)
         k_apply_tile<T, true, TY, TZ, CX><<<grid, TY * TZ, 0, s>>>(x, y, cr, ck, g, f, nby, nbz, nblocks, op->st, check_done);
     else
@@ -210,6 +246,21 @@ void tile_launch_cx(heat_op* op, const T* x, T* y, const T* cr, const T* ck, boo
 template <typename T>
 int launch_tile(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool mass, FiArm f,
                 int check_done) {
+    if (op->colored_impl == HEAT_COLORED_TDMA) {
+#define TDMA(TY, TZ)                                                                  \
+    do {                                                                              \
+        if (mass) tdma_launch<T, true, TY, TZ>(op, x, y, cr, ck, f, check_done);      \
+        else tdma_launch<T, false, TY, TZ>(op, x, y, cr, ck, f, check_done);          \
+    } while (0)
+        switch (op->tile_shape) {
+            case 1: TDMA(4, 64); break;
+            case 2: TDMA(16, 16); break;
+            default: TDMA(8, 32); break;
+        }
+#undef TDMA
+        CHECK_LAUNCH();
+        return HEAT_OK;
+    }
     switch (op->tile_shape) {
         case 1: tile_launch_cx<T, 4, 64>(op, x, y, cr, ck, mass, f, check_done); break;
         case 2: tile_launch_cx<T, 16, 16>(op, x, y, cr, ck, mass, f, check_done); break;
@@ -575,6 +626,11 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     int rc = build_reference_matrices(op);
     if (rc) return rc;
     HIP_TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    // occupancy queries stay out of any later stream capture
+    tdma_resident<double, false>(op->tdma_res[0][0]);
+    tdma_resident<double, true>(op->tdma_res[0][1]);
+    tdma_resident<float, false>(op->tdma_res[1][0]);
+    tdma_resident<float, true>(op->tdma_res[1][1]);
     HIP_TRY(hipEventCreateWithFlags(&op->ev[0], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&op->ev[1], hipEventDisableTiming));
     const size_t nb8 = sizeof(double) * (size_t)g.nn, eb8 = sizeof(double) * (size_t)g.ne;
