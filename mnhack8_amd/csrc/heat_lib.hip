@@ -107,7 +107,7 @@ struct heat_op {
     FusedState* hfst = nullptr;   // pinned mirror
     double* fsp = nullptr;        // super-block partials, 5 x nsb
     double* ftp = nullptr;        // fused-apply tile partials, 4 x ntiles
-    int nsb = 0, fnby = 0, fnbz = 0, fntiles = 0;
+    int nsb = 0, fnby = 0, fnbz = 0, fnch = 0, fntiles = 0;
     // captured PCG chunk
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -489,7 +489,7 @@ FusedView<T> make_view(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, T* b,
     v.g = op->g;
     v.sp = op->fsp; v.sp_stride = op->nsb; v.nsb = op->nsb;
     v.tp = op->ftp; v.tp_stride = op->fntiles; v.ntiles = op->fntiles;
-    v.nby = op->fnby; v.nbz = op->fnbz;
+    v.nby = op->fnby; v.nbz = op->fnbz; v.nch = op->fnch;
     v.fs = op->fst; v.st = op->st;
     v.nwA = op->nwA; v.dt = op->cfg.dt;
     v.hist = hist;
@@ -643,7 +643,8 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     op->nsb = (int)((g.nn + SB - 1) / SB);
     op->fnby = (n + FTY - 1) / FTY;
     op->fnbz = (n + FTZ - 1) / FTZ;
-    op->fntiles = op->fnby * op->fnbz * ((g.nn1 + FCX - 1) / FCX);
+    op->fnch = fused_chunks(g.nn1, op->fnby * op->fnbz);
+    op->fntiles = op->fnby * op->fnbz * op->fnch;
     HIP_TRY(hipMalloc(&op->fsp, sizeof(double) * 5 * (size_t)op->nsb));
     HIP_TRY(hipMalloc(&op->p2, sizeof(double) * (size_t)g.nn));
     HIP_TRY(hipMalloc(&op->ftp, sizeof(double) * 4 * (size_t)op->fntiles));

@@ -12,9 +12,10 @@
 // same fixed order, so no separate finalize launch is needed.
 //
 // Reduction order (deterministic, independent of the autotuned apply schedule):
-//   K1 partials: tile (TY x TZ = 8 x 32 node columns, CX = 8 planes); each
-//     thread sums its column's products over its planes in plane order, then a
-//     256-wide tree gives one partial per tile;
+//   K1 partials: tile (TY x TZ = 8 x 32 node columns, a chunk of consecutive
+//     planes: fused_chunks() chunks per column); each thread sums its column's
+//     products over its planes in plane order, then a 256-wide tree gives one
+//     partial per tile;
 //   K2 / init partials: super-blocks of SB = 2048 consecutive nodes; thread t
 //     sums nodes base + 256 s + t for s = 0..7 in order, then a 256-wide tree;
 //   final: thread t sums partials t, t+256, ... in order, then a 256-wide tree.
@@ -35,7 +36,21 @@
 
 namespace heat {
 
-constexpr int FTY = 8, FTZ = 32, FCX = 8;    // fused apply tile
+constexpr int FTY = 8, FTZ = 32;              // fused apply tile (y, z)
+constexpr int FCX = 16;                       // target node planes per tile
+constexpr int FWAVE = 768;                    // tiles per wave: 256 CUs x 3 resident blocks
+
+// Chunks per (y,z) column of the fused apply: ceil(nn1 / FCX), raised until
+// the tiles fill whole waves of FWAVE (at most one plane per chunk).  A
+// function of the mesh alone, so the partial sums' order (and every bit of
+// the solve) does not depend on the device the solve runs on.
+__host__ __device__ inline int fused_chunks(int nn1, int ncol) {
+    int nch = (nn1 + FCX - 1) / FCX;
+    const long long waves = ((long long)ncol * nch + FWAVE - 1) / FWAVE;
+    const long long fill = waves * FWAVE / ncol;
+    if (fill > nch) nch = (int)(fill < nn1 ? fill : nn1);
+    return nch;
+}
 constexpr int SB = 2048;                      // nodes per update super-block
 constexpr int SBS = SB / NT;                  // strided nodes per thread
 
@@ -80,7 +95,7 @@ struct FusedView {
     Grid g;
     double* sp;                 // super-block partials (rows of sp_stride)
     double* tp;                 // tile partials (rows of tp_stride)
-    int sp_stride, nsb, tp_stride, ntiles, nby, nbz;
+    int sp_stride, nsb, tp_stride, ntiles, nby, nbz, nch;
     FusedState* fs;
     DevState* st;
     double nwA, dt;             // ||w_A||; time step of the right-hand side
@@ -192,7 +207,7 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     const int nby = v.nby, nbz = v.nbz, tp_stride = v.tp_stride;
     FusedState* fs = v.fs;
     double* __restrict__ hist = v.hist;
-    constexpr int TY = FTY, TZ = FTZ, CX = FCX, NTH = TY * TZ;
+    constexpr int TY = FTY, TZ = FTZ, NTH = TY * TZ;
     constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
     constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
     constexpr int XR = (XN + NTH - 1) / NTH, ER = (EN + NTH - 1) / NTH;
@@ -265,8 +280,9 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     const int iy = y0 + ty, iz = z0 + tz;
     const bool inmesh = (iy <= n) && (iz <= n);
     const bool interior = (iy < n) && (iz < n);
-    const int ix0 = chunk * CX;
-    const int ix1 = min(ix0 + CX, n + 1);
+    // nch chunks of consecutive node planes, lengths differing by at most one
+    const int ix0 = (int)(((long long)chunk * nn1) / v.nch);
+    const int ix1 = (int)(((long long)(chunk + 1) * nn1) / v.nch);
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
     const bool first = (iter == 0);

@@ -654,14 +654,31 @@ int ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol
 /* reference's (src/heat_solver_het.cu:994-1039); only the fixed order of    */
 /* the inner-product sums differs:                                            */
 /*  - p.Ap and the D1 sums of the apply: tiles of 8 (y) x 32 (z) node columns */
-/*    by 8 planes (x); each of the tile's 256 threads sums its column, then a */
-/*    256-wide tree;                                                          */
+/*    by a chunk of consecutive planes (x; fused_chunks() chunks per column,  */
+/*    lengths differing by at most one); each of the tile's 256 threads sums  */
+/*    its column, then a 256-wide tree;                                       */
 /*  - r.r, r.z (and b.b at the start): super-blocks of 2048 nodes; thread t   */
 /*    sums nodes base + 256 s + t, s = 0..7, then a 256-wide tree;            */
 /*  - final: thread t sums partials t, t+256, ... then a 256-wide tree.       */
 /* ------------------------------------------------------------------------- */
-enum { F_TY = 8, F_TZ = 32, F_CX = 8, F_NT = 256, F_SB = 2048 };
+enum { F_TY = 8, F_TZ = 32, F_CX = 16, F_WAVE = 768, F_NT = 256, F_SB = 2048 };
 enum { F_XW = F_TZ + 2, F_XN = (F_TZ + 2) * (F_TY + 2), F_XR = (F_XN + F_NT - 1) / F_NT };
+
+/* chunks per (y,z) column: ceil(nn1 / 16), raised until the tiles fill whole */
+/* waves of 768 (mnhack8_amd/csrc/heat_pcg_fused.h fused_chunks)             */
+static int fused_chunks(int nn1, int ncol) {
+    int nch = (nn1 + F_CX - 1) / F_CX;
+    const long long waves = ((long long)ncol * nch + F_WAVE - 1) / F_WAVE;
+    const long long fill = waves * F_WAVE / ncol;
+    if (fill > nch) nch = (int)(fill < nn1 ? fill : nn1);
+    return nch;
+}
+
+static long fused_ntiles(const ho_grid* g) {
+    const int n = g->n;
+    const int ncol = ((n + F_TY - 1) / F_TY) * ((n + F_TZ - 1) / F_TZ);
+    return (long)ncol * fused_chunks(g->nn1, ncol);
+}
 
 static double tree_256(double* s) {
     for (int st = F_NT / 2; st > 0; st >>= 1)
@@ -705,11 +722,13 @@ static void tile_sums(const ho_grid* g, const double* p, const double* Ap, const
                       double* part, double out[4]) {
     const int n = g->n;
     const int nby = (n + F_TY - 1) / F_TY, nbz = (n + F_TZ - 1) / F_TZ;
-    const long ntiles = (long)nby * nbz * ((g->nn1 + F_CX - 1) / F_CX);
+    const int nch = fused_chunks(g->nn1, nby * nbz);
+    const long ntiles = (long)nby * nbz * nch;
 #pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
     for (long b = 0; b < ntiles; ++b) {
         const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), chunk = (int)(b / ((long)nbz * nby));
-        const int ix0 = chunk * F_CX, ix1 = (ix0 + F_CX < n + 1) ? ix0 + F_CX : n + 1;
+        const int ix0 = (int)(((long long)chunk * g->nn1) / nch);
+        const int ix1 = (int)(((long long)(chunk + 1) * g->nn1) / nch);
         double s[4][F_NT];
         for (int tid = 0; tid < F_NT; ++tid) {
             const int ty = tid / F_TZ, tz = tid % F_TZ;
@@ -743,8 +762,7 @@ static void tile_sums(const ho_grid* g, const double* p, const double* Ap, const
 }
 
 static long fused_part_len(const ho_grid* g) {
-    const int n = g->n;
-    const long ntiles = (long)((n + F_TY - 1) / F_TY) * ((n + F_TZ - 1) / F_TZ) * ((g->nn1 + F_CX - 1) / F_CX);
+    const long ntiles = fused_ntiles(g);
     const long nsb = (g->nn + F_SB - 1) / F_SB;
     return 4 * (ntiles > nsb ? ntiles : nsb);
 }

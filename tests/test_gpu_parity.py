@@ -225,12 +225,13 @@ def test_pcg_edge_cases(mode):
     np.testing.assert_array_equal(hist, hist_o)
 
 
-@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 31, 33, 40])
+@pytest.mark.parametrize("n", [1, 2, 7, 8, 9, 31, 33, 40, 70])
 @pytest.mark.parametrize("detect", [False, True])
 def test_fused_pcg_tile_geometry(n, detect):
-    """Mesh sizes around the fused apply's 8 x 32 x 8 tile (partial tiles in
-    every direction, a single-plane last chunk at n = 8, 40): iterates,
-    history and the D1 record bit-identical to the oracle's fused order."""
+    """Mesh sizes around the fused apply's 8 x 32 node-column tile (partial
+    tiles in every direction; one-plane chunks up to n = 40, chunks of 2 and 3
+    planes at n = 70): iterates, history and the D1 record bit-identical to
+    the oracle's fused order."""
     dt = 1e-3
     print(f"[case] fused pcg n={n} detect={detect}", flush=True)
     k, rhoc = make_field(n, seed=30 + n)
@@ -663,10 +664,10 @@ def test_batch_shared_mesh_rhs(n):
 
 
 def test_fused_pcg_large_mesh_finalized_sums():
-    """At N=170 (2,427 super-blocks, 2,904 tiles) the partial sums are formed
+    """At N=200 (3,966 super-blocks, 2,275 tiles) the partial sums are formed
     by one-block finalize launches instead of in every block: the same order,
     so iterates and history stay bit-identical to the oracle."""
-    n, dt = 170, 1e-4
+    n, dt = 200, 1e-4
     print(f"[case] fused pcg n={n} (finalized partial sums)", flush=True)
     k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=9)
     k, rhoc = k.ravel(), rhoc.ravel()
