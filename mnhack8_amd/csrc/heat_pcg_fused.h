@@ -38,18 +38,20 @@ namespace heat {
 
 constexpr int FTY = 8, FTZ = 32;              // fused apply tile (y, z)
 constexpr int FCX = 16;                       // target node planes per tile
+constexpr int FCX_MIN = 8;                    // shortest chunk the wave fill may make
 constexpr int FWAVE = 768;                    // tiles per wave: 256 CUs x 3 resident blocks
 
 // Chunks per (y,z) column of the fused apply: ceil(nn1 / FCX), raised until
-// the tiles fill whole waves of FWAVE (at most one plane per chunk).  A
-// function of the mesh alone, so the partial sums' order (and every bit of
-// the solve) does not depend on the device the solve runs on.
+// the tiles fill whole waves of FWAVE, but never to chunks shorter than
+// FCX_MIN planes (small meshes, whose tiles a batch launches together, keep
+// long chunks).  A function of the mesh alone, so the partial sums' order
+// (and every bit of the solve) does not depend on the device.
 __host__ __device__ inline int fused_chunks(int nn1, int ncol) {
-    int nch = (nn1 + FCX - 1) / FCX;
+    const int nch = (nn1 + FCX - 1) / FCX;
     const long long waves = ((long long)ncol * nch + FWAVE - 1) / FWAVE;
-    const long long fill = waves * FWAVE / ncol;
-    if (fill > nch) nch = (int)(fill < nn1 ? fill : nn1);
-    return nch;
+    long long fill = waves * FWAVE / ncol;
+    if (fill > nn1 / FCX_MIN) fill = nn1 / FCX_MIN;
+    return fill > nch ? (int)fill : nch;
 }
 constexpr int SB = 2048;                      // nodes per update super-block
 constexpr int SBS = SB / NT;                  // strided nodes per thread

@@ -661,17 +661,18 @@ int ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol
 /*    sums nodes base + 256 s + t, s = 0..7, then a 256-wide tree;            */
 /*  - final: thread t sums partials t, t+256, ... then a 256-wide tree.       */
 /* ------------------------------------------------------------------------- */
-enum { F_TY = 8, F_TZ = 32, F_CX = 16, F_WAVE = 768, F_NT = 256, F_SB = 2048 };
+enum { F_TY = 8, F_TZ = 32, F_CX = 16, F_CX_MIN = 8, F_WAVE = 768, F_NT = 256, F_SB = 2048 };
 enum { F_XW = F_TZ + 2, F_XN = (F_TZ + 2) * (F_TY + 2), F_XR = (F_XN + F_NT - 1) / F_NT };
 
 /* chunks per (y,z) column: ceil(nn1 / 16), raised until the tiles fill whole */
-/* waves of 768 (mnhack8_amd/csrc/heat_pcg_fused.h fused_chunks)             */
+/* waves of 768 but not to chunks shorter than 8 planes                       */
+/* (mnhack8_amd/csrc/heat_pcg_fused.h fused_chunks)                           */
 static int fused_chunks(int nn1, int ncol) {
-    int nch = (nn1 + F_CX - 1) / F_CX;
+    const int nch = (nn1 + F_CX - 1) / F_CX;
     const long long waves = ((long long)ncol * nch + F_WAVE - 1) / F_WAVE;
-    const long long fill = waves * F_WAVE / ncol;
-    if (fill > nch) nch = (int)(fill < nn1 ? fill : nn1);
-    return nch;
+    long long fill = waves * F_WAVE / ncol;
+    if (fill > nn1 / F_CX_MIN) fill = nn1 / F_CX_MIN;
+    return fill > nch ? (int)fill : nch;
 }
 
 static long fused_ntiles(const ho_grid* g) {
