@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include <type_traits>
+#include <utility>
 
 namespace heat {
 
@@ -597,21 +598,46 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
 template <typename T, int TY, int TZ>
 struct TdmaGeom {
     static constexpr int NTH = TY * TZ;
-    static constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
-    static constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
-    static constexpr int WPV = (int)(sizeof(T) / 4);          // dwords per value
-    static constexpr int XJ = (XN * WPV + NTH - 1) / NTH;     // copies per thread per x tile
-    static constexpr int EJ = (EN * WPV + NTH - 1) / NTH;     // copies per thread per element tile
+    static constexpr int VPU = (int)(16 / sizeof(T));         // values per 16-byte copy
+    static constexpr int XW = TZ + 2, XH = TY + 2;            // x tile (halo of one node)
+    static constexpr int EW = TZ + 1, EH = TY + 1;            // element tile
+    // LDS row strides rounded up to whole copies: a row is copied by whole
+    // 16-byte units (the pad column holds values no node reads)
+    static constexpr int XWP = (XW + VPU - 1) / VPU * VPU, EWP = (EW + VPU - 1) / VPU * VPU;
+    static constexpr int XN = XWP * XH, EN = EWP * EH;
+    static constexpr int XU = XN / VPU, EU = EN / VPU;        // 16-byte copies per tile
+    static constexpr int XJ = (XU + NTH - 1) / NTH;           // copies per thread per x tile
+    static constexpr int EJ = (EU + NTH - 1) / NTH;           // copies per thread per element tile
 };
+
+// node planes per batch (per ring half).  Three-plane batches keep more
+// bytes in flight but run slower at N=128 (16.1 us against 15.4 us).
+constexpr int TDMA_BP = 2;
 
 template <typename T, bool MASS, int TY, int TZ>
 struct TdmaHalf {
-    T xs[2][TdmaGeom<T, TY, TZ>::XN];
-    T er[2][TdmaGeom<T, TY, TZ>::EN];
-    T ek[2][MASS ? 1 : TdmaGeom<T, TY, TZ>::EN];
+    T xs[TDMA_BP][TdmaGeom<T, TY, TZ>::XN];
+    T er[TDMA_BP][TdmaGeom<T, TY, TZ>::EN];
+    T ek[TDMA_BP][MASS ? 1 : TdmaGeom<T, TY, TZ>::EN];
 };
 
+// f(integral_constant<int, K>) for K = 0, 1, ... in order, while f returns true
+template <typename F, int... K>
+__device__ __forceinline__ bool tdma_steps(F&& f, std::integer_sequence<int, K...>) {
+    bool go = true;
+    ((go = go && f(std::integral_constant<int, K>{})), ...);
+    return go;
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
+
+// 16-byte buffer-to-LDS copy: lane i writes dst + 16 i.  The 16-byte form is
+// a gfx950 instruction, which the host pass (x86) cannot check.
+__device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t r, lds_void* dst, int off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, dst, 16, off, 0, 0, 0);
+#endif
+}
 
 template <typename T, bool MASS, int TY, int TZ>
 __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
@@ -620,8 +646,8 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
                                                        int nby, int nbz, int nch, int nblocks,
                                                        const DevState* st, int check_done) {
     using G = TdmaGeom<T, TY, TZ>;
-    constexpr int NTH = G::NTH, XW = G::XW, XN = G::XN, EW = G::EW, EN = G::EN;
-    constexpr int WPV = G::WPV, XJ = G::XJ, EJ = G::EJ;
+    constexpr int NTH = G::NTH, VPU = G::VPU, XWP = G::XWP, EWP = G::EWP;
+    constexpr int XU = G::XU, EU = G::EU, XJ = G::XJ, EJ = G::EJ;
     __shared__ TdmaHalf<T, MASS, TY, TZ> H0, H1;
     if (check_done && st->done) return;
     const int b = xcd_slab_block(nblocks);
@@ -645,33 +671,31 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
     const int ix1 = min(2 * (int)(((long long)(chunk + 1) * npair) / nch), nn1);
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
-    // byte offsets (within a plane) of this thread's copy slots; -1 = no copy
+    // byte offsets (within a plane) of this thread's 16-byte copy units; -1 =
+    // no copy.  Units past the mesh edge read the next row or, past the
+    // plane, are clamped to zero by the buffer's size: no node reads them.
     int xo[XJ], eo[EJ];
 #pragma unroll
     for (int j = 0; j < XJ; ++j) {
-        const int d = j * NTH + tid;
-        const int i = d / WPV, half = d - i * WPV;
-        const int r = i / XW, c = i - r * XW;
-        const int yy = y0 - 1 + r, zz = z0 - 1 + c;
-        xo[j] = (i < XN && yy <= n && zz <= n) ? (int)((yy * nn1 + zz) * sizeof(T)) + 4 * half : -1;
+        const int u = j * NTH + tid;
+        const int r = u / (XWP / VPU), c = (u - r * (XWP / VPU)) * VPU;
+        xo[j] = (u < XU) ? (int)(((y0 - 1 + r) * nn1 + (z0 - 1 + c)) * sizeof(T)) : -1;
     }
 #pragma unroll
     for (int j = 0; j < EJ; ++j) {
-        const int d = j * NTH + tid;
-        const int i = d / WPV, half = d - i * WPV;
-        const int r = i / EW, c = i - r * EW;
-        const int ey = y0 - 1 + r, ez = z0 - 1 + c;
-        eo[j] = (i < EN && ey < n && ez < n) ? (int)((ey * n + ez) * sizeof(T)) + 4 * half : -1;
+        const int u = j * NTH + tid;
+        const int r = u / (EWP / VPU), c = (u - r * (EWP / VPU)) * VPU;
+        eo[j] = (u < EU) ? (int)(((y0 - 1 + r) * n + (z0 - 1 + c)) * sizeof(T)) : -1;
     }
     // copies of the batch starting at node plane ixb into half Hh:
-    // x planes ixb+1, ixb+2 and element planes ixb, ixb+1
+    // x planes ixb+1 .. ixb+BP and element planes ixb .. ixb+BP-1
     // Buffer-addressed copies: one scalar resource per plane (base, size) and
     // 32-bit per-lane byte offsets, so no 64-bit address arithmetic per copy.
     // Word 3 of the resource is the gfx9-family raw-buffer configuration.
     constexpr int RSRC3 = 0x00020000;
-    auto issue = [&](TdmaHalf<T, MASS, TY, TZ>& Hh, int ixb) {
+    auto issue = [&](TdmaHalf<T, MASS, TY, TZ>& Hh, int ixb) __attribute__((always_inline)) {
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < TDMA_BP; ++p) {
             const int xp = ixb + 1 + p, ep = ixb + p;
             if (xp <= n) {
                 const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
@@ -679,9 +703,8 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
 #pragma unroll
                 for (int j = 0; j < XJ; ++j)
                     if (xo[j] >= 0)
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            rx, (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 4 * (j * NTH + lane_base)), 4,
-                            xo[j], 0, 0, 0);
+                        lds_copy16(rx, (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 16 * (j * NTH + lane_base)),
+                                   xo[j]);
             }
             if (ep < n) {
                 const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
@@ -691,13 +714,11 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
 #pragma unroll
                 for (int j = 0; j < EJ; ++j)
                     if (eo[j] >= 0) {
-                        __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                            rr, (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 4 * (j * NTH + lane_base)), 4,
-                            eo[j], 0, 0, 0);
+                        lds_copy16(rr, (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 16 * (j * NTH + lane_base)),
+                                   eo[j]);
                         if (!MASS)
-                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                                rk, (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 4 * (j * NTH + lane_base)), 4,
-                                eo[j], 0, 0, 0);
+                            lds_copy16(rk, (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 16 * (j * NTH + lane_base)),
+                                       eo[j]);
                     }
             }
         }
@@ -707,16 +728,16 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz) {
-            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EW + (tz + 1 - (cz ^ pz));
+            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EWP + (tz + 1 - (cz ^ pz));
             eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
         }
     int xidx[3][3];
 #pragma unroll
     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
+        for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XWP + (tz + 1 + (c - 1) * sz);
     // window planes rotate through the slots of W (period 3) and element
-    // planes through E (period 2): step k of a 12-plane group finds plane
+    // planes through E (period 2): step k of a group finds plane
     // ix-1 in W[k % 3], ix in W[(k+1) % 3], loads ix+1 into W[(k+2) % 3],
     // and finds element plane ix-1 in E[k & 1], loads ix into E[(k & 1) ^ 1]
     T W[3][3][3];
@@ -748,21 +769,22 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
     // one apply there); this kernel never flips
     const FiArm nofi{0, 0, 0};
     (void)fi;
-    // one node plane, step k of the group (k & 1 = position in the batch = ix parity)
-    auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) {
+    // one node plane, step k of the group: k & 1 = ix parity (chunks start
+    // on even planes), k % BP = the plane's slot in its batch
+    auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
-        constexpr int p = k & 1;
+        constexpr int p = k & 1, q = k % TDMA_BP;
         if (interior && ix <= n - 1) {
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[(k + 2) % 3][bb][c] = Hh.xs[p][xidx[bb][c]];
+                for (int c = 0; c < 3; ++c) W[(k + 2) % 3][bb][c] = Hh.xs[q][xidx[bb][c]];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
-                    E[p ^ 1][cy][cz].cr = Hh.er[p][eidx[cy][cz]];
-                    E[p ^ 1][cy][cz].ck = MASS ? T(0) : Hh.ek[p][eidx[cy][cz]];
+                    E[p ^ 1][cy][cz].cr = Hh.er[q][eidx[cy][cz]];
+                    E[p ^ 1][cy][cz].ck = MASS ? T(0) : Hh.ek[q][eidx[cy][cz]];
                 }
         }
         if (inmesh) {
@@ -779,35 +801,31 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
         // keep two windows live at once
         __builtin_amdgcn_sched_barrier(0);
     };
-    // one batch of two planes (steps k0, k1) from ring half HC (compile time:
-    // with 12-plane groups every batch's half is fixed, so LDS offsets fold
-    // into the reads' immediates)
-    auto batch = [&](int ixb, auto hc, auto k0, auto k1) {
-        constexpr int HC = decltype(hc)::value;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        TdmaHalf<T, MASS, TY, TZ>& cur = HC ? H1 : H0;
-        TdmaHalf<T, MASS, TY, TZ>& nxt = HC ? H0 : H1;
-        if (ixb + 2 < ix1) issue(nxt, ixb + 2);
-        step(cur, k0, ixb);
-        if (ixb + 1 < ix1) step(cur, k1, ixb + 1);
+    // Steps in groups of GL planes, the least common multiple of the W period
+    // (3), the E period (2) and the ring period (2 BP): every step's window
+    // roles, parity, ring half and batch slot are compile time, so no
+    // register moves between planes and every LDS offset is an immediate.
+    // A batch starts with "all of this wave's copies done" and one barrier,
+    // then issues the copies of the next batch into the other half.
+    constexpr int GL = (TDMA_BP % 3 == 0) ? 2 * TDMA_BP : 6 * TDMA_BP;
+    auto one = [&](int ixg, auto kc) __attribute__((always_inline)) -> bool {
+        constexpr int k = decltype(kc)::value;
+        constexpr int HC = (k / TDMA_BP) & 1;
+        const int ix = ixg + k;
+        if (ix >= ix1) return false;
+        if constexpr (k % TDMA_BP == 0) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (ix + TDMA_BP < ix1) issue(HC ? H0 : H1, ix + TDMA_BP);
+        }
+        step(HC ? H1 : H0, kc, ix);
+        return true;
     };
-    using H_0 = std::integral_constant<int, 0>;
-    using H_1 = std::integral_constant<int, 1>;
     issue(H0, ix0);
-    for (int ixg = ix0; ixg < ix1; ixg += 12) {
-        batch(ixg, H_0{}, std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
-        if (ixg + 2 >= ix1) break;
-        batch(ixg + 2, H_1{}, std::integral_constant<int, 2>{}, std::integral_constant<int, 3>{});
-        if (ixg + 4 >= ix1) break;
-        batch(ixg + 4, H_0{}, std::integral_constant<int, 4>{}, std::integral_constant<int, 5>{});
-        if (ixg + 6 >= ix1) break;
-        batch(ixg + 6, H_1{}, std::integral_constant<int, 6>{}, std::integral_constant<int, 7>{});
-        if (ixg + 8 >= ix1) break;
-        batch(ixg + 8, H_0{}, std::integral_constant<int, 8>{}, std::integral_constant<int, 9>{});
-        if (ixg + 10 >= ix1) break;
-        batch(ixg + 10, H_1{}, std::integral_constant<int, 10>{}, std::integral_constant<int, 11>{});
-    }
+    for (int ixg = ix0; ixg < ix1; ixg += GL)
+        if (!tdma_steps([&](auto kc) __attribute__((always_inline)) { return one(ixg, kc); },
+                        std::make_integer_sequence<int, GL>{}))
+            break;
     __builtin_amdgcn_s_waitcnt(0);     // no copy outlives the block
 }
 
