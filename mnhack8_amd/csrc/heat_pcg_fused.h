@@ -44,8 +44,10 @@ constexpr int FWAVE = 768;                    // tiles per wave: 256 CUs x 3 res
 // Chunks per (y,z) column of the fused apply: ceil(nn1 / FCX), raised until
 // the tiles fill whole waves of FWAVE, but never to chunks shorter than
 // FCX_MIN planes (small meshes, whose tiles a batch launches together, keep
-// long chunks).  A function of the mesh alone, so the partial sums' order
-// (and every bit of the solve) does not depend on the device.
+// long chunks).  The chunks split the column's plane pairs as evenly as
+// possible (at most one pair apart; the result never exceeds the pair
+// count).  A function of the mesh alone, so the partial sums' order (and
+// every bit of the solve) does not depend on the device.
 __host__ __device__ inline int fused_chunks(int nn1, int ncol) {
     const int nch = (nn1 + FCX - 1) / FCX;
     const long long waves = ((long long)ncol * nch + FWAVE - 1) / FWAVE;
@@ -282,9 +284,11 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     const int iy = y0 + ty, iz = z0 + tz;
     const bool inmesh = (iy <= n) && (iz <= n);
     const bool interior = (iy < n) && (iz < n);
-    // nch chunks of consecutive node planes, lengths differing by at most one
-    const int ix0 = (int)(((long long)chunk * nn1) / v.nch);
-    const int ix1 = (int)(((long long)(chunk + 1) * nn1) / v.nch);
+    // nch chunks of whole plane pairs per column, lengths differing by at
+    // most one pair
+    const int npair = (nn1 + 1) >> 1;
+    const int ix0 = 2 * (int)(((long long)chunk * npair) / v.nch);
+    const int ix1 = min(2 * (int)(((long long)(chunk + 1) * npair) / v.nch), nn1);
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
     const bool first = (iter == 0);

@@ -654,9 +654,9 @@ int ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol
 /* reference's (src/heat_solver_het.cu:994-1039); only the fixed order of    */
 /* the inner-product sums differs:                                            */
 /*  - p.Ap and the D1 sums of the apply: tiles of 8 (y) x 32 (z) node columns */
-/*    by a chunk of consecutive planes (x; fused_chunks() chunks per column,  */
-/*    lengths differing by at most one); each of the tile's 256 threads sums  */
-/*    its column, then a 256-wide tree;                                       */
+/*    by a chunk of consecutive planes (x; fused_chunks() chunks per column  */
+/*    of whole plane pairs, lengths differing by at most one pair); each of   */
+/*    the tile's 256 threads sums its column, then a 256-wide tree;           */
 /*  - r.r, r.z (and b.b at the start): super-blocks of 2048 nodes; thread t   */
 /*    sums nodes base + 256 s + t, s = 0..7, then a 256-wide tree;            */
 /*  - final: thread t sums partials t, t+256, ... then a 256-wide tree.       */
@@ -728,8 +728,11 @@ static void tile_sums(const ho_grid* g, const double* p, const double* Ap, const
 #pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
     for (long b = 0; b < ntiles; ++b) {
         const int bz = (int)(b % nbz), by = (int)((b / nbz) % nby), chunk = (int)(b / ((long)nbz * nby));
-        const int ix0 = (int)(((long long)chunk * g->nn1) / nch);
-        const int ix1 = (int)(((long long)(chunk + 1) * g->nn1) / nch);
+        /* chunks of whole plane pairs, lengths differing by at most one pair */
+        const int npair = (g->nn1 + 1) / 2;
+        const int ix0 = 2 * (int)(((long long)chunk * npair) / nch);
+        int ix1 = 2 * (int)(((long long)(chunk + 1) * npair) / nch);
+        if (ix1 > g->nn1) ix1 = g->nn1;
         double s[4][F_NT];
         for (int tid = 0; tid < F_NT; ++tid) {
             const int ty = tid / F_TZ, tz = tid % F_TZ;

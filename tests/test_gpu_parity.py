@@ -230,8 +230,9 @@ def test_pcg_edge_cases(mode):
 def test_fused_pcg_tile_geometry(n, detect):
     """Mesh sizes around the fused apply's 8 x 32 node-column tile (partial
     tiles in every direction; one chunk of all planes up to n = 14, chunks of
-    8 and 9 planes at n = 33, 40, 70): iterates, history and the D1 record
-    bit-identical to the oracle's fused order."""
+    8 and 10 planes, and an odd plane count in the last chunk, at n = 33, 40,
+    70): iterates, history and the D1 record bit-identical to the oracle's
+    fused order."""
     dt = 1e-3
     print(f"[case] fused pcg n={n} detect={detect}", flush=True)
     k, rhoc = make_field(n, seed=30 + n)
