@@ -639,19 +639,17 @@ __device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t r, lds_void* d
 #endif
 }
 
-template <typename T, bool MASS, int TY, int TZ>
-__global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
-                                                       const T* __restrict__ cr,
-                                                       const T* __restrict__ ck, Grid g, FiArm fi,
-                                                       int nby, int nbz, int nch, int nblocks,
-                                                       const DevState* st, int check_done) {
+// The walk of tile b (of nby x nbz columns x nch chunks) through the LDS
+// ring H0 / H1.  With SUMS, each thread also sums p.Ap (x at the node times
+// the node's y) and 1.Ap over its interior nodes, in plane order.
+template <typename T, bool MASS, int TY, int TZ, bool SUMS>
+__device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ cr,
+                                          const T* __restrict__ ck, const Grid& g, int nby, int nbz, int nch,
+                                          int b, TdmaHalf<T, MASS, TY, TZ>& H0, TdmaHalf<T, MASS, TY, TZ>& H1,
+                                          double& s_pap, double& s_ap) {
     using G = TdmaGeom<T, TY, TZ>;
     constexpr int NTH = G::NTH, VPU = G::VPU, XWP = G::XWP, EWP = G::EWP;
     constexpr int XU = G::XU, EU = G::EU, XJ = G::XJ, EJ = G::EJ;
-    __shared__ TdmaHalf<T, MASS, TY, TZ> H0, H1;
-    if (check_done && st->done) return;
-    const int b = xcd_slab_block(nblocks);
-    if (b >= nblocks) return;
     const int bz = b % nbz;
     const int by = (b / nbz) % nby;
     const int chunk = b / (nbz * nby);
@@ -766,9 +764,8 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
     }
     const int par_yz = (py << 1) | pz;
     // an armed level-R fault is applied by k_apply_gather (the host routes that
-    // one apply there); this kernel never flips
+    // one apply there); this walk never flips
     const FiArm nofi{0, 0, 0};
-    (void)fi;
     // one node plane, step k of the group: k & 1 = ix parity (chunks start
     // on even planes), k % BP = the plane's slot in its batch
     auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) __attribute__((always_inline)) {
@@ -789,8 +786,13 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
         }
         if (inmesh) {
             T acc = T(0);
-            if (interior && ix >= 1 && ix <= n - 1)
+            if (interior && ix >= 1 && ix <= n - 1) {
                 acc = march_node_rot<T, MASS, p, k % 3, p>(W, E, par_yz, ix, eoff, nsq, nofi);
+                if (SUMS) {
+                    s_pap += (double)W[(k + 1) % 3][1][1] * (double)acc;
+                    s_ap += (double)acc;
+                }
+            }
             T* yp = y + (size_t)ix * S;                            // plane base; 32-bit offsets in it
             yp[iy * nn1 + iz] = acc;
             if (by == 0 && ty == 0) yp[iz] = T(0);
@@ -827,6 +829,21 @@ __global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ 
                         std::make_integer_sequence<int, GL>{}))
             break;
     __builtin_amdgcn_s_waitcnt(0);     // no copy outlives the block
+}
+
+template <typename T, bool MASS, int TY, int TZ>
+__global__ __launch_bounds__(TY* TZ, 3) void k_apply_tdma(const T* __restrict__ x, T* __restrict__ y,
+                                                       const T* __restrict__ cr,
+                                                       const T* __restrict__ ck, Grid g, FiArm fi,
+                                                       int nby, int nbz, int nch, int nblocks,
+                                                       const DevState* st, int check_done) {
+    __shared__ TdmaHalf<T, MASS, TY, TZ> H0, H1;
+    (void)fi;
+    if (check_done && st->done) return;
+    const int b = xcd_slab_block(nblocks);
+    if (b >= nblocks) return;
+    double s_pap = 0.0, s_ap = 0.0;
+    tdma_body<T, MASS, TY, TZ, false>(x, y, cr, ck, g, nby, nbz, nch, b, H0, H1, s_pap, s_ap);
 }
 
 // ---------------------------------------------------------------------------

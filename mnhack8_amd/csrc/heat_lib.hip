@@ -509,9 +509,19 @@ int launch_fused_iteration(heat_op* op, const FusedView<T>& v, bool detect) {
         k_fused_fin_sp<T><<<1, NT, 0, s>>>(v);
         CHECK_LAUNCH();
     }
-    if (detect) k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
-    else k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
-    CHECK_LAUNCH();
+    if (op->pcg_fi.armed) {
+        // an upset in the PCG state is injected by the one-launch K1
+        if (detect) k_fused_apply<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
+        else k_fused_apply<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
+        CHECK_LAUNCH();
+    } else {
+        if (detect) k_fused_dir<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
+        else k_fused_dir<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
+        CHECK_LAUNCH();
+        if (detect) k_fused_tdma<T, true><<<grid1, FTY * FTZ, 0, s>>>(v);
+        else k_fused_tdma<T, false><<<grid1, FTY * FTZ, 0, s>>>(v);
+        CHECK_LAUNCH();
+    }
     if (v.pre_tp) {
         if (detect) k_fused_fin_tp<T, true><<<1, NT, 0, s>>>(v);
         else k_fused_fin_tp<T, false><<<1, NT, 0, s>>>(v);
@@ -552,7 +562,8 @@ int launch_chunk_generic(heat_op* op, const void* const key[6], Body body) {
 }
 
 int launch_chunk(heat_op* op, double* x, bool detect) {
-    const void* key6[6] = {x, (const void*)(intptr_t)(detect ? 1 : 0), (const void*)(intptr_t)op->pcg_chunk,
+    const void* key6[6] = {x, (const void*)(intptr_t)((detect ? 1 : 0) + 2 * op->pcg_fi.armed),
+                           (const void*)(intptr_t)op->pcg_chunk,
                            op->cur_hist, (const void*)(intptr_t)(op->pcg_mode + 10 * op->colored_impl),
                            (const void*)(intptr_t)(op->march_cx + 100 * op->tile_shape)};
     if (op->pcg_mode == HEAT_PCG_FUSED) {
@@ -1088,7 +1099,8 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     CHECK_LAUNCH();
     const bool detect = op->cfg.detect != 0;
     int nout = 0;
-    const void* key6[6] = {op->e32, (const void*)(intptr_t)(detect ? 9 : 7), (const void*)(intptr_t)op->pcg_chunk,
+    const void* key6[6] = {op->e32, (const void*)(intptr_t)((detect ? 9 : 7) + 16 * op->pcg_fi.armed),
+                           (const void*)(intptr_t)op->pcg_chunk,
                            nullptr, (const void*)(intptr_t)(100 + op->colored_impl), nullptr};
     for (;;) {
         double* Ax = op->vec[HEAT_VEC_AP];

@@ -192,37 +192,20 @@ __global__ __launch_bounds__(NT) void k_fused_init(FusedView<T> v) {
     fused_init_body<T>(v, blockIdx.x);
 }
 
-// K1.  sp: K2 partials (r.r, r.z) or, at iteration 0, init partials (b.b, r.r, r.z).
-// tp: output partials of this launch [p.Ap | 1.Ap | w.p | p.p] x ntiles.
-// (tile b of the solve's ntiles)
-template <typename T, bool DETECT>
-__device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
-    T* __restrict__ P0 = v.P0;
-    T* __restrict__ P1 = v.P1;
-    const T* __restrict__ z = v.z;
-    T* __restrict__ Ap = v.Ap;
-    const T* __restrict__ cr = v.cr;
-    const T* __restrict__ ck = v.ck;
-    const Grid g = v.g;
-    const double* __restrict__ w = v.w;
+// Start of K1 (tile b): beta and the convergence test from K2's partials (or,
+// at iteration 0, the init partials), in the reference's order of checks
+// (:1020-1035); block 0 records the history and the done flag.  False: the
+// solve is done and the block does nothing more (uniform over the grid).
+template <typename T>
+__device__ __forceinline__ bool fused_head(const FusedView<T>& v, int b, double (*sd)[NT], double& beta) {
     const double* __restrict__ sp = v.sp;
     const int nsb = v.nsb, sp_stride = v.sp_stride;
-    double* __restrict__ tp = v.tp;
-    const int nby = v.nby, nbz = v.nbz, tp_stride = v.tp_stride;
     FusedState* fs = v.fs;
     double* __restrict__ hist = v.hist;
-    constexpr int TY = FTY, TZ = FTZ, NTH = TY * TZ;
-    constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
-    constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
-    constexpr int XR = (XN + NTH - 1) / NTH, ER = (EN + NTH - 1) / NTH;
-    __shared__ T xs[2][XN];
-    __shared__ T es_r[2][EN];
-    __shared__ T es_k[2][EN];
-    __shared__ double sd[5][NT];         // reductions (partials in, tile sums out)
-    if (fs->done) return;
+    if (fs->done) return false;
     const int iter = fs->iter;
     // ---- beta, convergence (reference order of checks, :1020-1035)
-    double beta = 0.0;
+    beta = 0.0;
     if (iter == 0) {
         double o[5];
         if (fs->check_outer) {
@@ -240,7 +223,7 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
                 fs->outer_done = outer_done;
                 if (outer_done) fs->done = 1;
             }
-            if (outer_done) return;
+            if (outer_done) return false;
         } else {
             if (v.pre_sp) { for (int k = 0; k < 3; ++k) o[k] = fs->red_sp[k]; }
             else reduce_partials<3>(sd, sp, nsb, sp_stride, o);
@@ -254,7 +237,7 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
             if (hist) { hist[0] = sqrt(o[1]) / bnorm; hist[1] = __builtin_nan(""); hist[2] = __builtin_nan(""); }
             if (fs->max_iter <= 0) fs->done = 1;
         }
-        if (fs->max_iter <= 0) return;
+        if (fs->max_iter <= 0) return false;
     } else {
         double o[2];
         if (v.pre_sp) { o[0] = fs->red_sp[0]; o[1] = fs->red_sp[1]; }
@@ -271,8 +254,39 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
             if (conv || iter >= fs->max_iter) fs->done = 1;
             else { fs->rho[iter & 1] = o[1]; fs->beta = beta; fs->k1_iter = iter; }
         }
-        if (conv || iter >= fs->max_iter) return;      // uniform over the grid
+        if (conv || iter >= fs->max_iter) return false;      // uniform over the grid
     }
+    return true;
+}
+
+// K1.  sp: K2 partials (r.r, r.z) or, at iteration 0, init partials (b.b, r.r, r.z).
+// tp: output partials of this launch [p.Ap | 1.Ap | w.p | p.p] x ntiles.
+// (tile b of the solve's ntiles)
+template <typename T, bool DETECT>
+__device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
+    T* __restrict__ P0 = v.P0;
+    T* __restrict__ P1 = v.P1;
+    const T* __restrict__ z = v.z;
+    T* __restrict__ Ap = v.Ap;
+    const T* __restrict__ cr = v.cr;
+    const T* __restrict__ ck = v.ck;
+    const Grid g = v.g;
+    const double* __restrict__ w = v.w;
+    double* __restrict__ tp = v.tp;
+    const int nby = v.nby, nbz = v.nbz, tp_stride = v.tp_stride;
+    FusedState* fs = v.fs;
+    constexpr int TY = FTY, TZ = FTZ, NTH = TY * TZ;
+    constexpr int XW = TZ + 2, XH = TY + 2, XN = XW * XH;
+    constexpr int EW = TZ + 1, EH = TY + 1, EN = EW * EH;
+    constexpr int XR = (XN + NTH - 1) / NTH, ER = (EN + NTH - 1) / NTH;
+    __shared__ T xs[2][XN];
+    __shared__ T es_r[2][EN];
+    __shared__ T es_k[2][EN];
+    __shared__ double sd[5][NT];         // reductions (partials in, tile sums out)
+    if (fs->done) return;
+    const int iter = fs->iter;
+    double beta;
+    if (!fused_head<T>(v, b, sd, beta)) return;
     // ---- tile apply on p = z + beta p (iteration 0: p = z)
     const int bz = b % nbz;
     const int by = (b / nbz) % nby;
@@ -499,6 +513,131 @@ __global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_apply(FusedView<T> v) {
     const int b = xcd_slab_block(v.ntiles);
     if (b >= v.ntiles) return;
     fused_apply_body<T, DETECT>(v, b);
+}
+
+// K1 in two launches (no upset armed): K1a forms the new direction, K1b
+// applies the operator to it.  Same tiles, same partials, same bits as K1.
+//
+// K1a: beta and convergence as K1, then p = z + beta p_old.  With the
+// detectors, over the tile's own nodes (each node of the mesh exactly once),
+// in plane order, slot order, with the D1 sums w.p and p.p of K1 (partial
+// rows 2 and 3); without them, as a plain stream.
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ) void k_fused_dir(FusedView<T> v) {
+    __shared__ double sd[5][NT];
+    const int b = xcd_slab_block(v.ntiles);
+    if (b >= v.ntiles) return;
+    double beta;
+    if (!fused_head<T>(v, b, sd, beta)) return;
+    constexpr int NTH = FTY * FTZ, XW = FTZ + 2, XL = XW * (FTY + 2);
+    constexpr int XR = (XL + NTH - 1) / NTH;
+    const FusedState* fs = v.fs;
+    const int iter = fs->iter;
+    const bool first = (iter == 0);
+    const T* __restrict__ z = v.z;
+    const T* __restrict__ pold = (iter & 1) ? v.P0 : v.P1;
+    T* __restrict__ p = (iter & 1) ? v.P1 : v.P0;
+    const double* __restrict__ w = v.w;
+    if (!DETECT) {
+        // no sums: a contiguous stream of 16-byte units (grid stride), the
+        // last nn mod VPU nodes by the first block
+        constexpr int VPU = (int)(16 / sizeof(T));
+        struct alignas(16) U { T e[VPU]; };
+        const long long nn = v.g.nn, nu = nn / VPU;
+        const long long stride = (long long)v.ntiles * NTH;
+        for (long long u = (long long)b * NTH + threadIdx.x; u < nu; u += stride) {
+            const U zu = reinterpret_cast<const U*>(z)[u];
+            U pu;
+            if (first) {
+                pu = zu;
+            } else {
+                const U ou = reinterpret_cast<const U*>(pold)[u];
+#pragma unroll
+                for (int e = 0; e < VPU; ++e) pu.e[e] = (T)fma(beta, (double)ou.e[e], (double)zu.e[e]);
+            }
+            reinterpret_cast<U*>(p)[u] = pu;
+        }
+        if (b == 0 && threadIdx.x < nn - nu * VPU) {
+            const long long i = nu * VPU + threadIdx.x;
+            p[i] = first ? z[i] : (T)fma(beta, (double)pold[i], (double)z[i]);
+        }
+        return;
+    }
+    const int nby = v.nby, nbz = v.nbz;
+    const int bz = b % nbz, by = (b / nbz) % nby, chunk = b / (nbz * nby);
+    const int n = v.g.n, S = v.g.sx, nn1 = v.g.nn1;
+    const int tid = threadIdx.x;
+    const int y0 = 1 + by * FTY, z0 = 1 + bz * FTZ;
+    const int npair = (nn1 + 1) >> 1;
+    const int ix0 = 2 * (int)(((long long)chunk * npair) / v.nch);
+    const int ix1 = min(2 * (int)(((long long)(chunk + 1) * npair) / v.nch), nn1);
+    int xoff[XR];
+    bool xown[XR];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int i = tid + k * NTH;
+        const int r = i / XW, c = i - (i / XW) * XW;
+        const int yy = y0 - 1 + r, zz = z0 - 1 + c;
+        xown[k] = (i < XL) && yy <= n && zz <= n && r <= FTY && c <= FTZ && (r >= 1 || by == 0) &&
+                  (c >= 1 || bz == 0);
+        xoff[k] = yy * nn1 + zz;
+    }
+    double a_c = 0.0, a_xx = 0.0;
+    // UP planes of loads in flight per thread, then the stores and the sums
+    // in plane order, slot order
+    constexpr int UP = 4;
+    for (int xb = ix0; xb < ix1; xb += UP) {
+        T zv[UP][XR], ov[UP][XR];
+        double wv[UP][XR];
+#pragma unroll
+        for (int u = 0; u < UP; ++u)
+#pragma unroll
+            for (int k = 0; k < XR; ++k) {
+                const bool on = xb + u < ix1 && xown[k];
+                const size_t i = (size_t)(xb + u) * S + xoff[k];
+                zv[u][k] = on ? z[i] : T(0);
+                ov[u][k] = (on && !first) ? pold[i] : T(0);
+                wv[u][k] = (DETECT && on) ? w[i] : 0.0;
+            }
+#pragma unroll
+        for (int u = 0; u < UP; ++u)
+#pragma unroll
+            for (int k = 0; k < XR; ++k)
+                if (xb + u < ix1 && xown[k]) {
+                    const T val = first ? zv[u][k] : (T)fma(beta, (double)ov[u][k], (double)zv[u][k]);
+                    p[(size_t)(xb + u) * S + xoff[k]] = val;
+                    if (DETECT) {
+                        a_c += wv[u][k] * (double)val;
+                        a_xx += (double)val * (double)val;
+                    }
+                }
+    }
+    if (DETECT) {
+        const double t2 = tree256(sd, 2, a_c), t3 = tree256(sd, 3, a_xx);
+        if (tid == 0) { v.tp[2 * v.tp_stride + b] = t2; v.tp[3 * v.tp_stride + b] = t3; }
+    }
+}
+
+// K1b: Ap = P A p through the LDS-tiled walk with asynchronous copies, and
+// the partials p.Ap (and, with the detectors, 1.Ap) of each tile (rows 0, 1).
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_tdma(FusedView<T> v) {
+    __shared__ TdmaHalf<T, false, FTY, FTZ> H0, H1;
+    __shared__ double sd[2][NT];
+    const FusedState* fs = v.fs;
+    if (fs->done) return;
+    const int b = xcd_slab_block(v.ntiles);
+    if (b >= v.ntiles) return;
+    const T* p = (fs->iter & 1) ? v.P1 : v.P0;
+    double s_pap = 0.0, s_ap = 0.0;
+    tdma_body<T, false, FTY, FTZ, true>(p, v.Ap, v.cr, v.ck, v.g, v.nby, v.nbz, v.nch, b, H0, H1, s_pap, s_ap);
+    const double t0 = tree256(sd, 0, s_pap);
+    double t1 = 0.0;
+    if (DETECT) t1 = tree256(sd, 1, s_ap);
+    if (threadIdx.x == 0) {
+        v.tp[b] = t0;
+        if (DETECT) v.tp[v.tp_stride + b] = t1;
+    }
 }
 
 // K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
