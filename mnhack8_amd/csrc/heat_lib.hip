@@ -1449,7 +1449,9 @@ int batch_chunk(heat_batch* b) {
                 k_batch_fin_sp<<<count, NT, 0, s>>>(b->dviews);
                 CHECK_LAUNCH();
             }
-            k_batch_apply<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
+            k_batch_dir<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
+            CHECK_LAUNCH();
+            k_batch_tdma<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
             CHECK_LAUNCH();
             if (pre_tp) {
                 k_batch_fin_tp<<<count, NT, 0, s>>>(b->dviews);

@@ -523,10 +523,7 @@ __global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_apply(FusedView<T> v) {
 // in plane order, slot order, with the D1 sums w.p and p.p of K1 (partial
 // rows 2 and 3); without them, as a plain stream.
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_fused_dir(FusedView<T> v) {
-    __shared__ double sd[5][NT];
-    const int b = xcd_slab_block(v.ntiles);
-    if (b >= v.ntiles) return;
+__device__ __forceinline__ void fused_dir_body(const FusedView<T>& v, int b, double (*sd)[NT]) {
     double beta;
     if (!fused_head<T>(v, b, sd, beta)) return;
     constexpr int NTH = FTY * FTZ, XW = FTZ + 2, XL = XW * (FTY + 2);
@@ -618,16 +615,21 @@ __global__ __launch_bounds__(FTY* FTZ) void k_fused_dir(FusedView<T> v) {
     }
 }
 
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ) void k_fused_dir(FusedView<T> v) {
+    __shared__ double sd[5][NT];
+    const int b = xcd_slab_block(v.ntiles);
+    if (b >= v.ntiles) return;
+    fused_dir_body<T, DETECT>(v, b, sd);
+}
+
 // K1b: Ap = P A p through the LDS-tiled walk with asynchronous copies, and
 // the partials p.Ap (and, with the detectors, 1.Ap) of each tile (rows 0, 1).
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_tdma(FusedView<T> v) {
-    __shared__ TdmaHalf<T, false, FTY, FTZ> H0, H1;
-    __shared__ double sd[2][NT];
+__device__ __forceinline__ void fused_tdma_body(const FusedView<T>& v, int b, TdmaHalf<T, false, FTY, FTZ>& H0,
+                                                TdmaHalf<T, false, FTY, FTZ>& H1, double (*sd)[NT]) {
     const FusedState* fs = v.fs;
     if (fs->done) return;
-    const int b = xcd_slab_block(v.ntiles);
-    if (b >= v.ntiles) return;
     const T* p = (fs->iter & 1) ? v.P1 : v.P0;
     double s_pap = 0.0, s_ap = 0.0;
     tdma_body<T, false, FTY, FTZ, true>(p, v.Ap, v.cr, v.ck, v.g, v.nby, v.nbz, v.nch, b, H0, H1, s_pap, s_ap);
@@ -638,6 +640,15 @@ __global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_tdma(FusedView<T> v) {
         v.tp[b] = t0;
         if (DETECT) v.tp[v.tp_stride + b] = t1;
     }
+}
+
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_tdma(FusedView<T> v) {
+    __shared__ TdmaHalf<T, false, FTY, FTZ> H0, H1;
+    __shared__ double sd[2][NT];
+    const int b = xcd_slab_block(v.ntiles);
+    if (b >= v.ntiles) return;
+    fused_tdma_body<T, DETECT>(v, b, H0, H1, sd);
 }
 
 // K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
@@ -915,14 +926,28 @@ __global__ __launch_bounds__(NT) void k_batch_residual(const FusedView<double>* 
     }
 }
 
+// the two launches of K1 over a batch's tiles (no upset is armed in a batch)
 template <bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_batch_apply(const FusedView<double>* __restrict__ views,
-                                                          const int* __restrict__ tile_off, int count) {
+__global__ __launch_bounds__(FTY* FTZ) void k_batch_dir(const FusedView<double>* __restrict__ views,
+                                                        const int* __restrict__ tile_off, int count) {
+    __shared__ double sd[5][NT];
     const int total = tile_off[count];
     const int gb = xcd_slab_block(total);
     if (gb >= total) return;
     const int i = find_solve(tile_off, count, gb);
-    fused_apply_body<double, DETECT>(views[i], gb - tile_off[i]);
+    fused_dir_body<double, DETECT>(views[i], gb - tile_off[i], sd);
+}
+
+template <bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_batch_tdma(const FusedView<double>* __restrict__ views,
+                                                            const int* __restrict__ tile_off, int count) {
+    __shared__ TdmaHalf<double, false, FTY, FTZ> H0, H1;
+    __shared__ double sd[2][NT];
+    const int total = tile_off[count];
+    const int gb = xcd_slab_block(total);
+    if (gb >= total) return;
+    const int i = find_solve(tile_off, count, gb);
+    fused_tdma_body<double, DETECT>(views[i], gb - tile_off[i], H0, H1, sd);
 }
 
 template <bool DETECT>
