@@ -29,7 +29,7 @@
 namespace heat {
 
 constexpr int MR = 16;                                   // right-hand sides
-constexpr int MTY = 8, MTZ = 16, MCX = 16;               // node tile and planes per block
+constexpr int MTY = 7, MTZ = 16, MCX = 16;               // node tile (7 rows: 4 element rows per class and wave count) and planes per block
 
 typedef float mr_v4f __attribute__((ext_vector_type(4)));
 
@@ -53,13 +53,19 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
     constexpr int WY = TY + 2, WZ = TZ + 2, WN = WY * WZ;          // node window with halo
     constexpr int EY = TY + 1, EZ = TZ + 1, EN = EY * EZ;          // elements per plane
     constexpr int XSTRIDE = WN * MR + 16;                          // slot stride: odd multiple of 16 floats
+    // accumulator rows of TZ nodes plus a 16-float pad, with a pad row (and 16
+    // floats) before and a pad row after the tile: the halo outputs of the
+    // tile's edge elements (nodes another tile owns) land in pads, so no
+    // output needs a mask.  Plane slot s starts at s YSTRIDE + YBASE.
     constexpr int YROW = TZ * MR + 16;                              // accumulator row stride
-    constexpr int YSTRIDE = TY * YROW;
+    constexpr int YSTRIDE = (TY + 2) * YROW;
+    constexpr int YBASE = YROW + 16;
+    constexpr int YALL = 2 * YSTRIDE + 32;
     constexpr int V4 = WN * (MR / 4);                               // float4 per node window
     constexpr int XR = (V4 + 255) / 256;
     constexpr int ER = (EN + 255) / 256;
     __shared__ float xs[2 * XSTRIDE];
-    __shared__ float ys[2 * YSTRIDE];
+    __shared__ float ys[YALL];
     __shared__ float ec[2][2][EN];
     const int b = xcd_slab_block(ntiles);
     if (b >= ntiles) return;
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
         }
     };
     // accumulators of both node planes start at 0
-    for (int i = tid; i < 2 * YSTRIDE; i += 256) ys[i] = 0.f;
+    for (int i = tid; i < YALL; i += 256) ys[i] = 0.f;
     // prologue: node planes ix0-1 and ix0, element plane ix0-1
     fetch(ix0 - 1, ix0 - 1);
     park_x((ix0 - 1) & 1);
@@ -138,51 +144,70 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
             const float* xlo = xs + (ex & 1) * XSTRIDE;          // node plane ex
             const float* xhi = xs + ((ex + 1) & 1) * XSTRIDE;    // node plane ex + 1
             const float* xlane = (qx ? xhi : xlo) + qy * WZ * MR + col;      // B operand base of this lane
-            const bool own_lo = ex >= ix0, own_hi = ex + 1 < ix1;
-            float* ylo = ys + (ex & 1) * YSTRIDE;
-            float* yhi = ys + ((ex + 1) & 1) * YSTRIDE;
+            float* ylo = ys + (ex & 1) * YSTRIDE + YBASE;
+            float* yhi = ys + ((ex + 1) & 1) * YSTRIDE + YBASE;
+            // element rows of one class per wave (rows ky = ky0 + 2 kyi, kyi = wave
+            // mod 4), each row a compile-time run along z: every LDS offset in
+            // the run is an immediate, no per-element index arithmetic and no
+            // branch or mask is left.  Elements outside the mesh have zero
+            // coefficients (staging) and add zeros; halo outputs land in the
+            // accumulator pads.  Both planes' sums are always formed: a plane
+            // this block does not own is never written out and its accumulator
+            // is cleared after the plane.
+            auto run_row = [&](auto kz0c, int ky) __attribute__((always_inline)) {
+                constexpr int KZ0 = decltype(kz0c)::value;
+                const float* __restrict__ er_row = &ec[ex & 1][0][ky * EZ + KZ0];
+                const float* __restrict__ ek_row = &ec[ex & 1][1][ky * EZ + KZ0];
+                const float* __restrict__ x_row = xlane + (ky * WZ + KZ0) * MR;
+                const int o_row = (ky - 1 + gy) * YROW + (KZ0 - 1 + gz) * MR + col;   // output of element KZ0
+                constexpr int NE = (EZ - KZ0 + 1) / 2;                            // elements in the run
+                // all contractions of the run first (a back-to-back stream for
+                // the matrix pipe), then the accumulator updates
+                mr_v4f c[NE];
+#pragma unroll
+                for (int kzi = 0; kzi < NE; ++kzi) {
+                    const float b0 = x_row[(2 * kzi) * MR];                       // node (qx, qy, 0)
+                    const float b1 = x_row[(2 * kzi + 1) * MR];                   // node (qx, qy, 1)
+                    const mr_v4f zero = {0.f, 0.f, 0.f, 0.f};
+                    c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, zero, 0, 0, 0);
+                    c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[kzi], 0, 0, 0);
+                }
+#pragma unroll
+                for (int kzi = 0; kzi < NE; ++kzi) {
+                    const float ce_r = er_row[2 * kzi];
+                    const float ce_k = ek_row[2 * kzi];
+                    const float ya = MASS ? ce_r * c[kzi][0] : fmaf(ce_r, c[kzi][0], ce_k * c[kzi][2]);
+                    const float yb = MASS ? ce_r * c[kzi][1] : fmaf(ce_r, c[kzi][1], ce_k * c[kzi][3]);
+                    const int o = o_row + 2 * kzi * MR;
+                    ylo[o] += ya;
+                    yhi[o] += yb;
+                }
+            };
 #pragma unroll 1
             for (int cls = 0; cls < 4; ++cls) {
                 const int cy = cls >> 1, cz = cls & 1;
                 const int ky0 = (cy - (y0 - 1)) & 1, kz0 = (cz - (z0 - 1)) & 1;
-                const int nyc = (EY - ky0 + 1) >> 1, nzc = (EZ - kz0 + 1) >> 1;
-                const int nc = nyc * nzc;
-#pragma unroll 2
-                for (int idx = wave; idx < nc; idx += 4) {
-                    const int kyi = idx / nzc, kzi = idx - (idx / nzc) * nzc;
-                    const int ky = ky0 + 2 * kyi, kz = kz0 + 2 * kzi;
-                    const int ey = y0 - 1 + ky, ez = z0 - 1 + kz;
-                    if (ey < 0 || ey >= n || ez < 0 || ez >= n) continue;      // outside the mesh
-                    const float ce_r = ec[ex & 1][0][ky * EZ + kz];
-                    const float ce_k = ec[ex & 1][1][ky * EZ + kz];
-                    const int w0 = ky * WZ + kz;
-                    const float b0 = xlane[w0 * MR];                              // node (qx, qy, 0)
-                    const float b1 = xlane[(w0 + 1) * MR];                        // node (qx, qy, 1)
-                    mr_v4f c = {0.f, 0.f, 0.f, 0.f};
-                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, c, 0, 0, 0);
-                    c = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c, 0, 0, 0);
-                    const float ya = MASS ? ce_r * c[0] : fmaf(ce_r, c[0], ce_k * c[2]);
-                    const float yb = MASS ? ce_r * c[1] : fmaf(ce_r, c[1], ce_k * c[3]);
-                    const int ty = ky - 1 + gy, tz = kz - 1 + gz;
-                    if (ty >= 0 && ty < TY && tz >= 0 && tz < TZ) {
-                        const int o = ty * YROW + tz * MR + col;
-                        if (own_lo) ylo[o] += ya;
-                        if (own_hi) yhi[o] += yb;
-                    }
+                const int nyc = (EY - ky0 + 1) >> 1;
+#pragma unroll 1
+                for (int kyi = wave; kyi < nyc; kyi += 4) {
+                    const int ky = ky0 + 2 * kyi;
+                    if (kz0) run_row(std::integral_constant<int, 1>{}, ky);
+                    else run_row(std::integral_constant<int, 0>{}, ky);
                 }
                 __syncthreads();
             }
         } else {
             __syncthreads();
         }
-        if (ex >= ix0) {                                         // node plane ex is complete
-            float* yp = ys + (ex & 1) * YSTRIDE;
+        {                                                        // node plane ex is complete
+            const bool own = ex >= ix0;                          // (plane ix0 - 1 belongs to the previous chunk)
+            float* yp = ys + (ex & 1) * YSTRIDE + YBASE;
             for (int i = tid; i < TY * TZ * (MR / 4); i += 256) {
                 const int node = i >> 2, part = i & 3;
                 const int ty = node / TZ, tz = node - (node / TZ) * TZ;
                 const int iy = y0 + ty, iz = z0 + tz;
                 float4* src = reinterpret_cast<float4*>(yp + ty * YROW + tz * MR + part * 4);
-                if (iy <= n && iz <= n) {
+                if (own && iy <= n && iz <= n) {
                     const bool bnd = ex == 0 || ex == n || iy == 0 || iy == n || iz == 0 || iz == n;
                     const float4 v = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : *src;
                     *reinterpret_cast<float4*>(Y + ((size_t)ex * S + (size_t)iy * nn1 + iz) * MR + part * 4) = v;
