@@ -1165,8 +1165,8 @@ int heat_apply_mrhs(heat_op* op, const float* X, float* Y, double dt_eff) {
     if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
     const int ntiles = (int)nt;
     const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
-    if (mass) k_mrhs_apply<true><<<grid, 256, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
-    else k_mrhs_apply<false><<<grid, 256, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    else k_mrhs_apply<false><<<grid, MNT, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
     CHECK_LAUNCH();
     return HEAT_OK;
 }

@@ -29,7 +29,8 @@
 namespace heat {
 
 constexpr int MR = 16;                                   // right-hand sides
-constexpr int MTY = 7, MTZ = 16, MCX = 16;               // node tile (7 rows: 4 element rows per class and wave count) and planes per block
+constexpr int MTY = 15, MTZ = 16, MCX = 16;              // node tile and planes per block
+constexpr int MNT = 512;                                 // threads: 8 waves, one element row per wave and class
 
 typedef float mr_v4f __attribute__((ext_vector_type(4)));
 
@@ -46,7 +47,7 @@ __device__ __forceinline__ float mrhs_coef(int row, int j) {
 }
 
 template <bool MASS>
-__global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X, float* __restrict__ Y,
+__global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X, float* __restrict__ Y,
                                                     const float* __restrict__ cr, const float* __restrict__ ck,
                                                     Grid g, int nby, int nbz, int ntiles) {
     constexpr int TY = MTY, TZ = MTZ, CX = MCX;
@@ -62,8 +63,8 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
     constexpr int YBASE = YROW + 16;
     constexpr int YALL = 2 * YSTRIDE + 32;
     constexpr int V4 = WN * (MR / 4);                               // float4 per node window
-    constexpr int XR = (V4 + 255) / 256;
-    constexpr int ER = (EN + 255) / 256;
+    constexpr int XR = (V4 + MNT - 1) / MNT;
+    constexpr int ER = (EN + MNT - 1) / MNT;
     __shared__ float xs[2 * XSTRIDE];
     __shared__ float ys[YALL];
     __shared__ float ec[2][2][EN];
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
     auto fetch = [&](int p, int e) {
 #pragma unroll
         for (int k = 0; k < XR; ++k) {
-            const int v = tid + k * 256;
+            const int v = tid + k * MNT;
             float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
             if (v < V4) {
                 const int w = v >> 2, part = v & 3;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
         }
 #pragma unroll
         for (int k = 0; k < ER; ++k) {
-            const int v = tid + k * 256;
+            const int v = tid + k * MNT;
             float c0 = 0.f, c1 = 0.f;
             if (v < EN) {
                 const int ky = v / EZ, kz = v - (v / EZ) * EZ;
@@ -117,19 +118,19 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
     auto park_x = [&](int slot) {
 #pragma unroll
         for (int k = 0; k < XR; ++k) {
-            const int v = tid + k * 256;
+            const int v = tid + k * MNT;
             if (v < V4) *reinterpret_cast<float4*>(&xs[slot * XSTRIDE + v * 4]) = xf[k];
         }
     };
     auto park_e = [&](int slot) {
 #pragma unroll
         for (int k = 0; k < ER; ++k) {
-            const int v = tid + k * 256;
+            const int v = tid + k * MNT;
             if (v < EN) { ec[slot][0][v] = ef[0][k]; ec[slot][1][v] = ef[1][k]; }
         }
     };
     // accumulators of both node planes start at 0
-    for (int i = tid; i < YALL; i += 256) ys[i] = 0.f;
+    for (int i = tid; i < YALL; i += MNT) ys[i] = 0.f;
     // prologue: node planes ix0-1 and ix0, element plane ix0-1
     fetch(ix0 - 1, ix0 - 1);
     park_x((ix0 - 1) & 1);
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
             float* ylo = ys + (ex & 1) * YSTRIDE + YBASE;
             float* yhi = ys + ((ex + 1) & 1) * YSTRIDE + YBASE;
             // element rows of one class per wave (rows ky = ky0 + 2 kyi, kyi = wave
-            // mod 4), each row a compile-time run along z: every LDS offset in
+            // mod 8), each row a compile-time run along z: every LDS offset in
             // the run is an immediate, no per-element index arithmetic and no
             // branch or mask is left.  Elements outside the mesh have zero
             // coefficients (staging) and add zeros; halo outputs land in the
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
                 const int ky0 = (cy - (y0 - 1)) & 1, kz0 = (cz - (z0 - 1)) & 1;
                 const int nyc = (EY - ky0 + 1) >> 1;
 #pragma unroll 1
-                for (int kyi = wave; kyi < nyc; kyi += 4) {
+                for (int kyi = wave; kyi < nyc; kyi += MNT / 64) {
                     const int ky = ky0 + 2 * kyi;
                     if (kz0) run_row(std::integral_constant<int, 1>{}, ky);
                     else run_row(std::integral_constant<int, 0>{}, ky);
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_mrhs_apply(const float* __restrict__ X,
         {                                                        // node plane ex is complete
             const bool own = ex >= ix0;                          // (plane ix0 - 1 belongs to the previous chunk)
             float* yp = ys + (ex & 1) * YSTRIDE + YBASE;
-            for (int i = tid; i < TY * TZ * (MR / 4); i += 256) {
+            for (int i = tid; i < TY * TZ * (MR / 4); i += MNT) {
                 const int node = i >> 2, part = i & 3;
                 const int ty = node / TZ, tz = node - (node / TZ) * TZ;
                 const int iy = y0 + ty, iz = z0 + tz;
