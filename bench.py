@@ -198,7 +198,7 @@ def main():
         for label, mode in runs:
             op.set_pcg_mode(mode)
             best = None
-            for rep in range(3):
+            for rep in range(7):
                 op.upload("T", np.zeros(nn))
                 op.synchronize()
                 t0 = time.perf_counter()
