@@ -444,6 +444,44 @@ def test_pcg_state_upset_fp64_bitwise(vec):
         assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
 
 
+def test_upset_and_clean_solves_share_a_handle():
+    """An armed upset runs K1 as one launch, a clean solve as two; solves on
+    one handle alternate between them (each with its own captured graph),
+    FP64 and refined, and every one stays bit-identical to the oracle."""
+    n, dt = 12, 5e-4
+    k, rhoc = make_field(n, seed=52)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=True, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True, source="const", source_value=1.0)
+    b = orc.build_rhs(np.zeros(op.nn))
+    rng = np.random.default_rng(9)
+    for round_ in range(2):
+        for armed in (True, False):
+            for refined in (False, True):
+                node, bit, site = int(rng.integers(op.nn)), (51 if not refined else 29), 3
+                print(f"[case] shared handle round={round_} armed={armed} refined={refined}", flush=True)
+                op.upload("b", b)
+                op.upload("T", np.zeros(op.nn))
+                op.abft_take()
+                orc.abft_take()
+                if armed:
+                    op.arm_pcg_fault("p", node, bit, site)
+                    orc.arm_pcg_fault("p", node, bit, site)
+                if refined:
+                    got = op.pcg_solve_mixed("b", "T", 8, 60, 1e-10, 1e-4)
+                    want = orc.pcg_mixed_fused(b, np.zeros(op.nn), 8, 60, 1e-10, 1e-4)
+                    assert tuple(got) == tuple(want[:2])
+                    xo = want[2]
+                else:
+                    it = op.pcg_solve("b", "T", 200, 1e-10)
+                    ito, xo, _ = orc.pcg_fused(b, np.zeros(op.nn), 200, 1e-10)
+                    assert it == ito
+                assert op.pcg_fault_fired() == orc.pcg_fault_fired()
+                assert np.array_equal(op.download("T"), xo, equal_nan=True)
+                d1, calls = op.abft_take()
+                d1o, callso = orc.abft_take()
+                assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
+
+
 @pytest.mark.parametrize("vec", SEU_VECS)
 def test_pcg_state_upset_refined_bitwise(vec):
     """Upsets of the FP32 inner state of the refined solve, with the FP32 D1
