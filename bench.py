@@ -223,8 +223,10 @@ def main():
 
     if not a.no_ensemble:
         from mnhack8_amd import ensemble
-        log("ensemble: 96 members x 20 steps, batched and one after the other")
-        result["ensemble"] = ensemble.run(96, 20, 1e-8, log=log)
+        log("ensemble: 96 members x 20 steps, FP32-refined, batched and one after the other")
+        result["ensemble"] = ensemble.run(96, 20, 1e-8, log=log, precision="refined")
+        log("ensemble: the same with plain FP64 PCG")
+        result["ensemble_fp64"] = ensemble.run(96, 20, 1e-8, log=log, precision="fp64")
 
     if not a.no_mrhs:
         from mnhack8_amd import mrhs

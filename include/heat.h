@@ -231,6 +231,14 @@ int heat_batch_vector(heat_batch* b, int i, int which, double** ptr);
  * Jacobi-PCG from x0 = T (src/heat_solver_het.cu:1308-1311).  ITERS
  * (optional) receives NSTEPS x COUNT iteration counts (-1: masked). */
 int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* iters);
+/* NSTEPS backward-Euler steps of every active solve of a batch of handles
+ * (heat_batch_setup), each solve as heat_pcg_solve_mixed (FP32 inner
+ * Jacobi-PCG in fused order inside FP64 defect correction to TOL, inner
+ * tolerance TOL_INNER; new, no counterpart in the reference, whose solves are
+ * FP64 only): every member bit-identical to that single solve.  INNER_ITERS
+ * (optional) receives NSTEPS x COUNT inner-iteration totals (-1: masked). */
+int heat_batch_step_mixed(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
+                          int* inner_iters);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);

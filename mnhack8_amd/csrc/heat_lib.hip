@@ -1358,6 +1358,12 @@ __global__ void k_batch_iters(const FusedView<double>* __restrict__ views, int c
     if (i < count) out[i] = views[i].active ? views[i].fs->iter : -1;
 }
 
+// inner iterations of each refined solve of the step (-1: masked)
+__global__ void k_batch_inner(const FusedView<double>* __restrict__ views, int count, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) out[i] = views[i].active ? views[i].fs->inner_total : -1;
+}
+
 }  // namespace
 
 struct heat_batch {
@@ -1370,6 +1376,8 @@ struct heat_batch {
     int* hiters = nullptr;                   // pinned
     std::vector<FusedView<double>> views;
     FusedView<double>* dviews = nullptr;
+    std::vector<FusedView<float>> views32;   // refined steps: the FP32 inner solves (made on first use)
+    FusedView<float>* dviews32 = nullptr;
     std::vector<int> tile_off, sb_off, node_off;
     int *d_tile_off = nullptr, *d_sb_off = nullptr, *d_node_off = nullptr;
     double *sp = nullptr, *tp = nullptr;
@@ -1377,6 +1385,9 @@ struct heat_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_chunk = 0;
+    hipGraph_t graph32 = nullptr;            // the FP32 inner chunk of refined steps
+    hipGraphExec_t gexec32 = nullptr;
+    int graph_chunk32 = 0;
     heat_op* mesh = nullptr;                 // shared-mesh batch: the borrowed handle
     std::vector<void*> owned;                // shared-mesh batch: member vectors and states
 };
@@ -1388,6 +1399,9 @@ int batch_upload_views(heat_batch* b);
 void batch_free(heat_batch* b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->gexec32) (void)hipGraphExecDestroy(b->gexec32);
+    if (b->graph32) (void)hipGraphDestroy(b->graph32);
+    (void)hipFree(b->dviews32);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     if (b->hflag) (void)hipHostFree(b->hflag);
     if (b->hiters) (void)hipHostFree(b->hiters);
@@ -1414,7 +1428,7 @@ int batch_finish(heat_batch* b) {
     HIP_TRY(hipMalloc(&b->d_tile_off, (count + 1) * sizeof(int)));
     HIP_TRY(hipMalloc(&b->d_sb_off, (count + 1) * sizeof(int)));
     HIP_TRY(hipMalloc(&b->d_node_off, (count + 1) * sizeof(int)));
-    HIP_TRY(hipMalloc(&b->sp, 3 * sizeof(double) * (size_t)total_sb));
+    HIP_TRY(hipMalloc(&b->sp, 5 * sizeof(double) * (size_t)total_sb));   // 5 rows: refined steps
     HIP_TRY(hipMalloc(&b->tp, 4 * sizeof(double) * (size_t)total_tiles));
     for (int i = 0; i < count; ++i) {
         FusedView<double>& v = b->views[i];
@@ -1432,53 +1446,113 @@ int batch_finish(heat_batch* b) {
 int batch_upload_views(heat_batch* b) {
     HIP_TRY(hipMemcpyAsync(b->dviews, b->views.data(), sizeof(FusedView<double>) * b->views.size(),
                            hipMemcpyHostToDevice, b->stream));
+    if (!b->views32.empty()) {
+        for (size_t i = 0; i < b->views.size(); ++i) b->views32[i].active = b->views[i].active;
+        HIP_TRY(hipMemcpyAsync(b->dviews32, b->views32.data(), sizeof(FusedView<float>) * b->views32.size(),
+                               hipMemcpyHostToDevice, b->stream));
+    }
     HIP_TRY(hipStreamSynchronize(b->stream));
     return HEAT_OK;
 }
 
-int batch_chunk(heat_batch* b) {
-    const int count = (int)b->views.size();
+// One chunk of pcg_chunk batched fused iterations over the views `dv` (FP64
+// solves, or the FP32 inner solves of refined ones), then the all-done flag;
+// captured once per value type as a hipGraph.
+template <typename T>
+int batch_chunk_t(heat_batch* b, const std::vector<FusedView<T>>& views, const FusedView<T>* dv,
+                  hipGraph_t& graph_, hipGraphExec_t& gexec_, int& graph_chunk_) {
+    const int count = (int)views.size();
     hipStream_t s = b->stream;
     const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
     const unsigned grid1 = (unsigned)(((total_tiles + NXCD - 1) / NXCD) * NXCD);
     bool pre_sp = false, pre_tp = false;
-    for (const auto& v : b->views) { pre_sp = pre_sp || v.pre_sp; pre_tp = pre_tp || v.pre_tp; }
+    for (const auto& v : views) { pre_sp = pre_sp || v.pre_sp; pre_tp = pre_tp || v.pre_tp; }
     auto body = [&]() -> int {
         for (int i = 0; i < b->pcg_chunk; ++i) {
             if (pre_sp) {
-                k_batch_fin_sp<<<count, NT, 0, s>>>(b->dviews);
+                k_batch_fin_sp<T><<<count, NT, 0, s>>>(dv);
                 CHECK_LAUNCH();
             }
-            k_batch_dir<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
+            k_batch_dir<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
             CHECK_LAUNCH();
-            k_batch_tdma<false><<<grid1, FTY * FTZ, 0, s>>>(b->dviews, b->d_tile_off, count);
+            k_batch_tdma<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
             CHECK_LAUNCH();
             if (pre_tp) {
-                k_batch_fin_tp<<<count, NT, 0, s>>>(b->dviews);
+                k_batch_fin_tp<T><<<count, NT, 0, s>>>(dv);
                 CHECK_LAUNCH();
             }
-            k_batch_update<false><<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
+            k_batch_update<T, false><<<total_sb, NT, 0, s>>>(dv, b->d_sb_off, count);
             CHECK_LAUNCH();
         }
-        k_batch_all_done<<<1, NT, 0, s>>>(b->dviews, count, b->dflag);
+        k_batch_all_done<T><<<1, NT, 0, s>>>(dv, count, b->dflag);
         CHECK_LAUNCH();
         return HEAT_OK;
     };
     if (!b->use_graph) return body();
-    if (!b->gexec || b->graph_chunk != b->pcg_chunk) {
-        if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-        if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (!gexec_ || graph_chunk_ != b->pcg_chunk) {
+        if (gexec_) { (void)hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
+        if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         const int rc = body();
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(s, &graph);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
         if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
-        b->graph = graph;
-        HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-        b->graph_chunk = b->pcg_chunk;
+        graph_ = graph;
+        HIP_TRY(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+        graph_chunk_ = b->pcg_chunk;
     }
-    HIP_TRY(hipGraphLaunch(b->gexec, s));
+    HIP_TRY(hipGraphLaunch(gexec_, s));
+    return HEAT_OK;
+}
+
+int batch_chunk(heat_batch* b) {
+    return batch_chunk_t<double>(b, b->views, b->dviews, b->graph, b->gexec, b->graph_chunk);
+}
+
+// chunks until every solve of the batch reports done; the flag of chunk k is
+// read while chunk k+1 runs
+template <typename Chunk>
+int batch_until_done(heat_batch* b, int max_iter, Chunk chunk) {
+    hipStream_t s = b->stream;
+    const int max_chunks = max_iter / b->pcg_chunk + 3;
+    int rc = chunk();
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(&b->hflag[0], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipEventRecord(b->ev[0], s));
+    for (int kk = 1;; ++kk) {
+        if (kk > max_chunks) return fail(HEAT_E_STATE, "batch PCG did not report completion");
+        rc = chunk();
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(&b->hflag[kk & 1], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipEventRecord(b->ev[kk & 1], s));
+        HIP_TRY(hipEventSynchronize(b->ev[(kk - 1) & 1]));
+        if (b->hflag[(kk - 1) & 1]) break;
+    }
+    return HEAT_OK;
+}
+
+// the FP32 views of a batch of handles (refined steps), made on first use
+int batch_views32(heat_batch* b) {
+    if (!b->views32.empty()) return HEAT_OK;
+    if (b->mesh) return fail(HEAT_E_STATE, "refined batch steps need a batch of handles (heat_batch_setup)");
+    const int count = (int)b->views.size();
+    for (int i = 0; i < count; ++i) {
+        heat_op* op = b->ops[i];
+        int rc = ensure_mixed(op);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(op->stream));
+        FusedView<float> v = make_view<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32, nullptr,
+                                              op->d32, op->cr32, op->ck32, nullptr);
+        v.sp = b->views[i].sp;
+        v.sp_stride = b->views[i].sp_stride;
+        v.tp = b->views[i].tp;
+        v.tp_stride = b->views[i].tp_stride;
+        v.active = b->views[i].active;
+        b->views32.push_back(v);
+    }
+    HIP_TRY(hipMalloc(&b->dviews32, count * sizeof(FusedView<float>)));
+    HIP_TRY(hipMemcpy(b->dviews32, b->views32.data(), count * sizeof(FusedView<float>), hipMemcpyHostToDevice));
     return HEAT_OK;
 }
 
@@ -1632,26 +1706,57 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
         CHECK_LAUNCH();
         k_batch_residual<<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
         CHECK_LAUNCH();
-        // chunks until every solve is done; the flag of chunk k is read while chunk k+1 runs
-        const int max_chunks = max_iter / b->pcg_chunk + 3;
-        int rc = batch_chunk(b);
+        const int rc = batch_until_done(b, max_iter, [&]() { return batch_chunk(b); });
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(&b->hflag[0], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipEventRecord(b->ev[0], s));
-        for (int kk = 1;; ++kk) {
-            if (kk > max_chunks) return fail(HEAT_E_STATE, "batch PCG did not report completion");
-            rc = batch_chunk(b);
-            if (rc) return rc;
-            HIP_TRY(hipMemcpyAsync(&b->hflag[kk & 1], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
-            HIP_TRY(hipEventRecord(b->ev[kk & 1], s));
-            HIP_TRY(hipEventSynchronize(b->ev[(kk - 1) & 1]));
-            if (b->hflag[(kk - 1) & 1]) break;
-        }
         k_batch_iters<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, b->diters);
         CHECK_LAUNCH();
         HIP_TRY(hipMemcpyAsync(b->hiters, b->diters, count * sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         if (iters) memcpy(iters + (size_t)step * count, b->hiters, count * sizeof(int));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return HEAT_OK;
+}
+
+int heat_batch_step_mixed(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
+                          int* inner_iters) {
+    if (!b || nsteps < 0 || max_outer < 0 || max_inner < 0) return fail(HEAT_E_ARG, "bad batch step arguments");
+    int rc = batch_views32(b);
+    if (rc) return rc;
+    const int count = (int)b->views.size();
+    hipStream_t s = b->stream;
+    for (heat_op* op : b->ops) HIP_TRY(hipStreamSynchronize(op->stream));
+    const int total_sb = b->sb_off[count];
+    const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
+    for (int step = 0; step < nsteps; ++step) {
+        k_batch_rhs<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->d_node_off, count);
+        CHECK_LAUNCH();
+        k_batch_params_mixed<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, tol, tol_inner, max_inner);
+        CHECK_LAUNCH();
+        // outer passes in step for every open solve, as heat_pcg_solve_mixed's
+        // loop: FP64 residual, FP32 inner solve (whose first check may find the
+        // outer tolerance met), then x += e unless the pass limit is reached
+        for (int nout = 0;; ++nout) {
+            k_batch_mixed_init<<<total_sb, NT, 0, s>>>(b->dviews, b->dviews32, b->d_sb_off, count);
+            CHECK_LAUNCH();
+            rc = batch_until_done(b, max_inner, [&]() {
+                return batch_chunk_t<float>(b, b->views32, b->dviews32, b->graph32, b->gexec32, b->graph_chunk32);
+            });
+            if (rc) return rc;
+            if (nout >= max_outer) break;
+            k_batch_mixed_correct<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->dviews32, b->d_node_off, count);
+            CHECK_LAUNCH();
+            k_batch_outer_done<<<1, NT, 0, s>>>(b->dviews, count, b->dflag);
+            CHECK_LAUNCH();
+            HIP_TRY(hipMemcpyAsync(&b->hflag[0], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (b->hflag[0]) break;
+        }
+        k_batch_inner<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, b->diters);
+        CHECK_LAUNCH();
+        HIP_TRY(hipMemcpyAsync(b->hiters, b->diters, count * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (inner_iters) memcpy(inner_iters + (size_t)step * count, b->hiters, count * sizeof(int));
     }
     HIP_TRY(hipStreamSynchronize(s));
     return HEAT_OK;

@@ -926,47 +926,147 @@ __global__ __launch_bounds__(NT) void k_batch_residual(const FusedView<double>* 
     }
 }
 
-// the two launches of K1 over a batch's tiles (no upset is armed in a batch)
-template <bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ) void k_batch_dir(const FusedView<double>* __restrict__ views,
+// the two launches of K1 over a batch's tiles (no upset is armed in a batch);
+// T = double: the FP64 solves, T = float: the inner solves of refined ones
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ) void k_batch_dir(const FusedView<T>* __restrict__ views,
                                                         const int* __restrict__ tile_off, int count) {
     __shared__ double sd[5][NT];
     const int total = tile_off[count];
     const int gb = xcd_slab_block(total);
     if (gb >= total) return;
     const int i = find_solve(tile_off, count, gb);
-    fused_dir_body<double, DETECT>(views[i], gb - tile_off[i], sd);
+    fused_dir_body<T, DETECT>(views[i], gb - tile_off[i], sd);
 }
 
-template <bool DETECT>
-__global__ __launch_bounds__(FTY* FTZ, 3) void k_batch_tdma(const FusedView<double>* __restrict__ views,
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(FTY* FTZ, 3) void k_batch_tdma(const FusedView<T>* __restrict__ views,
                                                             const int* __restrict__ tile_off, int count) {
-    __shared__ TdmaHalf<double, false, FTY, FTZ> H0, H1;
+    __shared__ TdmaHalf<T, false, FTY, FTZ> H0, H1;
     __shared__ double sd[2][NT];
     const int total = tile_off[count];
     const int gb = xcd_slab_block(total);
     if (gb >= total) return;
     const int i = find_solve(tile_off, count, gb);
-    fused_tdma_body<double, DETECT>(views[i], gb - tile_off[i], H0, H1, sd);
+    fused_tdma_body<T, DETECT>(views[i], gb - tile_off[i], H0, H1, sd);
 }
 
-template <bool DETECT>
-__global__ __launch_bounds__(NT) void k_batch_update(const FusedView<double>* __restrict__ views,
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(NT) void k_batch_update(const FusedView<T>* __restrict__ views,
                                                      const int* __restrict__ sb_off, int count) {
     if ((int)blockIdx.x >= sb_off[count]) return;
     const int i = find_solve(sb_off, count, blockIdx.x);
-    fused_update_body<double, DETECT>(views[i], blockIdx.x - sb_off[i]);
+    fused_update_body<T, DETECT>(views[i], blockIdx.x - sb_off[i]);
 }
 
 // one block per solve (solves with few partials skip: their blocks reduce in place)
-__global__ __launch_bounds__(NT) void k_batch_fin_sp(const FusedView<double>* __restrict__ views) {
-    const FusedView<double>& v = views[blockIdx.x];
-    if (v.pre_sp) fused_fin_sp_body<double>(v);
+template <typename T>
+__global__ __launch_bounds__(NT) void k_batch_fin_sp(const FusedView<T>* __restrict__ views) {
+    const FusedView<T>& v = views[blockIdx.x];
+    if (v.pre_sp) fused_fin_sp_body<T>(v);
 }
 
-__global__ __launch_bounds__(NT) void k_batch_fin_tp(const FusedView<double>* __restrict__ views) {
-    const FusedView<double>& v = views[blockIdx.x];
-    if (v.pre_tp) fused_fin_tp_body<double, false>(v);
+template <typename T>
+__global__ __launch_bounds__(NT) void k_batch_fin_tp(const FusedView<T>* __restrict__ views) {
+    const FusedView<T>& v = views[blockIdx.x];
+    if (v.pre_tp) fused_fin_tp_body<T, false>(v);
+}
+
+// ---- FP32-refined solves of a batch (each solve as heat_pcg_solve_mixed) ----
+// v64: the solve's FP64 vectors (x = T, b, cr, ck); v32: its FP32 inner
+// solve (x = e, r, z, d, the partial arrays) -- both views share the state.
+
+// per-solve parameters of a refined batch step (one thread per solve)
+__global__ void k_batch_params_mixed(const FusedView<double>* __restrict__ v64, int count, double tol,
+                                     double tol_inner, int max_inner) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    FusedState* fs = v64[i].fs;
+    fs->tol = tol_inner;
+    fs->max_iter = max_inner;
+    fs->check_outer = 1;
+    fs->tol_outer = tol;
+    fs->inner_total = 0;
+    fs->inner_base = 0;
+    fs->fi_armed = 0;
+    fs->fi_fired = 0;
+    fs->outer_done = v64[i].active ? 0 : 1;
+    fs->done = v64[i].active ? 0 : 1;
+}
+
+// Outer pass start of every refined solve that is still open: the FP64
+// residual r = b - P A x at each node (the colour-ordered sum, as the single
+// solve's apply), then e = 0, r32, z32 and the five super-block partials of
+// k_fused_mixed_init; grid = all super-blocks.
+__global__ __launch_bounds__(NT) void k_batch_mixed_init(const FusedView<double>* __restrict__ v64,
+                                                         const FusedView<float>* __restrict__ v32,
+                                                         const int* __restrict__ sb_off, int count) {
+    __shared__ double sd[5][NT];
+    if ((int)blockIdx.x >= sb_off[count]) return;
+    const int i = find_solve(sb_off, count, blockIdx.x);
+    const FusedView<double>& V = v64[i];
+    const FusedView<float>& F = v32[i];
+    FusedState* fs = V.fs;
+    if (fs->outer_done) return;                      // finished (or masked): keep its state
+    const int j = blockIdx.x - sb_off[i];
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const size_t base = (size_t)j * SB + threadIdx.x;
+    for (int s = 0; s < SBS; ++s) {
+        const size_t n = base + (size_t)s * NT;
+        if (n < (size_t)V.g.nn) {
+            int ix, iy, iz;
+            node_coords(V.g, (int)n, ix, iy, iz);
+            const double ax = node_is_boundary(V.g, ix, iy, iz)
+                                  ? 0.0 : node_apply_any<double, false>(V.x, V.cr, V.ck, V.g, ix, iy, iz);
+            const double bi = V.b[n];
+            const double ri = fma(-1.0, ax, bi);
+            const float rf = (float)ri;
+            const float zf = rf / F.d[n];
+            F.x[n] = 0.0f;
+            F.r[n] = rf;
+            F.z[n] = zf;
+            a[0] += (double)rf * (double)rf;
+            a[1] += (double)rf * (double)rf;
+            a[2] += (double)rf * (double)zf;
+            a[3] += bi * bi;
+            a[4] += ri * ri;
+        }
+    }
+    double t[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = tree256(sd, k, a[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) F.sp[(size_t)k * F.sp_stride + j] = t[k];
+        if (j == 0) {
+            fs->iter = 0;
+            fs->done = 0;
+            fs->inner_base = fs->inner_total;
+        }
+    }
+}
+
+// x += e for every node of every solve still open (k_mixed_correct)
+__global__ __launch_bounds__(NT) void k_batch_mixed_correct(const FusedView<double>* __restrict__ v64,
+                                                            const FusedView<float>* __restrict__ v32,
+                                                            const int* __restrict__ node_off, int count) {
+    const int gid = blockIdx.x * NT + threadIdx.x;
+    if (gid >= node_off[count]) return;
+    const int i = find_solve(node_off, count, gid);
+    if (v64[i].fs->outer_done) return;
+    const int n = gid - node_off[i];
+    v64[i].x[n] += (double)v32[i].x[n];
+}
+
+// *flag = 1 when every refined solve of the batch has met the outer tolerance
+__global__ void k_batch_outer_done(const FusedView<double>* __restrict__ views, int count, int* flag) {
+    __shared__ int open;
+    if (threadIdx.x == 0) open = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        if (!views[i].fs->outer_done) atomicOr(&open, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *flag = open ? 0 : 1;
 }
 
 // per-solve PCG parameters of a batch step (one thread per solve)
@@ -987,7 +1087,8 @@ __global__ void k_batch_params(const FusedView<double>* __restrict__ views, int 
 }
 
 // *flag = 1 when every solve of the batch is done
-__global__ void k_batch_all_done(const FusedView<double>* __restrict__ views, int count, int* flag) {
+template <typename T>
+__global__ void k_batch_all_done(const FusedView<T>* __restrict__ views, int count, int* flag) {
     __shared__ int open;
     if (threadIdx.x == 0) open = 0;
     __syncthreads();

@@ -11,7 +11,8 @@ Workload (the ensemble row of the measurement plan): COUNT members, mesh size
 drawn uniformly from {16, 24, 32, 48, 64, 96}, kappa log-normal with sigma
 drawn from {0.5, 1, 2} and the reference's correlated model (L = 0.125,
 scripts/ensemble/run_ensemble.py:35-36), rho*c = 1, f = 1, T^0 = 0,
-dt = 1e-3, 20 steps, PCG to 1e-8.
+dt = 1e-3, 20 steps, PCG to 1e-8: FP32-refined (FP32 inner Jacobi-PCG to
+1e-4 inside FP64 defect correction), or plain FP64 with --precision fp64.
 """
 from __future__ import annotations
 
@@ -39,9 +40,26 @@ def members(count=96, seed=1, dt=1e-3, sizes=SIZES, sigmas=SIGMAS):
     return out
 
 
-def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log=None):
+def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log=None, precision="refined",
+        tol_inner=1e-4, max_outer=50):
     """Time the batch (and, for comparison, the same members one after the
-    other through the single-solve API) and return a summary."""
+    other through the single-solve API) and return a summary.  precision
+    "refined": every solve FP32-refined (iteration counts are inner ones);
+    "fp64": plain FP64 PCG."""
+    refined = precision == "refined"
+    if precision not in ("refined", "fp64"):
+        raise ValueError("precision must be 'refined' or 'fp64'")
+
+    def batch_steps(batch, nsteps):
+        if refined:
+            return batch.step_mixed(nsteps, max_outer, max_iter, tol, tol_inner)
+        return batch.step(nsteps, max_iter, tol)
+
+    def solve_one(op):
+        if refined:
+            return op.pcg_solve_mixed("b", "T", max_outer, max_iter, tol, tol_inner)[1]
+        return op.pcg_solve("b", "T", max_iter, tol)
+
     from .heat import HeatBatch
     ms = members(count, seed)
     t0 = time.perf_counter()
@@ -52,18 +70,18 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
     for op, z in zip(batch.ops, zero):
         op.upload("T", z)
         op.synchronize()
-    batch.step(1, max_iter, tol)                 # warm-up (graph capture, clocks)
+    batch_steps(batch, 1)                        # warm-up (graph capture, clocks)
     for op, z in zip(batch.ops, zero):
         op.upload("T", z)
         op.synchronize()
     t0 = time.perf_counter()
-    its = batch.step(steps, max_iter, tol)
+    its = batch_steps(batch, steps)
     batch_s = time.perf_counter() - t0
     if log:
-        log(f"batch: {count} members, {nodes} nodes, {steps} steps in {batch_s * 1e3:.1f} ms")
+        log(f"batch ({precision}): {count} members, {nodes} nodes, {steps} steps in {batch_s * 1e3:.1f} ms")
     dof_it = int(sum(op.nn * int(its[:, i].sum()) for i, op in enumerate(batch.ops)))
     out = {
-        "members": count, "steps": steps, "nodes": int(nodes), "tol": tol,
+        "members": count, "steps": steps, "nodes": int(nodes), "tol": tol, "precision": precision,
         "dof_iterations": dof_it,
         "gdof_iterations_per_s": round(dof_it / batch_s / 1e9, 3),
         "sizes": sorted({m["n"] for m in ms}),
@@ -85,7 +103,7 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
         for op in batch.ops:
             for _ in range(steps):
                 op.build_rhs("T", "b")
-                seq_its += op.pcg_solve("b", "T", max_iter, tol)
+                seq_its += solve_one(op)
             op.synchronize()
         seq_s = time.perf_counter() - t0
         same = all(np.array_equal(op.download("T"), f) for op, f in zip(batch.ops, finals))
@@ -105,9 +123,11 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--precision", choices=("refined", "fp64"), default="refined")
     a = ap.parse_args(argv)
     log = lambda m: print(f"[ensemble] {m}", file=sys.stderr, flush=True)
-    print(json.dumps(run(a.members, a.steps, a.tol, seed=a.seed, sequential=not a.no_sequential, log=log)))
+    print(json.dumps(run(a.members, a.steps, a.tol, seed=a.seed, sequential=not a.no_sequential, log=log,
+                         precision=a.precision)))
 
 
 if __name__ == "__main__":

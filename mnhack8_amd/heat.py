@@ -93,6 +93,7 @@ SIGNATURES = {
     "heat_batch_set_active": (_I, [_P, _P]),
     "heat_batch_set_tuning": (_I, [_P, _I, _I]),
     "heat_batch_step": (_I, [_P, _I, _I, _D, _P]),
+    "heat_batch_step_mixed": (_I, [_P, _I, _I, _I, _D, _D, _P]),
     "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
@@ -420,6 +421,15 @@ class HeatBatch:
         (nsteps, count) iteration counts (-1: masked)."""
         it = np.zeros((max(nsteps, 1), self.count), dtype=np.int32)
         _check(self._lib.heat_batch_step(self.h, int(nsteps), int(max_iter), float(tol), it.ctypes.data))
+        return it[:nsteps]
+
+    def step_mixed(self, nsteps=1, max_outer=50, max_inner=500, tol=1e-8, tol_inner=1e-4):
+        """nsteps backward-Euler steps of every active solve, each an
+        FP32-refined solve (HeatOperator.pcg_solve_mixed); returns the
+        (nsteps, count) inner-iteration totals (-1: masked)."""
+        it = np.zeros((max(nsteps, 1), self.count), dtype=np.int32)
+        _check(self._lib.heat_batch_step_mixed(self.h, int(nsteps), int(max_outer), int(max_inner), float(tol),
+                                               float(tol_inner), it.ctypes.data))
         return it[:nsteps]
 
     def cleanup(self):

@@ -607,6 +607,51 @@ def test_batch_bitwise_with_oracle(graph):
     batch.cleanup()
 
 
+def _oracle_steps_mixed(orc, T, nsteps, max_outer, max_inner, tol, tol_inner):
+    its = []
+    for _ in range(nsteps):
+        b = orc.build_rhs(T)
+        _, inner, T = orc.pcg_mixed_fused(b, T, max_outer, max_inner, tol, tol_inner)
+        its.append(inner)
+    return its, T
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_mixed_bitwise_with_oracle(graph):
+    """FP32-refined steps of the whole ensemble at once: per-member inner
+    iteration totals and T after each step bit-identical to the oracle's
+    refined solve (fused order) of that member alone."""
+    print(f"[case] refined batch graph={graph}", flush=True)
+    batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS)
+    batch.set_tuning(4, graph)
+    nsteps, max_outer, max_inner, tol, tol_inner = 3, 8, 200, 1e-10, 1e-4
+    its = batch.step_mixed(nsteps, max_outer, max_inner, tol, tol_inner)
+    for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
+        ito, To = _oracle_steps_mixed(orc, T0, nsteps, max_outer, max_inner, tol, tol_inner)
+        print(f"  member {i} n={op.n}: inner iterations {list(its[:, i])} (oracle {ito})", flush=True)
+        assert list(its[:, i]) == ito
+        assert np.array_equal(op.download("T"), To)
+    batch.cleanup()
+
+
+def test_batch_mixed_mask_and_pass_limit():
+    """A masked member keeps its T through refined steps; with one outer pass
+    allowed every member stops as the single refined solve does."""
+    print("[case] refined batch mask / pass limit", flush=True)
+    batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS[:4])
+    mask = [1, 1, 0, 1]
+    batch.set_active(mask)
+    its = batch.step_mixed(2, 1, 200, 1e-10, 1e-4)
+    for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
+        if not mask[i]:
+            assert (its[:, i] == -1).all()
+            assert np.array_equal(op.download("T"), T0)
+            continue
+        ito, To = _oracle_steps_mixed(orc, T0, 2, 1, 200, 1e-10, 1e-4)
+        assert list(its[:, i]) == ito and np.array_equal(op.download("T"), To)
+    batch.cleanup()
+
+
 def test_batch_mask():
     """A masked member keeps its T and reports -1; the others go on exactly
     as without it."""
