@@ -239,6 +239,16 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
  * (optional) receives NSTEPS x COUNT inner-iteration totals (-1: masked). */
 int heat_batch_step_mixed(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
                           int* inner_iters);
+/* NSTEPS backward-Euler steps of a shared-mesh batch of exactly 16 right-hand
+ * sides (heat_batch_setup_rhs) solved together (new, no counterpart in the
+ * reference): FP32 Jacobi-PCG on all sixteen columns with the matrix-core
+ * operator of heat_apply_mrhs, per-column alpha, beta and convergence, inside
+ * per-column FP64 defect correction to TOL (inner tolerance TOL_INNER).  With
+ * a handle built with detectors, every inner apply is checked by D1 per
+ * column.  INNER_ITERS and D1MAX (optional) receive NSTEPS x 16 inner-iteration
+ * totals and largest D1 ratios. */
+int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
+                         int* inner_iters, double* d1max);
 /* Launch tuning (new): PCG iterations per convergence poll (1..256) and
  * whether a chunk is replayed as a captured hipGraph. */
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph);

@@ -20,6 +20,7 @@
 #include "heat_kernels.h"
 #include "heat_pcg_fused.h"
 #include "heat_mrhs.h"
+#include "heat_mrhs_solve.h"
 
 using namespace heat;
 
@@ -733,6 +734,22 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     return HEAT_OK;
 }
 
+int ensure_mixed(heat_op* op);
+
+// the matrix-core 16-RHS apply of a handle's mesh on stream s (FP32 coefficients ready)
+int launch_mrhs(heat_op* op, const float* X, float* Y, bool mass, hipStream_t s) {
+    const int n = op->g.n;
+    const int nby = (n + 1 + MTY - 1) / MTY, nbz = (n + 1 + MTZ - 1) / MTZ, nch = (n + 1 + MCX - 1) / MCX;
+    const long long nt = (long long)nby * nbz * nch;
+    if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
+    const int ntiles = (int)nt;
+    const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
+    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    else k_mrhs_apply<false><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
 int ensure_mixed(heat_op* op) {
     if (op->e32) return HEAT_OK;
     const Grid& g = op->g;
@@ -1159,16 +1176,7 @@ int heat_apply_mrhs(heat_op* op, const float* X, float* Y, double dt_eff) {
         return fail(HEAT_E_ARG, "dt_eff must be 0 or the handle's dt (%g), got %g", op->cfg.dt, dt_eff);
     int rc = ensure_mixed(op);
     if (rc) return rc;
-    const int n = op->g.n;
-    const int nby = (n + 1 + MTY - 1) / MTY, nbz = (n + 1 + MTZ - 1) / MTZ, nch = (n + 1 + MCX - 1) / MCX;
-    const long long nt = (long long)nby * nbz * nch;
-    if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
-    const int ntiles = (int)nt;
-    const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
-    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
-    else k_mrhs_apply<false><<<grid, MNT, 0, op->stream>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
-    CHECK_LAUNCH();
-    return HEAT_OK;
+    return launch_mrhs(op, X, Y, mass, op->stream);
 }
 
 int heat_time_mrhs(heat_op* op, const float* X, float* Y, int reps, double* avg_ms) {
@@ -1389,6 +1397,11 @@ struct heat_batch {
     hipGraphExec_t gexec32 = nullptr;
     int graph_chunk32 = 0;
     heat_op* mesh = nullptr;                 // shared-mesh batch: the borrowed handle
+    // 16-column refined solve of a shared-mesh batch (node-major FP32 columns)
+    float *mrE = nullptr, *mrR = nullptr, *mrZ = nullptr, *mrP = nullptr, *mrAP = nullptr;
+    double* mrpart = nullptr;
+    MrState* mrst = nullptr;
+    MrState* hmrst = nullptr;                // pinned
     std::vector<void*> owned;                // shared-mesh batch: member vectors and states
 };
 
@@ -1402,6 +1415,9 @@ void batch_free(heat_batch* b) {
     if (b->gexec32) (void)hipGraphExecDestroy(b->gexec32);
     if (b->graph32) (void)hipGraphDestroy(b->graph32);
     (void)hipFree(b->dviews32);
+    (void)hipFree(b->mrE); (void)hipFree(b->mrR); (void)hipFree(b->mrZ); (void)hipFree(b->mrP); (void)hipFree(b->mrAP);
+    (void)hipFree(b->mrpart); (void)hipFree(b->mrst);
+    if (b->hmrst) (void)hipHostFree(b->hmrst);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     if (b->hflag) (void)hipHostFree(b->hflag);
     if (b->hiters) (void)hipHostFree(b->hiters);
@@ -1608,7 +1624,8 @@ int heat_batch_setup_rhs(heat_op* op, int count, const double* F, heat_batch** o
     if (!op || !out || count < 1) return fail(HEAT_E_ARG, "bad batch arguments");
     *out = nullptr;
     if (!gather_path(op)) return fail(HEAT_E_STATE, "the batch runs the colour-ordered fast path only");
-    if (op->cfg.detect) return fail(HEAT_E_STATE, "the batch runs without detectors");
+    // (a handle with detectors is accepted: heat_batch_step runs without the D1
+    // check, heat_batch_step_mrhs applies it with the handle's w_A)
     heat_batch* b = new heat_batch();
     b->mesh = op;
     auto bail = [&](int rc) { batch_free(b); delete b; return rc; };
@@ -1715,6 +1732,97 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
         if (iters) memcpy(iters + (size_t)step * count, b->hiters, count * sizeof(int));
     }
     HIP_TRY(hipStreamSynchronize(s));
+    return HEAT_OK;
+}
+
+int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
+                         int* inner_iters, double* d1max) {
+    if (!b || nsteps < 0 || max_outer < 0 || max_inner < 0) return fail(HEAT_E_ARG, "bad batch step arguments");
+    const int count = (int)b->views.size();
+    if (!b->mesh || count != MRC)
+        return fail(HEAT_E_STATE, "the 16-column refined solve needs a shared-mesh batch of exactly %d members", MRC);
+    heat_op* m = b->mesh;
+    for (int i = 0; i < count; ++i)
+        if (!b->views[i].active) return fail(HEAT_E_STATE, "the 16-column refined solve has no member masks");
+    int rc = ensure_mixed(m);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    const Grid& g = m->g;
+    hipStream_t s = b->stream;
+    if (!b->mrE) {
+        const size_t bytes = sizeof(float) * (size_t)g.nn * MRC;
+        HIP_TRY(hipMalloc(&b->mrE, bytes));
+        HIP_TRY(hipMalloc(&b->mrR, bytes));
+        HIP_TRY(hipMalloc(&b->mrZ, bytes));
+        HIP_TRY(hipMalloc(&b->mrP, bytes));
+        HIP_TRY(hipMalloc(&b->mrAP, bytes));
+        HIP_TRY(hipMalloc(&b->mrpart, sizeof(double) * MRK * MRC * MRB));
+        HIP_TRY(hipMalloc(&b->mrst, sizeof(MrState)));
+        HIP_TRY(hipHostMalloc(&b->hmrst, sizeof(MrState), hipHostMallocDefault));
+    }
+    const bool detect = m->cfg.detect != 0;
+    const double* w = m->vec[HEAT_VEC_WA];
+    const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
+    auto read_state = [&]() -> int {
+        HIP_TRY(hipMemcpyAsync(b->hmrst, b->mrst, sizeof(MrState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return HEAT_OK;
+    };
+    for (int step = 0; step < nsteps; ++step) {
+        k_batch_rhs<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->d_node_off, count);
+        CHECK_LAUNCH();
+        k_mr_params<<<1, 1, 0, s>>>(b->mrst, tol, tol_inner, max_inner, detect ? m->nwA : 0.0);
+        CHECK_LAUNCH();
+        for (int nout = 0;; ++nout) {
+            k_mr_init<<<MRB, NT, 0, s>>>(b->dviews, m->cr, m->ck, m->d32, g, b->mrE, b->mrR, b->mrZ, b->mrP,
+                                         b->mrpart, b->mrst);
+            CHECK_LAUNCH();
+            k_mr_fin_init<<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
+            CHECK_LAUNCH();
+            k_mr_flags<<<1, 1, 0, s>>>(b->mrst);
+            CHECK_LAUNCH();
+            if ((rc = read_state())) return rc;
+            // inner iterations in chunks of pcg_chunk, until every column is done
+            for (int it = 0; !b->hmrst->all_done; it += b->pcg_chunk) {
+                if (it > max_inner + b->pcg_chunk) return fail(HEAT_E_STATE, "16-column PCG did not report completion");
+                for (int k = 0; k < b->pcg_chunk; ++k) {
+                    if ((rc = launch_mrhs(m, b->mrP, b->mrAP, false, s))) return rc;
+                    if (detect) {
+                        k_mr_pap<true><<<MRB, NT, 0, s>>>(b->mrP, b->mrAP, w, g, b->mrpart, b->mrst);
+                        CHECK_LAUNCH();
+                        k_mr_fin_pap<true><<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
+                    } else {
+                        k_mr_pap<false><<<MRB, NT, 0, s>>>(b->mrP, b->mrAP, w, g, b->mrpart, b->mrst);
+                        CHECK_LAUNCH();
+                        k_mr_fin_pap<false><<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
+                    }
+                    CHECK_LAUNCH();
+                    k_mr_update<<<MRB, NT, 0, s>>>(b->mrE, b->mrP, b->mrR, b->mrAP, b->mrZ, m->d32, g, b->mrpart,
+                                                   b->mrst);
+                    CHECK_LAUNCH();
+                    k_mr_fin_update<<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
+                    CHECK_LAUNCH();
+                    k_mr_dir<<<MRB, NT, 0, s>>>(b->mrZ, b->mrP, g, b->mrst);
+                    CHECK_LAUNCH();
+                }
+                k_mr_flags<<<1, 1, 0, s>>>(b->mrst);
+                CHECK_LAUNCH();
+                if ((rc = read_state())) return rc;
+            }
+            if (nout >= max_outer) break;
+            k_mr_correct<<<MRB, NT, 0, s>>>(b->dviews, b->mrE, g, b->mrst);
+            CHECK_LAUNCH();
+            k_mr_flags<<<1, 1, 0, s>>>(b->mrst);
+            CHECK_LAUNCH();
+            if ((rc = read_state())) return rc;
+            if (b->hmrst->all_outer_done) break;
+        }
+        if ((rc = read_state())) return rc;
+        for (int c = 0; c < MRC; ++c) {
+            if (inner_iters) inner_iters[(size_t)step * MRC + c] = b->hmrst->inner_total[c];
+            if (d1max) d1max[(size_t)step * MRC + c] = b->hmrst->d1max[c];
+        }
+    }
     return HEAT_OK;
 }
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "heat_batch_set_tuning": (_I, [_P, _I, _I]),
     "heat_batch_step": (_I, [_P, _I, _I, _D, _P]),
     "heat_batch_step_mixed": (_I, [_P, _I, _I, _I, _D, _D, _P]),
+    "heat_batch_step_mrhs": (_I, [_P, _I, _I, _I, _D, _D, _P, _P]),
     "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
@@ -431,6 +432,17 @@ class HeatBatch:
         _check(self._lib.heat_batch_step_mixed(self.h, int(nsteps), int(max_outer), int(max_inner), float(tol),
                                                float(tol_inner), it.ctypes.data))
         return it[:nsteps]
+
+    def step_mrhs(self, nsteps=1, max_outer=50, max_inner=500, tol=1e-8, tol_inner=1e-4):
+        """nsteps backward-Euler steps of a shared-mesh batch of 16 right-hand
+        sides solved together: FP32 inner PCG with the matrix-core operator in
+        FP64 defect correction.  Returns the (nsteps, 16) inner-iteration
+        totals and the (nsteps, 16) largest D1 ratios (0 without detectors)."""
+        it = np.zeros((max(nsteps, 1), self.count), dtype=np.int32)
+        d1 = np.zeros((max(nsteps, 1), self.count), dtype=np.float64)
+        _check(self._lib.heat_batch_step_mrhs(self.h, int(nsteps), int(max_outer), int(max_inner), float(tol),
+                                              float(tol_inner), it.ctypes.data, d1.ctypes.data))
+        return it[:nsteps], d1[:nsteps]
 
     def cleanup(self):
         if getattr(self, "h", None):

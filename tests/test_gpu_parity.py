@@ -747,6 +747,54 @@ def test_batch_shared_mesh_rhs(n):
     batch.cleanup()
 
 
+@pytest.mark.parametrize("detect", [False, True])
+def test_mrhs_refined_solve(detect):
+    """Sixteen right-hand sides solved together (FP32 inner PCG on the
+    matrix-core operator, FP64 defect correction per column): after each
+    step every column's FP64 residual, formed by the oracle, meets the outer
+    tolerance, its T agrees with the oracle's FP64 solve, and the D1 ratios
+    of the clean solve stay far below the FP32 threshold."""
+    n, dt, tol = 11, 1e-3, 1e-10
+    print(f"[case] 16-column refined solve n={n} detect={detect}", flush=True)
+    k, rhoc = make_field(n, seed=90)
+    h = heat()
+    op = h.HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0, detect=detect)
+    rng = np.random.default_rng(91)
+    R = 16
+    F = rng.standard_normal((R, op.nn)) * 0.1
+    F[:, op.download("F") == 0] = 0.0
+    batch = h.HeatBatch.shared(op, F)
+    T = [rng.standard_normal(op.nn) * (op.download("F") != 0) for _ in range(R)]
+    for i in range(R):
+        op.upload(batch.vector(i, "T"), T[i])
+    op.synchronize()
+    orcs = []
+    for i in range(R):
+        orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+        orc.set_vector("F", F[i])
+        orcs.append(orc)
+    for step in range(2):
+        its, d1 = batch.step_mrhs(1, 20, 300, tol, 1e-4)
+        print(f"  step {step}: inner iterations {list(its[0])}, largest D1 {d1.max():.3e}", flush=True)
+        assert (its[0] > 0).all()
+        if detect:
+            assert np.isfinite(d1).all() and d1.max() < 4.76e-8
+        else:
+            assert (d1 == 0).all()
+        for i in range(R):
+            b = orcs[i].build_rhs(T[i])
+            Tg = op.download(batch.vector(i, "T"))
+            res = np.linalg.norm(b - orcs[i].apply(Tg, dt)) / np.linalg.norm(b)
+            _, To, _ = orcs[i].pcg(b, T[i], 3000, 1e-14)
+            err = np.linalg.norm(Tg - To) / np.linalg.norm(To)
+            assert res < 1.01 * tol, (i, res)        # the same FP64 residual, summed in another order
+            assert err < 1e-8, (i, err)
+            T[i] = Tg
+    # the FP64 batch step still runs on the same batch afterwards
+    batch.step(1, 300, 1e-10)
+    batch.cleanup()
+
+
 def test_fused_pcg_large_mesh_finalized_sums():
     """At N=200 (3,966 super-blocks, 2,275 tiles) the partial sums are formed
     by one-block finalize launches instead of in every block: the same order,
