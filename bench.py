@@ -233,6 +233,8 @@ def main():
         log("16-RHS apply at N=320 (matrix-core contraction)")
         result["multi_rhs"] = {"apply": mrhs.bench_apply(320, log=log),
                                "solve": mrhs.bench_solve(320, steps=1, log=log)}
+        log("16-column refined solve at N=320 (matrix-core operator, FP64 defect correction, D1 on): 1 step")
+        result["multi_rhs"]["solve_refined"] = mrhs.bench_solve_refined(320, steps=1, log=log)
 
     if not a.no_cpu:
         try:

@@ -88,11 +88,16 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--n", type=int, default=320)
     ap.add_argument("--solve-steps", type=int, default=0, help="also time this many steps of the 16-RHS solve")
+    ap.add_argument("--refined-steps", type=int, default=0,
+                    help="also time this many steps of the 16-column refined solve (matrix-core operator)")
+    ap.add_argument("--max-outer", type=int, default=12)
     a = ap.parse_args(argv)
     log = lambda m: print(f"[mrhs] {m}", file=sys.stderr, flush=True)
     out = {"apply": bench_apply(a.n, log=log)}
     if a.solve_steps:
         out["solve"] = bench_solve(a.n, a.solve_steps, log=log)
+    if a.refined_steps:
+        out["solve_refined"] = bench_solve_refined(a.n, a.refined_steps, max_outer=a.max_outer, log=log)
     print(json.dumps(out))
 
 
@@ -156,6 +161,65 @@ def bench_solve(n=320, steps=20, tol=1e-8, max_iter=500, log=None):
            "sequential_matches_batch_bitwise": bool(same), "setup_s": round(setup_s, 2)}
     if log:
         log(f"16 RHS x {steps} steps at N={n}: batched {batch_s * 1e3:.0f} ms, one after another {seq_s * 1e3:.0f} ms")
+    op.cleanup()
+    return out
+
+
+def bench_solve_refined(n=320, steps=20, tol=1e-8, max_outer=12, max_inner=500, tol_inner=1e-4, log=None,
+                        fp64_steps=None):
+    """The sixteen right-hand sides as one 16-column refined solve: FP32 inner
+    Jacobi-PCG on the matrix-core operator inside per-column FP64 defect
+    correction, ABFT D1 on every inner apply.  The same steps of the fused
+    FP64 batch give the reference the refined result's error is stated
+    against."""
+    from .heat import HeatBatch, HeatOperator
+    k, rhoc = workload_fields(n)
+    dt = 1e-4
+    op = HeatOperator(n, dt, k.ravel(), rhoc.ravel(), kernel="fast", scatter="colored", source="const",
+                      source_value=1.0, detect=True)
+    F = load_vectors(n)
+    batch = HeatBatch.shared(op, F)
+    its, d1s, times = [], [], []
+    for st in range(steps):
+        t0 = time.perf_counter()
+        it, d1 = batch.step_mrhs(1, max_outer, max_inner, tol, tol_inner)
+        times.append(time.perf_counter() - t0)
+        its.append(it[0])
+        d1s.append(d1[0])
+        if log:
+            log(f"refined step {st + 1}/{steps}: {times[-1] * 1e3:.0f} ms, inner iterations max {int(it.max())} "
+                f"min {int(it.min())}, largest D1 {float(d1.max()):.2e}")
+    refined = [op.download(batch.vector(i, "T")) for i in range(NRHS)]
+    batch.cleanup()
+    # the same steps in plain FP64 (fused batch) for the error figure
+    ref = HeatBatch.shared(op, F)
+    f64_s = 0.0
+    nref = steps if fp64_steps is None else fp64_steps
+    for st in range(nref):
+        t0 = time.perf_counter()
+        ref.step(1, max_inner * max_outer, tol)
+        f64_s += time.perf_counter() - t0
+        if log:
+            log(f"FP64 step {st + 1}/{nref} done ({f64_s * 1e3:.0f} ms so far)")
+    errs = []
+    if nref == steps:
+        for i in range(NRHS):
+            t64 = op.download(ref.vector(i, "T"))
+            errs.append(float(np.linalg.norm(refined[i] - t64) / max(np.linalg.norm(t64), 1e-300)))
+    ref.cleanup()
+    its, d1s = np.array(its), np.array(d1s)
+    total = float(sum(times))
+    out = {"n": n, "rhs": NRHS, "steps": steps, "tol": tol, "tol_inner": tol_inner, "max_outer": max_outer,
+           "max_inner": max_inner, "nodes": op.nn, "refined_ms": round(total * 1e3, 1),
+           "rhs_steps_per_s": round(NRHS * steps / total, 2),
+           "inner_iterations_total": int(its.sum()),
+           "inner_iterations_max_per_step": [int(x) for x in its.max(axis=1)],
+           "d1_max": float(d1s.max()), "d1_threshold_fp32": 4.76e-8,
+           "fp64_ms": round(f64_s * 1e3, 1), "fp64_steps": nref}
+    if errs:
+        out["rel_err_vs_fp64_max"] = max(errs)
+    if log:
+        log(f"16-column refined: {steps} steps in {total * 1e3:.0f} ms; FP64 batch {f64_s * 1e3:.0f} ms")
     op.cleanup()
     return out
 
