@@ -184,6 +184,30 @@ def main():
         "schedule": {"picked": picked, "key": sched_key, "us_per_apply": round(pick[3] * 1e3, 2)},
     }
 
+    # the reference's two element-centric scatter forms on the same mesh and
+    # vector: the 8-launch colour sweep and the atomic scatter (both kept)
+    op.set_colored_impl("sweep")
+    op.time_matvec("p", "Ap", 3)
+    ms_sweep, _ = op.time_matvec("p", "Ap", 50)
+    y_col = op.download("Ap")
+    op.set_colored_impl(pick[0])
+    op.set_march_chunk(int(pick[1]))
+    op.set_tile_shape(shape_id)
+    opa = HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="atomic", source="const", source_value=1.0)
+    opa.upload("p", x)
+    opa.time_matvec("p", "Ap", 3)
+    ms_atomic, _ = opa.time_matvec("p", "Ap", 50)
+    y_at = opa.download("Ap")
+    opa.cleanup()
+    result["scatter_compare"] = {
+        "node_centric_colour_order": {"us_per_apply": round(avg_ms * 1e3, 2), "gdof_s": round(gdofs, 2)},
+        "colour_sweep_8_launches": {"us_per_apply": round(ms_sweep * 1e3, 2),
+                                    "gdof_s": round(nn / (ms_sweep * 1e-3) / 1e9, 2)},
+        "atomic_scatter": {"us_per_apply": round(ms_atomic * 1e3, 2), "gdof_s": round(nn / (ms_atomic * 1e-3) / 1e9, 2),
+                           "max_rel_diff_vs_colour": float(np.max(np.abs(y_at - y_col)) / np.max(np.abs(y_col)))},
+    }
+    log(f"scatter forms: colour sweep {ms_sweep * 1e3:.1f} us, atomic {ms_atomic * 1e3:.1f} us")
+
     if not a.no_solve:
         # time to relative residual 1e-8 for the first time step (T^0 = 0, b = dt F)
         op.upload("T", np.zeros(nn))
