@@ -588,12 +588,14 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
 // LDS-tiled plane walk with asynchronous copies (k_apply_tile's sums again).
 //
 // Same tiles, window and arithmetic as k_apply_tile, but the block's x-plane
-// and element-plane tiles reach LDS through buffer loads to LDS (the hardware
-// copies global memory straight into LDS; no register holds the data).  The
-// tiles of two consecutive planes form a batch; while a batch is computed,
-// the copies of the next batch are in flight into the other half of a
-// two-half LDS ring.  Ordering needs only "all of this wave's copies done"
-// (s_waitcnt 0) and one barrier per batch.
+// and element-plane tiles reach LDS through 16-byte buffer loads to LDS (the
+// hardware copies global memory straight into LDS; no register holds the
+// data; tile rows are padded to whole copies).  The tiles of two consecutive
+// planes form a batch; while a batch is computed, the copies of the next
+// batch are in flight into the other half of a two-half LDS ring.  Ordering
+// needs only "all of this wave's copies done" (s_waitcnt 0) and one barrier
+// per batch.  The launch sizes the chunks so that the blocks fill whole waves
+// of resident blocks (tdma_chunks on the host).
 // ---------------------------------------------------------------------------
 template <typename T, int TY, int TZ>
 struct TdmaGeom {
