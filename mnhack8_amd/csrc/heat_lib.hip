@@ -736,16 +736,29 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
 
 int ensure_mixed(heat_op* op);
 
-// the matrix-core 16-RHS apply of a handle's mesh on stream s (FP32 coefficients ready)
-int launch_mrhs(heat_op* op, const float* X, float* Y, bool mass, hipStream_t s) {
-    const int n = op->g.n;
-    const int nby = (n + 1 + MTY - 1) / MTY, nbz = (n + 1 + MTZ - 1) / MTZ, nch = (n + 1 + MCX - 1) / MCX;
-    const long long nt = (long long)nby * nbz * nch;
+// tiles of the matrix-core 16-RHS apply of a mesh
+long long mrhs_tiles(const Grid& g, int* nby, int* nbz) {
+    const int n = g.n;
+    *nby = (n + 1 + MTY - 1) / MTY;
+    *nbz = (n + 1 + MTZ - 1) / MTZ;
+    const int nch = (n + 1 + MCX - 1) / MCX;
+    return (long long)*nby * *nbz * nch;
+}
+
+// the matrix-core 16-RHS apply of a handle's mesh on stream s (FP32 coefficients ready);
+// SUMS (1, 2): per-tile column sums into part (see k_mrhs_apply)
+int launch_mrhs(heat_op* op, const float* X, float* Y, bool mass, hipStream_t s, int sums = 0,
+                double* part = nullptr, const double* w = nullptr) {
+    int nby, nbz;
+    const long long nt = mrhs_tiles(op->g, &nby, &nbz);
     if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
     const int ntiles = (int)nt;
     const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
-    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
-    else k_mrhs_apply<false><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, op->g, nby, nbz, ntiles);
+    const Grid& g = op->g;
+    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles);
+    else if (sums == 1) k_mrhs_apply<false, 1><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, part, w);
+    else if (sums == 2) k_mrhs_apply<false, 2><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, part, w);
+    else k_mrhs_apply<false><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles);
     CHECK_LAUNCH();
     return HEAT_OK;
 }
@@ -1400,6 +1413,7 @@ struct heat_batch {
     // 16-column refined solve of a shared-mesh batch (node-major FP32 columns)
     float *mrE = nullptr, *mrR = nullptr, *mrZ = nullptr, *mrP = nullptr, *mrAP = nullptr;
     double* mrpart = nullptr;
+    double* mrtile = nullptr;                // the apply's tile partials (4 x 16 x tiles)
     MrState* mrst = nullptr;
     MrState* hmrst = nullptr;                // pinned
     std::vector<void*> owned;                // shared-mesh batch: member vectors and states
@@ -1416,7 +1430,7 @@ void batch_free(heat_batch* b) {
     if (b->graph32) (void)hipGraphDestroy(b->graph32);
     (void)hipFree(b->dviews32);
     (void)hipFree(b->mrE); (void)hipFree(b->mrR); (void)hipFree(b->mrZ); (void)hipFree(b->mrP); (void)hipFree(b->mrAP);
-    (void)hipFree(b->mrpart); (void)hipFree(b->mrst);
+    (void)hipFree(b->mrpart); (void)hipFree(b->mrtile); (void)hipFree(b->mrst);
     if (b->hmrst) (void)hipHostFree(b->hmrst);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     if (b->hflag) (void)hipHostFree(b->hflag);
@@ -1757,11 +1771,15 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
         HIP_TRY(hipMalloc(&b->mrP, bytes));
         HIP_TRY(hipMalloc(&b->mrAP, bytes));
         HIP_TRY(hipMalloc(&b->mrpart, sizeof(double) * MRK * MRC * MRB));
+        int nby_, nbz_;
+        HIP_TRY(hipMalloc(&b->mrtile, sizeof(double) * MRK * MRC * (size_t)mrhs_tiles(g, &nby_, &nbz_)));
         HIP_TRY(hipMalloc(&b->mrst, sizeof(MrState)));
         HIP_TRY(hipHostMalloc(&b->hmrst, sizeof(MrState), hipHostMallocDefault));
     }
     const bool detect = m->cfg.detect != 0;
     const double* w = m->vec[HEAT_VEC_WA];
+    int nby_mr, nbz_mr;
+    const int ntiles_mr = (int)mrhs_tiles(g, &nby_mr, &nbz_mr);
     const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
     auto read_state = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(b->hmrst, b->mrst, sizeof(MrState), hipMemcpyDeviceToHost, s));
@@ -1786,16 +1804,10 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
             for (int it = 0; !b->hmrst->all_done; it += b->pcg_chunk) {
                 if (it > max_inner + b->pcg_chunk) return fail(HEAT_E_STATE, "16-column PCG did not report completion");
                 for (int k = 0; k < b->pcg_chunk; ++k) {
-                    if ((rc = launch_mrhs(m, b->mrP, b->mrAP, false, s))) return rc;
-                    if (detect) {
-                        k_mr_pap<true><<<MRB, NT, 0, s>>>(b->mrP, b->mrAP, w, g, b->mrpart, b->mrst);
-                        CHECK_LAUNCH();
-                        k_mr_fin_pap<true><<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
-                    } else {
-                        k_mr_pap<false><<<MRB, NT, 0, s>>>(b->mrP, b->mrAP, w, g, b->mrpart, b->mrst);
-                        CHECK_LAUNCH();
-                        k_mr_fin_pap<false><<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
-                    }
+                    // Ap = P A p with the tiles' p.Ap [and D1] sums, then alpha
+                    if ((rc = launch_mrhs(m, b->mrP, b->mrAP, false, s, detect ? 2 : 1, b->mrtile, w))) return rc;
+                    if (detect) k_mr_fin_pap<true><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
+                    else k_mr_fin_pap<false><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
                     CHECK_LAUNCH();
                     k_mr_update<<<MRB, NT, 0, s>>>(b->mrE, b->mrP, b->mrR, b->mrAP, b->mrZ, m->d32, g, b->mrpart,
                                                    b->mrst);

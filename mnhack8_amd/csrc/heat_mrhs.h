@@ -46,10 +46,17 @@ __device__ __forceinline__ float mrhs_coef(int row, int j) {
     return d == 0 ? -4.0f : (d == 1 ? 0.0f : 1.0f);
 }
 
-template <bool MASS>
+// SUMS = 0: Y = P A X only.  SUMS = 1: also each tile's per-column sums of
+// X.Y over its own nodes; SUMS = 2: also 1.Y, w.X and X.X (the D1 terms).
+// Partial row k of column c of tile b lands in part[(k * 16 + c) * ntiles + b]:
+// each thread sums its nodes in plane order, a wave combines its lanes of one
+// column, and the block adds its waves in order (a fixed order).
+template <bool MASS, int SUMS = 0>
 __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X, float* __restrict__ Y,
                                                     const float* __restrict__ cr, const float* __restrict__ ck,
-                                                    Grid g, int nby, int nbz, int ntiles) {
+                                                    Grid g, int nby, int nbz, int ntiles,
+                                                    double* __restrict__ part = nullptr,
+                                                    const double* __restrict__ w = nullptr) {
     constexpr int TY = MTY, TZ = MTZ, CX = MCX;
     constexpr int WY = TY + 2, WZ = TZ + 2, WN = WY * WZ;          // node window with halo
     constexpr int EY = TY + 1, EZ = TZ + 1, EN = EY * EZ;          // elements per plane
@@ -66,7 +73,7 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     constexpr int XR = (V4 + MNT - 1) / MNT;
     constexpr int ER = (EN + MNT - 1) / MNT;
     __shared__ float xs[2 * XSTRIDE];
-    __shared__ float ys[YALL];
+    __shared__ __align__(16) float ys[YALL];
     __shared__ float ec[2][2][EN];
     const int b = xcd_slab_block(ntiles);
     if (b >= ntiles) return;
@@ -80,6 +87,12 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
     const int gy = q & 1, gz = q >> 1;                               // output node pair (y, z) of group q
     const float a0 = mrhs_coef(col, q), a1 = mrhs_coef(col, 4 + q);  // A: row = lane % 16, k = lane / 16
+    constexpr int NK = SUMS == 2 ? 4 : 1;                            // sums per column
+    double acc_s[NK][4];                                             // this thread's 4 columns: part = tid & 3
+#pragma unroll
+    for (int k = 0; k < NK; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc_s[k][j] = 0.0;
 
     // ---- staging: node window of plane p and element coefficients of plane e
     float4 xf[XR];
@@ -212,13 +225,57 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
                     const bool bnd = ex == 0 || ex == n || iy == 0 || iy == n || iz == 0 || iz == n;
                     const float4 v = bnd ? make_float4(0.f, 0.f, 0.f, 0.f) : *src;
                     *reinterpret_cast<float4*>(Y + ((size_t)ex * S + (size_t)iy * nn1 + iz) * MR + part * 4) = v;
+                    if (SUMS) {
+                        // X of this node: window node (ty + 1, tz + 1) of plane ex (not yet replaced)
+                        const float4 xv = *reinterpret_cast<const float4*>(
+                            xs + (ex & 1) * XSTRIDE + ((ty + 1) * WZ + (tz + 1)) * MR + part * 4);
+                        const float yv[4] = {v.x, v.y, v.z, v.w}, xw[4] = {xv.x, xv.y, xv.z, xv.w};
+                        const double wn = SUMS == 2 ? w[(size_t)ex * S + (size_t)iy * nn1 + iz] : 0.0;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            acc_s[0][j] += (double)xw[j] * (double)yv[j];
+                            if (SUMS == 2) {
+                                acc_s[NK > 1 ? 1 : 0][j] += (double)yv[j];
+                                acc_s[NK > 2 ? 2 : 0][j] += wn * (double)xw[j];
+                                acc_s[NK > 3 ? 3 : 0][j] += (double)xw[j] * (double)xw[j];
+                            }
+                        }
+                    }
                 }
                 *src = make_float4(0.f, 0.f, 0.f, 0.f);
             }
         }
+        // with the sums, every thread must have read plane ex's window before
+        // plane ex + 2 replaces it
+        if (SUMS) __syncthreads();
         park_x(ex & 1);                                          // plane ex + 2 replaces plane ex
         park_e((ex + 1) & 1);                                    // element plane ex + 1
         __syncthreads();
+    }
+    if (SUMS) {
+        // lanes of one wave with the same column group (lane & 3) combine, then
+        // the block's waves are added in order; ys is free after the walk
+#pragma unroll
+        for (int k = 0; k < NK; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int off = 4; off < 64; off <<= 1) acc_s[k][j] += __shfl_xor(acc_s[k][j], off);
+        double* red = reinterpret_cast<double*>(ys);            // [wave][k][column]
+        constexpr int NW = MNT / 64;
+        if (lane < 4) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) red[(wave * NK + k) * MR + lane * 4 + j] = acc_s[k][j];
+        }
+        __syncthreads();
+        if (tid < NK * MR) {
+            const int k = tid / MR, c = tid - k * MR;
+            double t = 0.0;
+            for (int wv = 0; wv < NW; ++wv) t += red[(wv * NK + k) * MR + c];
+            part[((size_t)k * MR + c) * ntiles + b] = t;
+        }
     }
 }
 

@@ -10,7 +10,8 @@
 //   outer pass: r = b - P A x in FP64 (the colour-ordered sum), e = 0,
 //     r32 = (float) r, z32 = r32 / d, p = z; stop a column whose FP64
 //     residual meets the outer tolerance (or is not finite);
-//   inner iteration: Ap = P A p (matrix cores); p.Ap [and D1]; alpha;
+//   inner iteration: Ap = P A p (matrix cores; p.Ap [and the D1 sums] per
+//     tile in the same kernel); alpha;
 //     e += alpha p, r -= alpha Ap, z = r / d; r.r, r.z; convergence, beta;
 //     p = z + beta p;
 //   then x += e for every column still open.
@@ -58,16 +59,17 @@ __device__ __forceinline__ void mr_block_partials(double (*sd)[NT], const double
     }
 }
 
-// one block per column: the MRB partials of rows 0..K-1 folded (thread t sums
-// t, t + 256, ... in order, then a 256-wide tree); all threads get the sums
+// one block per column: the COUNT partials of rows 0..K-1 (layout
+// [k][c][count]) folded (thread t sums t, t + 256, ... in order, then a
+// 256-wide tree); all threads get the sums
 template <int K>
 __device__ __forceinline__ void mr_column_sums(double (*sd)[NT], const double* __restrict__ part, int c,
-                                               double out[K]) {
+                                               double out[K], int count = MRB) {
     const int t = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double a = 0.0;
-        for (int i = t; i < MRB; i += NT) a += part[((size_t)k * MRC + c) * MRB + i];
+        for (int i = t; i < count; i += NT) a += part[((size_t)k * MRC + c) * count + i];
         sd[k][t] = a;
     }
     __syncthreads();
@@ -146,37 +148,14 @@ __global__ __launch_bounds__(NT) void k_mr_fin_init(const double* __restrict__ p
     }
 }
 
-// p.Ap of every open column [and the D1 sums 1_int.Ap, w.p, p.p]
+// part: the matrix-core apply's tile partials [p.Ap | 1.Ap | w.p | p.p] x ntiles
 template <bool DETECT>
-__global__ __launch_bounds__(NT) void k_mr_pap(const float* __restrict__ P, const float* __restrict__ AP,
-                                               const double* __restrict__ w, Grid g, double* __restrict__ part,
-                                               const MrState* __restrict__ st) {
-    __shared__ double sd[MRK][NT];
-    const int c = threadIdx.x & (MRC - 1);
-    double a[MRK] = {0.0, 0.0, 0.0, 0.0};
-    if (!st->done[c]) {
-        for (int n = blockIdx.x * (NT / MRC) + (threadIdx.x >> 4); n < g.nn; n += MRB * (NT / MRC)) {
-            const size_t i = (size_t)n * MRC + c;
-            const double pi = (double)P[i], api = (double)AP[i];
-            a[0] += pi * api;
-            if (DETECT) {
-                int ix, iy, iz;
-                node_coords(g, n, ix, iy, iz);
-                if (!node_is_boundary(g, ix, iy, iz)) a[1] += api;
-                a[2] += w[n] * pi;
-                a[3] += pi * pi;
-            }
-        }
-    }
-    mr_block_partials<MRK>(sd, a, part);
-}
-
-template <bool DETECT>
-__global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ part, MrState* st) {
+__global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ part, int ntiles, MrState* st) {
     __shared__ double sd[MRK][NT];
     const int c = blockIdx.x;
-    double o[MRK];
-    mr_column_sums<MRK>(sd, part, c, o);
+    double o[MRK] = {0.0, 0.0, 0.0, 0.0};
+    if (DETECT) mr_column_sums<MRK>(sd, part, c, o, ntiles);
+    else mr_column_sums<1>(sd, part, c, o, ntiles);
     if (threadIdx.x == 0 && !st->done[c]) {
         if (DETECT) {
             // D1 of this apply, recorded before the breakdown check (as abft_record)
