@@ -11,7 +11,8 @@
 // reference hard-codes 1e-8 and 500), --source-const Q, --ic sine,
 // --precision fp64|fp32 (FP32-refined solve), --inner-tol X,
 // --colored-impl gather|sweep|march|tile|tdma, --pcg-chunk K, --no-graph,
-// --pcg-order fused|reference (summation order of the PCG inner products).
+// --pcg-order fused|reference (summation order of the PCG inner products),
+// --campaign FILE (many fault runs of one field in one process, below).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -51,6 +52,7 @@ struct Opts {
     int mixed = 0;
     double inner_tol = 1e-4;
     int colored_impl = -1, pcg_chunk = 8, use_graph = 1, pcg_order = HEAT_PCG_FUSED;
+    std::string campaign;
 };
 
 void usage(const char* prog) {
@@ -62,7 +64,7 @@ void usage(const char* prog) {
             "   [--diag-cg FILE]   (one CG solve, logs iteration,rel_residual to FILE, then exits)\n"
             "   [--tol X] [--max-iter N] [--source-const Q] [--precision fp64|fp32] [--inner-tol X]\n"
             "   [--colored-impl gather|sweep|march|tile|tdma] [--pcg-chunk K] [--no-graph]\n"
-            "   [--pcg-order fused|reference]\n"
+            "   [--pcg-order fused|reference] [--campaign FILE]\n"
             "  LEVEL: 1=G global memory, 2=R accumulator, 3=H halo staging buffer\n"
             "  BIT:   0-51 mantissa, 52-62 exponent, 63 sign\n",
             prog);
@@ -234,6 +236,126 @@ bool write_field(const std::vector<double>& T, const std::string& outdir, const 
     return true;
 }
 
+// Field statistics and labels shared by every case of one invocation.
+struct CaseLabels {
+    int n_steps, nprocs;
+    const char *kname, *sname;
+    double kmin, kmax, rmin, rmax;
+};
+
+// fi_trace_<tag>.csv, fi_summary_<tag>.csv and meta_<tag>.txt of one twin run
+// into DIR, and its stdout summary (src/heat_solver_het.cu:1520-1621). o holds
+// this run's fault (fi_level 0: a clean run without F). False when a file
+// cannot be written.
+bool write_case(const Opts& o, const std::string& dir, const CaseLabels& c, const RunResult& rA,
+                const RunResult& rB, const RunResult& rF, int on_bnd) {
+    const bool fault = o.fi_level > 0;
+    double Efl_max = 0, Efa_max = 0, Efa_int = 0, Efa_final = 0, Efl_final = 0, spread_at_max = 0,
+           spread_final = 0;
+    int step_max = -1, iters_delta_max = 0;
+    for (size_t s = 0; s < rB.E.size(); ++s) {
+        Efl_max = std::fmax(Efl_max, rB.E[s]);
+        Efl_final = rB.E[s];
+        if (fault) {
+            const double E = rF.E[s];
+            if (std::isfinite(E) && E > Efa_max) {
+                Efa_max = E;
+                step_max = rB.snap_steps[s];
+                spread_at_max = rF.spread[s];
+            }
+            Efa_final = E;
+            spread_final = rF.spread[s];
+            const double w = (s + 1 < rB.E.size()) ? (rB.snap_steps[s + 1] - rB.snap_steps[s]) * o.dt : 0.0;
+            if (std::isfinite(E)) Efa_int += E * w;
+        }
+    }
+    for (size_t s = 0; fault && s < rF.cg_iters.size(); ++s)
+        iters_delta_max = std::max(iters_delta_max, rF.cg_iters[s] - rA.cg_iters[s]);
+    const double thr = std::fmax(o.thr_user, 10.0 * Efl_max);
+    const char* cls = "clean";
+    if (fault) {
+        if (rF.nonfinite) cls = "detected";
+        else if (Efa_max <= thr) cls = "benign";
+        else if (Efa_final <= thr) cls = "transient";
+        else cls = "persistent";
+    }
+    const RunResult& rX = fault ? rF : rA;
+    const std::string tp = dir + "/fi_trace_" + o.tag + ".csv";
+    FILE* tf = fopen(tp.c_str(), "w");
+    if (!tf) {
+        fprintf(stderr, "cannot write %s\n", tp.c_str());
+        return false;
+    }
+    fprintf(tf, "step,t,E_noise_floor,E_fault,spread_F,cg_A,cg_B,cg_F,d1_abft_A,d1_abft_F,d2_resid_A,"
+                "d2_resid_F,d3_tmin_F,d3_tmax_F,hyb_block_err_A,hyb_block_err_F,hyb_d4_F\n");
+    for (size_t s = 0; s < rB.E.size(); ++s) {
+        const int st = rB.snap_steps[s];
+        fprintf(tf, "%d,%.6e,%.10e,%.10e,%.4f,%d,%d,%d", st, st * o.dt, rB.E[s], fault ? rF.E[s] : 0.0,
+                fault ? rF.spread[s] : 0.0, rA.cg_iters[st], rB.cg_iters[st], fault ? rF.cg_iters[st] : -1);
+        if (o.detect)
+            fprintf(tf, ",%.3e,%.3e,%.3e,%.3e,%.6e,%.6e", rA.d1_abft[st], rX.d1_abft[st], rA.d2_resid[st],
+                    rX.d2_resid[st], rX.d3_tmin[st], rX.d3_tmax[st]);
+        else
+            fprintf(tf, ",,,,,,");
+        fprintf(tf, ",,,\n");
+    }
+    fclose(tf);
+    double meanA = 0;
+    for (int it : rA.cg_iters) meanA += it;
+    meanA /= (double)std::max<size_t>(1, rA.cg_iters.size());
+    const std::string sp = dir + "/fi_summary_" + o.tag + ".csv";
+    FILE* sf = fopen(sp.c_str(), "w");
+    if (!sf) {
+        fprintf(stderr, "cannot write %s\n", sp.c_str());
+        return false;
+    }
+    fprintf(sf, "tag,N,dt,n_steps,nprocs,kernel,scatter,reproject_bc,kmin,kmax,rhocmin,rhocmax,"
+                "fi_level,fi_rank,fi_step,fi_target,fi_bit,target_on_boundary,"
+                "E_floor_max,E_floor_final,E_fault_max,step_at_max,E_fault_int,E_fault_final,"
+                "thr,iters_delta_max,mean_cg_iters_A,solve_ms_A,solve_ms_B,solve_ms_F,class,spread_at_max,spread_final,"
+                "detect,d1_first,d2_first,d3_first,d1_max_F,d2_max_F,d1_max_A,d2_max_A,detected_by,"
+                "hybrid,hyb_block_err_max_A,hyb_block_err_max_F,d4_first_F\n");
+    fprintf(sf, "%s,%d,%.3e,%d,%d,%s,%s,%d,%g,%g,%g,%g,%d,%d,%d,%ld,%d,%d,"
+                "%.6e,%.6e,%.6e,%d,%.6e,%.6e,%.3e,%d,%.2f,%.3f,%.3f,%.3f,%s",
+            o.tag.c_str(), o.N, o.dt, c.n_steps, c.nprocs, c.kname, c.sname, o.reproject, c.kmin, c.kmax, c.rmin, c.rmax,
+            o.fi_level, o.fi_rank, o.fi_step, o.fi_target, o.fi_bit, on_bnd, Efl_max, Efl_final, Efa_max,
+            step_max, Efa_int, Efa_final, thr, iters_delta_max, meanA, rA.solve_ms, rB.solve_ms,
+            rF.solve_ms, cls);
+    fprintf(sf, ",%.4f,%.4f", spread_at_max, spread_final);
+    std::string by;
+    if (o.detect) {
+        if (rX.d1_first >= 0) by += "D1";
+        if (rX.d2_first >= 0) by += (by.empty() ? "" : "+") + std::string("D2");
+        if (rX.d3_first >= 0) by += (by.empty() ? "" : "+") + std::string("D3");
+        if (by.empty()) by = "none";
+    } else {
+        by = "off";
+    }
+    fprintf(sf, ",%d,%d,%d,%d,%.3e,%.3e,%.3e,%.3e,%s", o.detect, rX.d1_first, rX.d2_first, rX.d3_first,
+            rX.d1_max, rX.d2_max, rA.d1_max, rA.d2_max, by.c_str());
+    fprintf(sf, ",%d,%.3e,%.3e,%d\n", 0, 0.0, 0.0, -1);
+    if (o.detect)
+        printf("detectors    D1 first %d (max %.2e)  D2 first %d (max %.2e)  D3 first %d  -> %s   "
+               "[clean A: D1 %.2e D2 %.2e]\n",
+               rX.d1_first, rX.d1_max, rX.d2_first, rX.d2_max, rX.d3_first, by.c_str(), rA.d1_max, rA.d2_max);
+    fclose(sf);
+    printf("\n=== Done (%s) ===\n", o.tag.c_str());
+    printf("solve wall A/B/F (ms): %.1f / %.1f / %.1f   mean CG iters A: %.2f\n", rA.solve_ms, rB.solve_ms,
+           rF.solve_ms, meanA);
+    printf("noise floor  max %.3e  final %.3e\n", Efl_max, Efl_final);
+    if (fault)
+        printf("fault        max %.3e @step %d  int %.3e  final %.3e  iters+%d  bnd %d  -> %s\n", Efa_max,
+               step_max, Efa_int, Efa_final, iters_delta_max, on_bnd, cls);
+    const std::string mp = dir + "/meta_" + o.tag + ".txt";
+    FILE* mf = fopen(mp.c_str(), "w");
+    if (mf) {
+        fprintf(mf, "N %d\nnprocs %d\ndt %g\nn_steps %d\nfield %s\nkernel %s\nscatter %s\n", o.N, c.nprocs,
+                o.dt, c.n_steps, o.field.c_str(), c.kname, c.sname);
+        fclose(mf);
+    }
+    return true;
+}
+
 bool is_boundary(int N, int ix, int iy, int iz) {
     return ix == 0 || ix == N || iy == 0 || iy == N || iz == 0 || iz == N;
 }
@@ -326,6 +448,7 @@ int parse(int argc, char** argv, Opts& o) {
             o.pcg_order = !strcmp(argv[++i], "reference") ? HEAT_PCG_REFERENCE_ORDER : HEAT_PCG_FUSED;
         } else if (!strcmp(a, "--pcg-chunk") && has(1)) { o.pcg_chunk = atoi(argv[++i]);
         } else if (!strcmp(a, "--no-graph")) { o.use_graph = 0;
+        } else if (!strcmp(a, "--campaign") && has(1)) { o.campaign = argv[++i];
         } else if (!strcmp(a, "--qrank") || !strcmp(a, "--qblock")) {
             fprintf(stderr, "%s: the hybrid quantum-rank mode is not part of this solver\n", a);
             return 1;
@@ -353,7 +476,69 @@ int parse(int argc, char** argv, Opts& o) {
         fprintf(stderr, "BIT must be in 0..63\n");
         return 1;
     }
+    if (!o.campaign.empty() && (o.fi_level > 0 || o.dump_snaps || o.verify || !o.diag_cg.empty())) {
+        fprintf(stderr, "--campaign takes its faults from FILE and runs without --fi, --dump-snaps, --verify "
+                        "and --diag-cg\n");
+        return 1;
+    }
     return 0;
+}
+
+// --campaign FILE: the fault runs of one field in one process. Each line of
+// FILE is "DIR LEVEL STEP TARGET BIT". The clean runs A and B are the same
+// for every fault of the field (the solve is deterministic), so they run once;
+// each fault run F then writes the files of a single invocation with
+// --fi LEVEL STEP TARGET BIT into DIR (which must exist), except the T_final
+// dumps. This is the process-per-run loop of scripts/ensemble/run_ensemble.py:78-104
+// with one process per field. A line that a single invocation would reject
+// (level other than 1 or 2, BIT outside 0..63) is reported and skipped, with
+// no files written, and the return status is 1.
+int run_campaign(Solver& S, int n_steps, const CaseLabels& lab, const std::vector<std::vector<double>>& snapA,
+                 const RunResult& rA, const RunResult& rB) {
+    FILE* f = fopen(S.o.campaign.c_str(), "r");
+    if (!f) {
+        fprintf(stderr, "cannot read %s\n", S.o.campaign.c_str());
+        return 1;
+    }
+    struct Run {
+        std::string dir;
+        int level, step, bit;
+        long target;
+    };
+    std::vector<Run> runs;
+    char dir[1024];
+    int level, step, bit;
+    long target;
+    int fields;
+    while ((fields = fscanf(f, "%1023s %d %d %ld %d", dir, &level, &step, &target, &bit)) == 5)
+        runs.push_back({dir, level, step, bit, target});
+    const bool bad_line = fields != EOF;
+    fclose(f);
+    if (bad_line) {
+        fprintf(stderr, "%s: every line must read DIR LEVEL STEP TARGET BIT\n", S.o.campaign.c_str());
+        return 1;
+    }
+    int status = 0;
+    const double t0 = now_ms();
+    for (size_t i = 0; i < runs.size(); ++i) {
+        const Run& r = runs[i];
+        if ((r.level != 1 && r.level != 2) || r.bit < 0 || r.bit > 63) {
+            fprintf(stderr, "[campaign] %s: level %d bit %d cannot run on one rank\n", r.dir.c_str(), r.level, r.bit);
+            status = 1;
+            continue;
+        }
+        Opts oc = S.o;
+        oc.fi_level = r.level;
+        oc.fi_step = r.step;
+        oc.fi_target = r.target;
+        oc.fi_bit = r.bit;
+        RunResult rF;
+        run_case(S, n_steps, r.level, r.step, r.target, r.bit, nullptr, &snapA, rF, "fault", &rA);
+        if (!write_case(oc, r.dir, lab, rA, rB, rF, target_on_boundary(oc))) status = 1;
+        printf("[campaign] %zu/%zu %s  %.1f s\n", i + 1, runs.size(), r.dir.c_str(), (now_ms() - t0) / 1e3);
+        fflush(stdout);
+    }
+    return status;
 }
 
 int run(const Opts& o) {
@@ -466,124 +651,22 @@ int run(const Opts& o) {
         // twin-run protocol, src/heat_solver_het.cu:1505-1621
         std::vector<std::vector<double>> snapA;
         RunResult rA, rB, rF;
+        const CaseLabels lab = {n_steps, nprocs, kname, sname, kmin, kmax, rmin, rmax};
         run_case(S, n_steps, 0, -1, 0, 0, &snapA, nullptr, rA, "cleanA", nullptr);
-        if (!write_field(snapA.back(), o.outdir, o.tag + "_clean")) status = 1;
+        if (o.campaign.empty() && !write_field(snapA.back(), o.outdir, o.tag + "_clean")) status = 1;
         run_case(S, n_steps, 0, -1, 0, 0, nullptr, &snapA, rB, "cleanB", &rA);
         int on_bnd = -1;
-        if (o.fi_level > 0 && !status) {
+        if (!o.campaign.empty()) {
+            status = run_campaign(S, n_steps, lab, snapA, rA, rB);
+        } else if (o.fi_level > 0 && !status) {
             on_bnd = target_on_boundary(o);
             run_case(S, n_steps, o.fi_level, o.fi_step, o.fi_target, o.fi_bit, nullptr, &snapA, rF, "fault", &rA);
             std::vector<double> Tf;
             S.snapshot(Tf);
             if (!write_field(Tf, o.outdir, o.tag + "_fault")) status = 1;
         }
-        if (!status) {
-            const bool fault = o.fi_level > 0;
-            double Efl_max = 0, Efa_max = 0, Efa_int = 0, Efa_final = 0, Efl_final = 0, spread_at_max = 0,
-                   spread_final = 0;
-            int step_max = -1, iters_delta_max = 0;
-            for (size_t s = 0; s < rB.E.size(); ++s) {
-                Efl_max = std::fmax(Efl_max, rB.E[s]);
-                Efl_final = rB.E[s];
-                if (fault) {
-                    const double E = rF.E[s];
-                    if (std::isfinite(E) && E > Efa_max) {
-                        Efa_max = E;
-                        step_max = rB.snap_steps[s];
-                        spread_at_max = rF.spread[s];
-                    }
-                    Efa_final = E;
-                    spread_final = rF.spread[s];
-                    const double w = (s + 1 < rB.E.size()) ? (rB.snap_steps[s + 1] - rB.snap_steps[s]) * o.dt : 0.0;
-                    if (std::isfinite(E)) Efa_int += E * w;
-                }
-            }
-            for (size_t s = 0; fault && s < rF.cg_iters.size(); ++s)
-                iters_delta_max = std::max(iters_delta_max, rF.cg_iters[s] - rA.cg_iters[s]);
-            const double thr = std::fmax(o.thr_user, 10.0 * Efl_max);
-            const char* cls = "clean";
-            if (fault) {
-                if (rF.nonfinite) cls = "detected";
-                else if (Efa_max <= thr) cls = "benign";
-                else if (Efa_final <= thr) cls = "transient";
-                else cls = "persistent";
-            }
-            const RunResult& rX = fault ? rF : rA;
-            const std::string tp = o.outdir + "/fi_trace_" + o.tag + ".csv";
-            FILE* tf = fopen(tp.c_str(), "w");
-            if (!tf) {
-                fprintf(stderr, "cannot write %s\n", tp.c_str());
-                heat_cleanup(S.op);
-                return 1;
-            }
-            fprintf(tf, "step,t,E_noise_floor,E_fault,spread_F,cg_A,cg_B,cg_F,d1_abft_A,d1_abft_F,d2_resid_A,"
-                        "d2_resid_F,d3_tmin_F,d3_tmax_F,hyb_block_err_A,hyb_block_err_F,hyb_d4_F\n");
-            for (size_t s = 0; s < rB.E.size(); ++s) {
-                const int st = rB.snap_steps[s];
-                fprintf(tf, "%d,%.6e,%.10e,%.10e,%.4f,%d,%d,%d", st, st * o.dt, rB.E[s], fault ? rF.E[s] : 0.0,
-                        fault ? rF.spread[s] : 0.0, rA.cg_iters[st], rB.cg_iters[st], fault ? rF.cg_iters[st] : -1);
-                if (o.detect)
-                    fprintf(tf, ",%.3e,%.3e,%.3e,%.3e,%.6e,%.6e", rA.d1_abft[st], rX.d1_abft[st], rA.d2_resid[st],
-                            rX.d2_resid[st], rX.d3_tmin[st], rX.d3_tmax[st]);
-                else
-                    fprintf(tf, ",,,,,,");
-                fprintf(tf, ",,,\n");
-            }
-            fclose(tf);
-            double meanA = 0;
-            for (int it : rA.cg_iters) meanA += it;
-            meanA /= (double)std::max<size_t>(1, rA.cg_iters.size());
-            const std::string sp = o.outdir + "/fi_summary_" + o.tag + ".csv";
-            FILE* sf = fopen(sp.c_str(), "w");
-            if (!sf) {
-                fprintf(stderr, "cannot write %s\n", sp.c_str());
-                heat_cleanup(S.op);
-                return 1;
-            }
-            fprintf(sf, "tag,N,dt,n_steps,nprocs,kernel,scatter,reproject_bc,kmin,kmax,rhocmin,rhocmax,"
-                        "fi_level,fi_rank,fi_step,fi_target,fi_bit,target_on_boundary,"
-                        "E_floor_max,E_floor_final,E_fault_max,step_at_max,E_fault_int,E_fault_final,"
-                        "thr,iters_delta_max,mean_cg_iters_A,solve_ms_A,solve_ms_B,solve_ms_F,class,spread_at_max,spread_final,"
-                        "detect,d1_first,d2_first,d3_first,d1_max_F,d2_max_F,d1_max_A,d2_max_A,detected_by,"
-                        "hybrid,hyb_block_err_max_A,hyb_block_err_max_F,d4_first_F\n");
-            fprintf(sf, "%s,%d,%.3e,%d,%d,%s,%s,%d,%g,%g,%g,%g,%d,%d,%d,%ld,%d,%d,"
-                        "%.6e,%.6e,%.6e,%d,%.6e,%.6e,%.3e,%d,%.2f,%.3f,%.3f,%.3f,%s",
-                    o.tag.c_str(), o.N, o.dt, n_steps, nprocs, kname, sname, o.reproject, kmin, kmax, rmin, rmax,
-                    o.fi_level, o.fi_rank, o.fi_step, o.fi_target, o.fi_bit, on_bnd, Efl_max, Efl_final, Efa_max,
-                    step_max, Efa_int, Efa_final, thr, iters_delta_max, meanA, rA.solve_ms, rB.solve_ms,
-                    rF.solve_ms, cls);
-            fprintf(sf, ",%.4f,%.4f", spread_at_max, spread_final);
-            std::string by;
-            if (o.detect) {
-                if (rX.d1_first >= 0) by += "D1";
-                if (rX.d2_first >= 0) by += (by.empty() ? "" : "+") + std::string("D2");
-                if (rX.d3_first >= 0) by += (by.empty() ? "" : "+") + std::string("D3");
-                if (by.empty()) by = "none";
-            } else {
-                by = "off";
-            }
-            fprintf(sf, ",%d,%d,%d,%d,%.3e,%.3e,%.3e,%.3e,%s", o.detect, rX.d1_first, rX.d2_first, rX.d3_first,
-                    rX.d1_max, rX.d2_max, rA.d1_max, rA.d2_max, by.c_str());
-            fprintf(sf, ",%d,%.3e,%.3e,%d\n", 0, 0.0, 0.0, -1);
-            if (o.detect)
-                printf("detectors    D1 first %d (max %.2e)  D2 first %d (max %.2e)  D3 first %d  -> %s   "
-                       "[clean A: D1 %.2e D2 %.2e]\n",
-                       rX.d1_first, rX.d1_max, rX.d2_first, rX.d2_max, rX.d3_first, by.c_str(), rA.d1_max, rA.d2_max);
-            fclose(sf);
-            printf("\n=== Done (%s) ===\n", o.tag.c_str());
-            printf("solve wall A/B/F (ms): %.1f / %.1f / %.1f   mean CG iters A: %.2f\n", rA.solve_ms, rB.solve_ms,
-                   rF.solve_ms, meanA);
-            printf("noise floor  max %.3e  final %.3e\n", Efl_max, Efl_final);
-            if (fault)
-                printf("fault        max %.3e @step %d  int %.3e  final %.3e  iters+%d  bnd %d  -> %s\n", Efa_max,
-                       step_max, Efa_int, Efa_final, iters_delta_max, on_bnd, cls);
-            const std::string mp = o.outdir + "/meta_" + o.tag + ".txt";
-            FILE* mf = fopen(mp.c_str(), "w");
-            if (mf) {
-                fprintf(mf, "N %d\nnprocs %d\ndt %g\nn_steps %d\nfield %s\nkernel %s\nscatter %s\n", o.N, nprocs,
-                        o.dt, n_steps, o.field.c_str(), kname, sname);
-                fclose(mf);
-            }
+        if (!status && o.campaign.empty()) {
+            if (!write_case(o, o.outdir, lab, rA, rB, rF, on_bnd)) status = 1;
         }
     }
     heat_cleanup(S.op);

@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Copy the reference's own recorded validation outputs (data files: CSV,
-JSON, text) used as golden vectors into tests/golden/v10/.
+JSON, text) used as golden vectors into tests/golden/v10/ and
+tests/golden/ensemble/.
 
-Source: tests/baseline_validation/v10 of mariagarciapenalva/mnhack8 (the
-all-PASS run recorded on an RTX 3060 with CUDA 13.0).  Only output data is
-copied; no source file of the reference is.
+Sources: tests/baseline_validation/v10 of mariagarciapenalva/mnhack8 (the
+all-PASS run recorded on an RTX 3060 with CUDA 13.0) and its two recorded
+fault-injection campaigns, results/ensemble_N32 and results/ensemble_medium
+(ensemble.csv, trimmed to the columns used here, and agg/*.csv).  Only output
+data is copied; no source file of the reference is.
 
     python tests/golden/collect_fixtures.py /root/reference
 """
@@ -33,13 +36,42 @@ for lv in (1, 2):
 FILES += ["v3/G_boundary/fi_summary_fi.csv", "v3/G_boundary_reproject/fi_summary_fi.csv"]
 
 
+# The two recorded fault-injection campaigns (results/<name>/ensemble.csv of
+# scripts/ensemble/run_ensemble.py, N=32, with agg/ of scripts/ensemble/aggregate.py).
+# Only the columns that the grid, the statistics and the GPU comparison read are kept.
+CAMPAIGNS = ["ensemble_N32", "ensemble_medium"]
+CAMPAIGN_COLUMNS = ["run_id", "field", "sigma", "seed", "bit_class", "status", "fi_level", "fi_bit", "fi_step",
+                    "fi_target", "class", "E_fault_max", "E_fault_final", "detect", "d1_first", "d2_first",
+                    "d3_first", "iters_delta_max", "kmin", "kmax", "mean_cg_iters_A", "step_at_max",
+                    "detected_by", "target_on_boundary"]
+
+
+def collect_campaigns(ref_root, here):
+    import csv
+    for name in CAMPAIGNS:
+        src = os.path.join(ref_root, "results", name)
+        dst = os.path.join(here, "ensemble", name)
+        os.makedirs(os.path.join(dst, "agg"), exist_ok=True)
+        with open(os.path.join(src, "ensemble.csv"), newline="") as f:
+            rows = list(csv.DictReader(f))
+        with open(os.path.join(dst, "ensemble.csv"), "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=CAMPAIGN_COLUMNS, extrasaction="ignore", lineterminator="\n")
+            w.writeheader()
+            w.writerows(rows)
+        for fn in ("sensitivity.csv", "detection.csv", "contrast.csv"):
+            shutil.copyfile(os.path.join(src, "agg", fn), os.path.join(dst, "agg", fn))
+        print(f"{name}: {len(rows)} rows -> {dst}")
+
+
 def main(ref_root):
+    here = os.path.dirname(os.path.abspath(__file__))
     src = os.path.join(ref_root, "tests", "baseline_validation", "v10")
-    dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "v10")
+    dst = os.path.join(here, "v10")
     for rel in FILES:
         os.makedirs(os.path.dirname(os.path.join(dst, rel)), exist_ok=True)
         shutil.copyfile(os.path.join(src, rel), os.path.join(dst, rel))
     print(f"copied {len(FILES)} files to {dst}")
+    collect_campaigns(ref_root, here)
 
 
 if __name__ == "__main__":

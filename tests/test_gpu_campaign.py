@@ -1,0 +1,90 @@
+"""The one-process-per-field fault campaign on the GPU against the reference's
+one-process-per-run protocol (the same driver started once per run with --fi).
+
+Each fault run of `heat_solver_het --campaign` must write the fi_summary row
+and fi_trace file that a single invocation with --fi writes, character for
+character, apart from the wall-clock columns (solve_ms_A/B/F): the clean runs
+A and B, run once per field, are the same runs that every single invocation
+repeats.
+"""
+import csv
+import os
+
+import pytest
+
+from mnhack8_amd import campaign
+
+TIMING = {"solve_ms_A", "solve_ms_B", "solve_ms_F"}
+
+
+def _rows(out):
+    return {r["run_id"]: r for r in csv.DictReader(open(os.path.join(out, "ensemble.csv")))}
+
+
+@pytest.mark.gpu
+def test_campaign_equals_single_invocations(tmp_path):
+    common = ["--N", "12", "--t", "0.01", "--dt", "5e-4", "--sigmas", "1.0", "--seeds", "3", "--reps", "1",
+              "--targets", "1", "--detect"]
+    per_field, per_run = str(tmp_path / "field"), str(tmp_path / "run")
+    print("[case] campaign N=12, one process", flush=True)
+    assert campaign.main(["--out", per_field] + common) == 0
+    print("[case] campaign N=12, one process per run, shard 0/3", flush=True)
+    assert campaign.main(["--out", per_run, "--per-run", "--shard", "0/3"] + common) == 0
+    a, b = _rows(per_field), _rows(per_run)
+    assert len(a) == 36 and len(b) == 12
+    compared = 0
+    for rid, rb in b.items():
+        ra = a[rid]
+        if rb["fi_level"] == "3":
+            assert ra["status"].startswith("error: level H") and rb["status"].startswith("error: command failed")
+            continue
+        assert ra["status"] == rb["status"] == "ok"
+        diff = {k: (ra[k], rb[k]) for k in rb if k not in TIMING and ra[k] != rb[k]}
+        assert not diff, (rid, diff)
+        ta = open(os.path.join(per_field, "runs", rid, "fi_trace_e.csv")).read()
+        tb = open(os.path.join(per_run, "runs", rid, "fi_trace_e.csv")).read()
+        assert ta == tb, rid
+        compared += 1
+    assert compared == 8
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ensemble", "ensemble_N32")
+EXACT = ("detected_by", "d1_first", "d2_first", "d3_first", "iters_delta_max", "target_on_boundary",
+         "mean_cg_iters_A", "kmin", "kmax")
+# E_fault_max at or below this sits at the CG tolerance's noise level (tol
+# 1e-8; flips of the low mantissa bits), where the last bits of every solve
+# decide the class against thr = 1e-13 and the step of the maximum
+NOISE = 1e-10
+
+
+@pytest.mark.gpu
+def test_recorded_campaign_n32(tmp_path):
+    """The reference's recorded N=32 campaign (results/ensemble_N32: three
+    lognormal fields, sigma 0.5 / 1 / 2, 432 grid rows) rerun here: every
+    G and R run against its recorded row."""
+    out = str(tmp_path / "n32")
+    print("[case] recorded N=32 campaign, one process per field", flush=True)
+    assert campaign.main(["--out", out, "--N", "32", "--sigmas", "0.5,1.0,2.0", "--seeds", "1", "--reps", "2",
+                          "--targets", "2", "--detect"]) == 0
+    gold = {r["run_id"]: r for r in csv.DictReader(open(os.path.join(GOLD, "ensemble.csv")))}
+    mine = _rows(out)
+    assert set(mine) == set(gold)
+    n, class_diff = 0, []
+    for rid, g in gold.items():
+        m = mine[rid]
+        if g["fi_level"] == "3":
+            assert m["status"].startswith("error: level H")
+            continue
+        assert m["status"] == "ok"
+        n += 1
+        for k in EXACT:
+            assert m[k] == g[k], (rid, k, m[k], g[k])
+        eg, em = float(g["E_fault_max"]), float(m["E_fault_max"])
+        if eg > NOISE:
+            assert m["class"] == g["class"] and m["step_at_max"] == g["step_at_max"], rid
+            assert abs(em - eg) <= 1e-5 * eg, (rid, em, eg)
+        elif m["class"] != g["class"]:
+            assert em <= NOISE, (rid, em, eg)
+            class_diff.append(rid)
+    assert n == 246                      # unique G and R run ids (bit draws can repeat inside a class)
+    assert len(class_diff) <= 8, class_diff
