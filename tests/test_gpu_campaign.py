@@ -1,5 +1,7 @@
 """The one-process-per-field fault campaign on the GPU against the reference's
-one-process-per-run protocol (the same driver started once per run with --fi).
+one-process-per-run protocol (the same driver started once per run with --fi)
+and against the reference's recorded N=32 campaign; and the driver's
+per-snapshot field dumps.
 
 Each fault run of `heat_solver_het --campaign` must write the fi_summary row
 and fi_trace file that a single invocation with --fi writes, character for
@@ -9,10 +11,12 @@ repeats.
 """
 import csv
 import os
+import subprocess
 
+import numpy as np
 import pytest
 
-from mnhack8_amd import campaign
+from mnhack8_amd import build, campaign, fields, validations
 
 TIMING = {"solve_ms_A", "solve_ms_B", "solve_ms_F"}
 
@@ -88,3 +92,36 @@ def test_recorded_campaign_n32(tmp_path):
             class_diff.append(rid)
     assert n == 246                      # unique G and R run ids (bit draws can repeat inside a class)
     assert len(class_diff) <= 8, class_diff
+
+
+@pytest.mark.gpu
+def test_dump_snaps_files(tmp_path):
+    """--dump-snaps (src/heat_solver_het.cu:1239-1245, read by
+    scripts/quantum/scale_matched.py:49-50): the full field of every snapshot
+    step of each case as snap_<tag>_<case>_<step>_rank0.bin. The clean runs
+    are bitwise equal, the last snapshots are the T_final dumps, and the
+    trace's E_fault is the relative 2-norm of F - A at each snapshot."""
+    N = 12
+    fp = str(tmp_path / "ln12")
+    k, r = fields.lognormal(N, sigma=1.0, L=0.125, seed=5)
+    fields.write_field(fp, k, r, {"n": N})
+    target = validations.node_index(6, 6, 6, N)
+    cmd = [build.DRIVER, str(N), "0.01", "5e-4", fp, str(tmp_path), "--tag", "sm", "--snap", "5", "--kernel", "fast",
+           "--scatter", "colored", "--dump-snaps", "--fi", "1", "4", str(target), "55"]
+    print(f"[case] {' '.join(cmd)}", flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    nn = (N + 1) ** 3
+    snap = lambda case, s: np.fromfile(str(tmp_path / f"snap_sm_{case}_{s:05d}_rank0.bin"))
+    steps = [0, 5, 10, 15, 19]
+    trace = list(csv.DictReader(open(tmp_path / "fi_trace_sm.csv")))
+    assert [int(t["step"]) for t in trace] == steps
+    for s, t in zip(steps, trace):
+        A, B, F = snap("cleanA", s), snap("cleanB", s), snap("fault", s)
+        assert A.size == B.size == F.size == nn
+        assert np.array_equal(A, B)
+        e = np.sqrt(np.sum((F - A) ** 2) / np.sum(A * A))
+        assert abs(e - float(t["E_fault"])) <= 1e-6 * e + 1e-300
+        assert (e > 0) == (s >= 4)
+    assert np.array_equal(snap("cleanA", 19), np.fromfile(str(tmp_path / "T_final_sm_clean_rank0.bin")))
+    assert np.array_equal(snap("fault", 19), np.fromfile(str(tmp_path / "T_final_sm_fault_rank0.bin")))
