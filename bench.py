@@ -157,6 +157,13 @@ def main():
 
     alg_bytes = 8 * (2 * nn + 2 * ne)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    # the same bytes as a plain 16-byte stream (no stencil, no halo): how
+    # close the apply comes to what this memory system streams at this size
+    op.time_stream("p", "Ap", 20)
+    stream_ms = op.time_stream("p", "Ap", 500)
+    stream_gbs = alg_bytes / (stream_ms * 1e-3) / 1e9
+    log(f"stream of the same bytes: {stream_ms * 1e3:.2f} us ({stream_gbs:.0f} GB/s); "
+        f"apply at {stream_ms / avg_ms:.3f} of it")
     achieved_cold = alg_bytes / (cold_ms * 1e-3) / 1e9
     traffic = read_traffic(sched_key)
 
@@ -180,6 +187,10 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
                        "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
+        "stream_ceiling": {"kernel": "k_stream_ceiling (x, rho*c*m3, dt*kappa*k2 read; y written; 16-byte units)",
+                           "us_per_launch": round(stream_ms * 1e3, 2), "achieved_GBs": round(stream_gbs, 1),
+                           "frac": round(stream_gbs / HBM_PEAK_GBS, 4),
+                           "apply_vs_stream": round(stream_ms / avg_ms, 4)},
         "algorithmic_bytes_per_apply": alg_bytes,
         "schedule": {"picked": picked, "key": sched_key, "us_per_apply": round(pick[3] * 1e3, 2)},
     }

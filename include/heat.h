@@ -264,6 +264,14 @@ int heat_set_tile_shape(heat_op* op, int shape);
  * timed alone.  *avg_ms and *min_ms receive the mean and minimum per apply. */
 int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t flush_bytes,
                      double* avg_ms, double* min_ms);
+/* Benchmark helper (new, no reference counterpart): the bandwidth ceiling of
+ * the apply's memory traffic.  Times `reps` back-to-back launches of a plain
+ * streaming kernel that moves exactly the FP64 apply's algorithmic bytes --
+ * reads x (nn values) and the two element coefficient arrays (ne values
+ * each), writes y (nn values) -- with 16-byte loads and stores and no
+ * stencil, halo or reduction.  y receives x plus a mix of the coefficients
+ * (a value of no meaning).  *avg_ms receives the mean per launch. */
+int heat_time_stream(heat_op* op, const double* x, double* y, int reps, double* avg_ms);
 /* Autotuner (new): time every colour-ordered apply schedule (one thread per
  * node, and plane marching with 2..32 planes per block) on this handle's
  * mesh with HIP events, keep the fastest for all later applies, and report it

@@ -1260,6 +1260,54 @@ int heat_abft_threshold(heat_op* op, double* chk_tol) {
     return HEAT_OK;
 }
 
+// the apply's algorithmic traffic as a plain stream: x, cr, ck read, y written
+// in 16-byte units (pairs of doubles; the odd last node by the first thread)
+__global__ __launch_bounds__(256) void k_stream_ceiling(const double* __restrict__ x, const double* __restrict__ cr,
+                                                        const double* __restrict__ ck, double* __restrict__ y,
+                                                        long long nn, long long ne) {
+    const double2* __restrict__ x2 = reinterpret_cast<const double2*>(x);
+    const double2* __restrict__ r2 = reinterpret_cast<const double2*>(cr);
+    const double2* __restrict__ k2 = reinterpret_cast<const double2*>(ck);
+    double2* __restrict__ y2 = reinterpret_cast<double2*>(y);
+    const long long nn2 = nn / 2, ne2 = ne / 2;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long i = t; i < nn2; i += stride) {
+        double2 a = x2[i];
+        if (i < ne2) {
+            const double2 r = r2[i], k = k2[i];
+            a.x += 0.5 * r.x + 0.25 * k.x;
+            a.y += 0.5 * r.y + 0.25 * k.y;
+        }
+        y2[i] = a;
+    }
+    if (t == 0 && (nn & 1)) y[nn - 1] = x[nn - 1];
+}
+
+int heat_time_stream(heat_op* op, const double* x, double* y, int reps, double* avg_ms) {
+    if (!op || !x || !y || reps < 1 || !avg_ms) return fail(HEAT_E_ARG, "bad timing request");
+    if (((uintptr_t)x | (uintptr_t)y) & 15) return fail(HEAT_E_ARG, "x and y must be 16-byte aligned");
+    hipStream_t s = op->stream;
+    const long long nn = op->g.nn, ne = op->g.ne;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < reps; ++i) {
+        k_stream_ceiling<<<2048, 256, 0, s>>>(x, op->cr, op->ck, y, nn, ne);
+    }
+    (void)hipEventRecord(e1, s);
+    int rc = HEAT_OK;
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(HEAT_E_HIP, "event sync failed");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == HEAT_OK && hipGetLastError() != hipSuccess) rc = fail(HEAT_E_HIP, "stream kernel launch failed");
+    return rc;
+}
+
 int heat_time_matvec(heat_op* op, const double* x, double* y, int reps, int64_t flush_bytes,
                      double* avg_ms, double* min_ms) {
     if (!op || !x || !y || reps < 1 || flush_bytes < 0) return fail(HEAT_E_ARG, "bad timing request");

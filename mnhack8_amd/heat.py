@@ -99,6 +99,7 @@ SIGNATURES = {
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
+    "heat_time_stream": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
 }
 
 _lib = None
@@ -326,6 +327,13 @@ class HeatOperator:
         _check(self._lib.heat_time_matvec(self.h, self.ptr(x), self.ptr(y), int(reps), int(flush_bytes),
                                           C.byref(a), C.byref(m)))
         return a.value, m.value
+
+    def time_stream(self, x, y, reps):
+        """Milliseconds per launch of a plain stream of the FP64 apply's
+        algorithmic bytes (x, the two coefficient arrays read; y written)."""
+        a = C.c_double()
+        _check(self._lib.heat_time_stream(self.h, self.ptr(x), self.ptr(y), int(reps), C.byref(a)))
+        return a.value
 
     # -- solvers --------------------------------------------------------------
     def pcg_solve(self, b, x, max_iter=500, tol=1e-8, history=False):

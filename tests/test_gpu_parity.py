@@ -820,3 +820,27 @@ def test_fused_pcg_large_mesh_finalized_sums():
     oo, io, xmo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 10, 200, 1e-8, 1e-3)
     assert (outer, inner) == (oo, io)
     assert np.array_equal(op.download("T"), xmo)
+
+
+@pytest.mark.parametrize("n", [3, 8])
+def test_stream_ceiling_kernel(n):
+    """heat_time_stream's kernel writes y = x + (0.5 rho*c*m3 + 0.25 dt*kappa*k2)
+    over the first even number of elements and y = x beyond (the bench's
+    stream of the apply's bytes)."""
+    print(f"[case] stream ceiling n={n}", flush=True)
+    k, rhoc = make_field(n)
+    dt = 1e-3
+    op = heat().HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="colored")
+    x = np.random.default_rng(n).standard_normal(op.nn)
+    op.upload("p", x)
+    ms = op.time_stream("p", "Ap", 3)
+    y = op.download("Ap")
+    h = 1.0 / n
+    cr = rhoc * (h ** 3 / 216.0)
+    ck = (dt * k) * (-h / 12.0)
+    m = 2 * (op.ne // 2)
+    want = x.copy()
+    want[:m] = x[:m] + (0.5 * cr[:m] + 0.25 * ck[:m])
+    assert ms > 0
+    assert relerr(y, want) < 1e-15
+    op.cleanup()
