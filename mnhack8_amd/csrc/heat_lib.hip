@@ -1068,6 +1068,9 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
     return HEAT_OK;
 }
 
+// iterations per polled chunk of the refined solve's inner passes (at most)
+constexpr int MIXED_CHUNK = 4;
+
 int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer, int max_inner,
                          double tol, double tol_inner, int* outer, int* inner_iters) {
     if (!op || !b || !x) return fail(HEAT_E_ARG, "null argument");
@@ -1129,31 +1132,49 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     CHECK_LAUNCH();
     const bool detect = op->cfg.detect != 0;
     int nout = 0;
+    // Inner passes poll in chunks of at most MIXED_CHUNK iterations: every
+    // pass ends with the rest of its last chunk and one more chunk of
+    // iterations that return at once, and a refined solve has several passes.
+    const int saved_chunk = op->pcg_chunk;
+    op->pcg_chunk = std::min(saved_chunk, MIXED_CHUNK);
     const void* key6[6] = {op->e32, (const void*)(intptr_t)((detect ? 9 : 7) + 16 * op->pcg_fi.armed),
                            (const void*)(intptr_t)op->pcg_chunk,
                            nullptr, (const void*)(intptr_t)(100 + op->colored_impl), nullptr};
+    auto inner_iteration = [&]() {
+        const FusedView<float> v = make_view<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32,
+                                                    nullptr, op->d32, op->cr32, op->ck32, nullptr);
+        return launch_fused_iteration<float>(op, v, detect);
+    };
     for (;;) {
         double* Ax = op->vec[HEAT_VEC_AP];
         rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
-        if (rc) return rc;
+        if (rc) break;
         k_fused_mixed_init<<<op->nsb, NT, 0, s>>>(b, Ax, op->d32, op->e32, op->r32, op->z32, g.nn, op->fsp,
                                                    op->nsb, op->fst);
-        CHECK_LAUNCH();
-        rc = run_until_done(op, max_inner, [&]() {
-            return launch_chunk_generic(op, key6, [&]() {
-                const FusedView<float> v = make_view<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32,
-                                                            op->ap32, nullptr, op->d32, op->cr32, op->ck32, nullptr);
-                return launch_fused_iteration<float>(op, v, detect);
-            });
-        }, &op->fst->done);
-        if (rc) return rc;
-        rc = read_fused(op);
-        if (rc) return rc;
+        if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
+        // After a correction the first inner iteration goes alone and is read
+        // back: its first step makes the outer test, so a solve that the
+        // correction finished stops here instead of polling chunks.
+        bool finished = false;
+        if (nout > 0) {
+            rc = inner_iteration();
+            if (rc == HEAT_OK) rc = read_fused(op);
+            if (rc) break;
+            finished = op->hfst->done != 0;
+        }
+        if (!finished) {
+            rc = run_until_done(op, max_inner, [&]() { return launch_chunk_generic(op, key6, inner_iteration); },
+                                &op->fst->done);
+            if (rc == HEAT_OK) rc = read_fused(op);
+            if (rc) break;
+        }
         if (op->hfst->outer_done || nout >= max_outer) break;
         k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
-        CHECK_LAUNCH();
+        if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
         ++nout;
     }
+    op->pcg_chunk = saved_chunk;
+    if (rc) return rc;
     op->pcg_fi_fired = op->pcg_fi.armed ? op->hfst->fi_fired : 0;
     op->pcg_fi.armed = 0;
     if (outer) *outer = nout;
