@@ -91,6 +91,8 @@ struct heat_op {
     DevState* st = nullptr;
     DevState* hst = nullptr;    // pinned host mirror
     int* hflag = nullptr;       // pinned done-flag mailboxes
+    int* hdone = nullptr;       // pinned word that the fused K2 stores the done flag into
+    int* hdone_dev = nullptr;   // its device address
     double* dhist = nullptr;
     int hist_cap = 0;
     double* cur_hist = nullptr;  // history rows written by the running solve, or null
@@ -498,6 +500,7 @@ FusedView<T> make_view(heat_op* op, T* x, T* p0, T* p1, T* r, T* z, T* Ap, T* b,
     v.active = 1;
     v.pre_sp = v.nsb > FUSED_INBLOCK_MAX;
     v.pre_tp = v.ntiles > FUSED_INBLOCK_MAX;
+    v.hflag = op->hdone_dev;
     return v;
 }
 
@@ -579,12 +582,32 @@ int launch_chunk(heat_op* op, double* x, bool detect) {
 }
 
 // Launch chunks until the device reports done; the flags of chunk k are read
-// while chunk k+1 runs.
+// while chunk k+1 runs.  With hdone (the fused solves), the update kernels
+// store the flag into that pinned word themselves and no copy is queued.
 template <typename LaunchFn>
-int run_until_done(heat_op* op, int max_iter, LaunchFn launch, const int* dflag = nullptr) {
+int run_until_done(heat_op* op, int max_iter, LaunchFn launch, const int* dflag = nullptr,
+                   const volatile int* hdone = nullptr) {
     if (!dflag) dflag = &op->st->done;
     const int max_chunks = max_iter / (op->pcg_chunk > 0 ? op->pcg_chunk : 1) + 3;
     int k = 0;
+    if (hdone) {
+        int rc = launch();
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(op->ev[0], op->stream));
+        for (;;) {
+            if (++k > max_chunks) return fail(HEAT_E_STATE, "PCG did not report completion");
+            rc = launch();
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(op->ev[k & 1], op->stream));
+            HIP_TRY(hipEventSynchronize(op->ev[(k - 1) & 1]));
+            // every chunk ends with an update launch, which stored the flag
+            // as of its iteration; a done solve stays done, so a newer value
+            // from the chunk now running is as good
+            if (*hdone) break;
+        }
+        HIP_TRY(hipStreamSynchronize(op->stream));
+        return HEAT_OK;
+    }
     int rc = launch();
     if (rc) return rc;
     int* slot[2] = {&op->hflag[0], &op->hflag[1]};   // pinned mailboxes, one per chunk in flight
@@ -614,6 +637,7 @@ void free_all(heat_op* op) {
     (void)hipFree(op->e32); (void)hipFree(op->r32); (void)hipFree(op->z32); (void)hipFree(op->p32); (void)hipFree(op->p32b); (void)hipFree(op->p2); (void)hipFree(op->ap32); (void)hipFree(op->d32);
     if (op->hst) (void)hipHostFree(op->hst);
     if (op->hflag) (void)hipHostFree(op->hflag);
+    if (op->hdone) (void)hipHostFree(op->hdone);
     if (op->hfst) (void)hipHostFree(op->hfst);
     (void)hipFree(op->fst); (void)hipFree(op->fsp); (void)hipFree(op->ftp);
     if (op->stream) (void)hipStreamDestroy(op->stream);
@@ -667,6 +691,9 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     HIP_TRY(hipHostMalloc(&op->hst, sizeof(DevState), hipHostMallocDefault));
     memset(op->hst, 0, sizeof(DevState));
     HIP_TRY(hipHostMalloc(&op->hflag, 2 * sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&op->hdone, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+    *op->hdone = 0;
+    HIP_TRY(hipHostGetDevicePointer((void**)&op->hdone_dev, op->hdone, 0));
     hipStream_t s = op->stream;
     HIP_TRY(hipMemcpyAsync(op->st, op->hst, sizeof(DevState), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(op->fst, op->hfst, sizeof(FusedState), hipMemcpyHostToDevice, s));
@@ -1010,7 +1037,7 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
             op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], Ap, const_cast<double*>(b),
             op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr));
         CHECK_LAUNCH();
-        rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); }, &op->fst->done);
+        rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); }, &op->fst->done, op->hdone);
         if (rc) return rc;
         rc = read_fused(op);
         if (rc) return rc;
@@ -1164,7 +1191,7 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         }
         if (!finished) {
             rc = run_until_done(op, max_inner, [&]() { return launch_chunk_generic(op, key6, inner_iteration); },
-                                &op->fst->done);
+                                &op->fst->done, op->hdone);
             if (rc == HEAT_OK) rc = read_fused(op);
             if (rc) break;
         }
@@ -1643,6 +1670,7 @@ int batch_views32(heat_batch* b) {
         HIP_TRY(hipStreamSynchronize(op->stream));
         FusedView<float> v = make_view<float>(op, op->e32, op->p32, op->p32b, op->r32, op->z32, op->ap32, nullptr,
                                               op->d32, op->cr32, op->ck32, nullptr);
+        v.hflag = nullptr;                       // a batch polls its own all-done flag
         v.sp = b->views[i].sp;
         v.sp_stride = b->views[i].sp_stride;
         v.tp = b->views[i].tp;
@@ -1696,6 +1724,7 @@ int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
         b->views.push_back(make_view<double>(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_P], op->p2,
                                              op->vec[HEAT_VEC_R], op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP],
                                              op->vec[HEAT_VEC_B], op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr));
+        b->views.back().hflag = nullptr;         // a batch polls its own all-done flag
     }
     const int rc = batch_finish(b);
     if (rc) return bail(rc);
@@ -1735,6 +1764,7 @@ int heat_batch_setup_rhs(heat_op* op, int count, const double* F, heat_batch** o
                                                 v + nn, op->vec[HEAT_VEC_DIAG], op->cr, op->ck, nullptr);
         w.F = v + 7 * nn;
         w.fs = fs;
+        w.hflag = nullptr;                       // a batch polls its own all-done flag
         b->views.push_back(w);
         b->tile_off[i + 1] = b->tile_off[i] + op->fntiles;
         b->sb_off[i + 1] = b->sb_off[i] + op->nsb;

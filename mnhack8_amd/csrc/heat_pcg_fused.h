@@ -106,6 +106,7 @@ struct FusedView {
     double* hist;
     int add_source, active;     // b = M T (+ dt F); a masked solve does no work
     int pre_sp, pre_tp;         // partial sums come finalized in fs->red_sp / red_tp
+    int* hflag;                 // pinned host word: K2 block 0 stores fs->done there (or null)
 };
 
 // Partial counts up to this are reduced by every block that needs them;
@@ -671,6 +672,10 @@ __device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) 
     DevState* st = v.st;
     const double nwA = v.nwA;
     __shared__ double sd[4][NT];
+    // the host polls this word after a chunk of iterations instead of copying
+    // the flag back (a system-scope store to pinned host memory)
+    if (v.hflag && j == 0 && threadIdx.x == 0)
+        __hip_atomic_store(v.hflag, fs->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     if (fs->done) return;
     const int k1 = fs->k1_iter;
     double o[4];
