@@ -52,7 +52,7 @@ extern "C" {
 
 /* PCG execution (both run the reference's algorithm; they differ in the fixed
  * order in which inner products are summed) */
-#define HEAT_PCG_FUSED           0  /* two launches per iteration, tile/super-block order (default) */
+#define HEAT_PCG_FUSED           0  /* three launches per iteration, tile/super-block order (default) */
 #define HEAT_PCG_REFERENCE_ORDER 1  /* six launches per iteration, the reference's 256/1024 order */
 
 /* handle-owned vectors for heat_vector() */

@@ -102,7 +102,7 @@ struct heat_op {
     // level R arming (host copy, passed by value)
     FiArm fi{0, 0, 0};
     int fi_level = 0;
-    // fused PCG (two launches per iteration)
+    // fused PCG (three launches per iteration)
     int pcg_mode = HEAT_PCG_FUSED;
     struct { int armed = 0, vec = 0, bit = 0, iter = 0; long long idx = 0; } pcg_fi;   // one-shot PCG-state SEU
     int pcg_fi_fired = 0;

@@ -1,4 +1,5 @@
-// heat_pcg_fused.h -- Jacobi-PCG in two launches per iteration.
+// heat_pcg_fused.h -- Jacobi-PCG in two kernels per iteration (K1 runs as two
+// launches, k_fused_dir and k_fused_tdma, unless a PCG-state upset is armed).
 //
 // The reference's iteration (src/heat_solver_het.cu:1009-1036) runs matvec,
 // p.Ap, two axpys, ||r||, z = r/d, r.z and p = z + beta p with a host round
@@ -9,7 +10,8 @@
 //   K2 k_fused_update: alpha = rho / p.Ap from K1's partials, x += alpha p,
 //                      r -= alpha Ap, z = r/d, partials of r.r and r.z
 // Every block reduces the other kernel's few hundred partials itself, in the
-// same fixed order, so no separate finalize launch is needed.
+// same fixed order, so no separate finalize launch is needed (beyond
+// FUSED_INBLOCK_MAX partials a one-block launch reduces them once).
 //
 // Reduction order (deterministic, independent of the autotuned apply schedule):
 //   K1 partials: tile (TY x TZ = 8 x 32 node columns, a chunk of consecutive

@@ -224,7 +224,7 @@ class HeatOperator:
         _check(self._lib.heat_set_tile_shape(self.h, int(shape)))
 
     def set_pcg_mode(self, mode):
-        """"fused" (two launches per iteration) or "reference" (the reference's dot order)."""
+        """"fused" (three launches per iteration) or "reference" (the reference's dot order)."""
         _check(self._lib.heat_set_pcg_mode(self.h, {"fused": 0, "reference": 1}[mode]))
 
     def apply_mrhs(self, X, Y, dt_eff):
