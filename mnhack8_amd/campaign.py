@@ -20,11 +20,12 @@ Same command line, grid and outputs as the reference:
 
 How the runs execute differs. The reference starts one solver process per
 run, and each process repeats the two clean runs A and B of the field. Here
-the pending runs of one field go to ONE driver process (heat_solver_het
---campaign), which loads the field once, runs A and B once and then every
-fault run F; each F writes the same files as a single invocation with --fi.
---per-run restores one process per run (any binary with the reference's
-command line then works as --bin).
+the pending runs of a field go to a few driver processes (heat_solver_het
+--campaign), each of which loads the field once, runs A and B once and then
+its share of the fault runs F; each F writes the same files as a single
+invocation with --fi. --jobs of these run at a time on the GPU (small meshes
+leave it mostly idle). --per-run restores one process per run (any binary
+with the reference's command line then works as --bin).
 
 Level H (3) needs two ranks; this solver owns the whole mesh on one GPU, so
 with --mpi >= 2 the level-H rows of the grid are recorded with an error
@@ -39,7 +40,9 @@ import os
 import random
 import subprocess
 import sys
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor, as_completed
 
 if __package__ in (None, ""):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -151,17 +154,19 @@ def solver_args(a, fp, outdir):
     return cmd
 
 
+_LOG_LOCK = threading.Lock()
+
+
 def _log(path, cmd, r, dt):
-    with open(path, "a") as f:
+    with _LOG_LOCK, open(path, "a") as f:
         f.write(f"$ {' '.join(cmd)}\n[{dt:.1f}s, rc={r.returncode}]\n{r.stdout}\n{r.stderr}\n")
 
 
-def run_field(a, pending, log):
-    """All pending runs of one field in one driver process; returns the rows
-    in grid order."""
+def run_field(a, pending, log, part=0):
+    """Pending runs of one field (distinct run ids) in one driver process;
+    returns their rows in the given order. The field files must exist."""
     g0 = pending[0]
-    ensure_field(a, g0)
-    spec = os.path.join(a.out, "runs", f"campaign_{g0['field']}.txt")
+    spec = os.path.join(a.out, "runs", f"campaign_{g0['field']}_{part}.txt")
     rows = {}
     lines = []
     for g in pending:
@@ -196,8 +201,8 @@ def run_field(a, pending, log):
 
 
 def run_single(a, g, log):
-    """One run as one solver process (scripts/ensemble/run_ensemble.py:84-96)."""
-    ensure_field(a, g)
+    """One run as one solver process (scripts/ensemble/run_ensemble.py:84-96).
+    The field files must exist."""
     d = os.path.join(a.out, "runs", run_id(g))
     os.makedirs(d, exist_ok=True)
     cmd = solver_args(a, g["fp"], d) + ["--fi", str(g["level"]), str(g["step"]), str(g["target"]), str(g["bit"])]
@@ -238,6 +243,8 @@ def parse_args(argv=None):
     ap.add_argument("--shard", default="0/1", help="i/n: run only rows with index %% n == i")
     ap.add_argument("--rng", type=int, default=12345)
     ap.add_argument("--per-run", action="store_true", help="one solver process per run, as the reference")
+    ap.add_argument("--jobs", type=int, default=8,
+                    help="solver processes at a time on the GPU (a field's runs are split between them)")
     return ap.parse_args(argv)
 
 
@@ -260,23 +267,36 @@ def main(argv=None):
     pending = [g for i, g in enumerate(grid) if i % shard_n == shard_i and run_id(g) not in done]
     log = os.path.join(a.out, "run.log")
     t0 = time.time()
-    n_run = 0
+    # every distinct run once (a grid can repeat a run id); rows go to the
+    # csv in grid order when all runs are back
+    distinct = list({run_id(g): g for g in pending}.values())
+    for g in {g["field"]: g for g in distinct}.values():
+        ensure_field(a, g)
     if a.per_run:
-        for g in pending:
-            out_csv.append(run_single(a, g, log))
-            n_run += 1
-            if n_run % 10 == 0:
-                el = time.time() - t0
-                print(f"[{n_run} runs, {el / 60:.1f} min, {el / n_run:.1f} s/run]", flush=True)
+        tasks = [[g] for g in distinct]
+        work = lambda part, gs: [run_single(a, gs[0], log)]
     else:
-        for field in dict.fromkeys(g["field"] for g in pending):
-            rows = run_field(a, [g for g in pending if g["field"] == field], log)
-            for row in rows:
-                out_csv.append(row)
-            n_run += len(rows)
+        fields_ = list(dict.fromkeys(g["field"] for g in distinct))
+        per_field = max(1, -(-max(1, a.jobs) // max(1, len(fields_))))
+        tasks = []
+        for field in fields_:
+            gs = [g for g in distinct if g["field"] == field]
+            k = min(per_field, len(gs))
+            tasks += [gs[i * len(gs) // k:(i + 1) * len(gs) // k] for i in range(k)]
+        work = lambda part, gs: run_field(a, gs, log, part)
+    rows, n_done = {}, 0
+    with ThreadPoolExecutor(max_workers=max(1, a.jobs)) as pool:
+        futures = {pool.submit(work, i, gs): gs for i, gs in enumerate(tasks)}
+        for fut in as_completed(futures):
+            gs = futures[fut]
+            for g, row in zip(gs, fut.result()):
+                rows[run_id(g)] = row
+            n_done += len(gs)
             el = time.time() - t0
-            print(f"[{n_run} runs, {el / 60:.1f} min, {el / max(n_run, 1):.2f} s/run] {field}", flush=True)
-    print(f"done: {n_run} new runs -> {out_csv.path}", flush=True)
+            print(f"[{n_done}/{len(distinct)} runs, {el / 60:.1f} min, {el / max(n_done, 1):.2f} s/run]", flush=True)
+    for g in pending:
+        out_csv.append(rows[run_id(g)])
+    print(f"done: {len(pending)} new runs in {time.time() - t0:.1f} s -> {out_csv.path}", flush=True)
     return 0
 
 
