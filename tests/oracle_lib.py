@@ -80,6 +80,9 @@ class OracleOp:
         if threads > 1:
             lib().ho_set_threads(h, threads)
 
+    def set_threads(self, threads):
+        lib().ho_set_threads(self.h, int(threads))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().ho_destroy(self.h)
