@@ -1489,6 +1489,8 @@ struct heat_batch {
     hipEvent_t ev[2] = {nullptr, nullptr};
     int* hflag = nullptr;                    // pinned mailboxes, one per chunk in flight
     int* dflag = nullptr;                    // device: all solves done
+    int* hdone = nullptr;                    // pinned host word: all solves done after the last chunk
+    int* hdone_dev = nullptr;                // its device address
     int* diters = nullptr;
     int* hiters = nullptr;                   // pinned
     std::vector<FusedView<double>> views;
@@ -1532,6 +1534,7 @@ void batch_free(heat_batch* b) {
     if (b->hflag) (void)hipHostFree(b->hflag);
     if (b->hiters) (void)hipHostFree(b->hiters);
     (void)hipFree(b->dflag); (void)hipFree(b->diters); (void)hipFree(b->dviews);
+    if (b->hdone) (void)hipHostFree(b->hdone);
     (void)hipFree(b->d_tile_off); (void)hipFree(b->d_sb_off); (void)hipFree(b->d_node_off);
     (void)hipFree(b->sp); (void)hipFree(b->tp);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1549,6 +1552,9 @@ int batch_finish(heat_batch* b) {
     HIP_TRY(hipHostMalloc(&b->hflag, 2 * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(&b->hiters, count * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipMalloc(&b->dflag, sizeof(int)));
+    HIP_TRY(hipHostMalloc(&b->hdone, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+    *b->hdone = 0;
+    HIP_TRY(hipHostGetDevicePointer((void**)&b->hdone_dev, b->hdone, 0));
     HIP_TRY(hipMalloc(&b->diters, count * sizeof(int)));
     HIP_TRY(hipMalloc(&b->dviews, count * sizeof(FusedView<double>)));
     HIP_TRY(hipMalloc(&b->d_tile_off, (count + 1) * sizeof(int)));
@@ -1610,7 +1616,7 @@ int batch_chunk_t(heat_batch* b, const std::vector<FusedView<T>>& views, const F
             k_batch_update<T, false><<<total_sb, NT, 0, s>>>(dv, b->d_sb_off, count);
             CHECK_LAUNCH();
         }
-        k_batch_all_done<T><<<1, NT, 0, s>>>(dv, count, b->dflag);
+        k_batch_all_done<T><<<1, NT, 0, s>>>(dv, count, b->hdone_dev);
         CHECK_LAUNCH();
         return HEAT_OK;
     };
@@ -1637,23 +1643,24 @@ int batch_chunk(heat_batch* b) {
 }
 
 // chunks until every solve of the batch reports done; the flag of chunk k is
-// read while chunk k+1 runs
+// read while chunk k+1 runs.  Each chunk ends with k_batch_all_done storing
+// the flag into a pinned host word; a done batch stays done, so a newer value
+// from the chunk now running is as good.
 template <typename Chunk>
 int batch_until_done(heat_batch* b, int max_iter, Chunk chunk) {
     hipStream_t s = b->stream;
+    const volatile int* hdone = b->hdone;
     const int max_chunks = max_iter / b->pcg_chunk + 3;
     int rc = chunk();
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(&b->hflag[0], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipEventRecord(b->ev[0], s));
     for (int kk = 1;; ++kk) {
         if (kk > max_chunks) return fail(HEAT_E_STATE, "batch PCG did not report completion");
         rc = chunk();
         if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(&b->hflag[kk & 1], b->dflag, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipEventRecord(b->ev[kk & 1], s));
         HIP_TRY(hipEventSynchronize(b->ev[(kk - 1) & 1]));
-        if (b->hflag[(kk - 1) & 1]) break;
+        if (*hdone) break;
     }
     return HEAT_OK;
 }

@@ -1093,7 +1093,8 @@ __global__ void k_batch_params(const FusedView<double>* __restrict__ views, int 
     fs->done = views[i].active ? 0 : 1;
 }
 
-// *flag = 1 when every solve of the batch is done
+// *flag = 1 when every solve of the batch is done (flag: a pinned host word,
+// stored at system scope, which the host polls)
 template <typename T>
 __global__ void k_batch_all_done(const FusedView<T>* __restrict__ views, int count, int* flag) {
     __shared__ int open;
@@ -1102,7 +1103,7 @@ __global__ void k_batch_all_done(const FusedView<T>* __restrict__ views, int cou
     for (int i = threadIdx.x; i < count; i += blockDim.x)
         if (!views[i].fs->done) atomicOr(&open, 1);
     __syncthreads();
-    if (threadIdx.x == 0) *flag = open ? 0 : 1;
+    if (threadIdx.x == 0) __hip_atomic_store(flag, open ? 0 : 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace heat
