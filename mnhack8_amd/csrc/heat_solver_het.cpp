@@ -38,6 +38,13 @@ struct ApiError {
         }                                                                                  \
     } while (0)
 
+// one line of a --campaign file
+struct CampaignRun {
+    std::string dir;
+    int level, step, bit;
+    long target;
+};
+
 struct Opts {
     int N = 0;
     double t_final = 0, dt = 0;
@@ -53,7 +60,42 @@ struct Opts {
     double inner_tol = 1e-4;
     int colored_impl = -1, pcg_chunk = 8, use_graph = 1, pcg_order = HEAT_PCG_FUSED;
     std::string campaign;
+    std::vector<CampaignRun> campaign_runs;
 };
+
+// Lines "DIR LEVEL STEP TARGET BIT"; DIR is everything before the last four
+// fields, so it may hold spaces. False (with a message) on a bad file.
+bool read_campaign(const std::string& path, std::vector<CampaignRun>& runs) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) {
+        fprintf(stderr, "cannot read %s\n", path.c_str());
+        return false;
+    }
+    char line[4096];
+    bool bad_line = false;
+    while (fgets(line, sizeof line, f)) {
+        std::string t(line);
+        while (!t.empty() && (t.back() == '\n' || t.back() == '\r' || t.back() == ' ')) t.pop_back();
+        if (t.empty()) continue;
+        size_t cut = t.size();
+        for (int k = 0; k < 4 && cut != std::string::npos; ++k) {
+            cut = t.find_last_of(' ', cut - 1);
+            if (cut == 0) cut = std::string::npos;
+        }
+        CampaignRun r;
+        char tail;
+        if (cut == std::string::npos ||
+            sscanf(t.c_str() + cut, "%d %d %ld %d %c", &r.level, &r.step, &r.target, &r.bit, &tail) != 4) {
+            bad_line = true;
+            break;
+        }
+        r.dir = t.substr(0, cut);
+        runs.push_back(r);
+    }
+    fclose(f);
+    if (bad_line) fprintf(stderr, "%s: every line must read DIR LEVEL STEP TARGET BIT\n", path.c_str());
+    return !bad_line;
+}
 
 void usage(const char* prog) {
     fprintf(stderr,
@@ -481,11 +523,12 @@ int parse(int argc, char** argv, Opts& o) {
                         "and --diag-cg\n");
         return 1;
     }
+    if (!o.campaign.empty() && !read_campaign(o.campaign, o.campaign_runs)) return 1;
     return 0;
 }
 
-// --campaign FILE: the fault runs of one field in one process. Each line of
-// FILE is "DIR LEVEL STEP TARGET BIT". The clean runs A and B are the same
+// --campaign FILE: the fault runs of one field in one process (the lines of
+// FILE, read by read_campaign). The clean runs A and B are the same
 // for every fault of the field (the solve is deterministic), so they run once;
 // each fault run F then writes the files of a single invocation with
 // --fi LEVEL STEP TARGET BIT into DIR (which must exist), except the T_final
@@ -495,39 +538,18 @@ int parse(int argc, char** argv, Opts& o) {
 // no files written, and the return status is 1.
 int run_campaign(Solver& S, int n_steps, const CaseLabels& lab, const std::vector<std::vector<double>>& snapA,
                  const RunResult& rA, const RunResult& rB) {
-    FILE* f = fopen(S.o.campaign.c_str(), "r");
-    if (!f) {
-        fprintf(stderr, "cannot read %s\n", S.o.campaign.c_str());
-        return 1;
-    }
-    struct Run {
-        std::string dir;
-        int level, step, bit;
-        long target;
-    };
-    std::vector<Run> runs;
-    char dir[1024];
-    int level, step, bit;
-    long target;
-    int fields;
-    while ((fields = fscanf(f, "%1023s %d %d %ld %d", dir, &level, &step, &target, &bit)) == 5)
-        runs.push_back({dir, level, step, bit, target});
-    const bool bad_line = fields != EOF;
-    fclose(f);
-    if (bad_line) {
-        fprintf(stderr, "%s: every line must read DIR LEVEL STEP TARGET BIT\n", S.o.campaign.c_str());
-        return 1;
-    }
+    const std::vector<CampaignRun>& runs = S.o.campaign_runs;
     int status = 0;
     const double t0 = now_ms();
     for (size_t i = 0; i < runs.size(); ++i) {
-        const Run& r = runs[i];
+        const CampaignRun& r = runs[i];
         if ((r.level != 1 && r.level != 2) || r.bit < 0 || r.bit > 63) {
             fprintf(stderr, "[campaign] %s: level %d bit %d cannot run on one rank\n", r.dir.c_str(), r.level, r.bit);
             status = 1;
             continue;
         }
         Opts oc = S.o;
+        oc.campaign_runs.clear();
         oc.fi_level = r.level;
         oc.fi_step = r.step;
         oc.fi_target = r.target;
@@ -551,6 +573,9 @@ int run(const Opts& o) {
            (long)(o.N + 1) * (o.N + 1) * (o.N + 1), nprocs, o.N / nprocs);
     printf("dt %.4e  t_final %.4e  steps %d  kernel %s  scatter %s  mode %s\n", o.dt, o.t_final, n_steps,
            kname, sname, o.verify ? "VERIFY" : "production");
+    if (!o.campaign.empty())
+        printf("Campaign: %zu fault runs from %s (first in %s)\n", o.campaign_runs.size(), o.campaign.c_str(),
+               o.campaign_runs.empty() ? "-" : o.campaign_runs[0].dir.c_str());
     std::vector<double> hk, hr;
     if (!load_fields(o, hk, hr)) return 1;
 

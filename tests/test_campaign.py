@@ -9,8 +9,9 @@ copied by tests/golden/collect_fixtures.py.
 * The statistics: the three agg tables recomputed from the recorded
   ensemble.csv, character for character (line ends aside).
 * The runner, on the CPU: one process per run through the oracle's command
-  line (the reference's protocol), resumption, shards and error rows. The
-  GPU tests compare the one-process-per-field campaign with this protocol.
+  line (the reference's protocol), resumption, shards and error rows; the
+  driver's --campaign file format. The GPU tests compare the campaign's
+  driver processes (--campaign) with this protocol.
 """
 import csv
 import os
@@ -140,3 +141,11 @@ def test_driver_campaign_flag_checks(tmp_path):
     assert r.returncode == 1 and "--campaign takes its faults from FILE" in r.stderr
     r = subprocess.run([build.DRIVER], capture_output=True, text=True)
     assert "--campaign FILE" in r.stderr
+    # a run directory may contain spaces: DIR is all before the last four fields
+    spec = tmp_path / "spec.txt"
+    spec.write_text(f"{tmp_path}/run one 1 4 123 55\n{tmp_path}/two 2 20 77 63\n\n")
+    r = subprocess.run(base[:6] + ["--campaign", str(spec)], capture_output=True, text=True)
+    assert f"Campaign: 2 fault runs from {spec} (first in {tmp_path}/run one)" in r.stdout
+    spec.write_text(f"{tmp_path}/x 1 4 123\n")
+    r = subprocess.run(base[:6] + ["--campaign", str(spec)], capture_output=True, text=True)
+    assert r.returncode == 1 and "every line must read DIR LEVEL STEP TARGET BIT" in r.stderr
