@@ -116,15 +116,28 @@ struct FusedView {
 constexpr int FUSED_INBLOCK_MAX = 2048;
 
 // 256-wide tree over one value per thread; result valid in thread 0.
+// The pairs and their order are those of the plain LDS tree (at stride st,
+// element t += element t + st, for st = 128, 64, ..., 1); the strides below
+// 64 run inside wave 0 with lane shuffles instead of LDS and barriers, which
+// adds the same values in the same order.
+__device__ __forceinline__ double wave0_tail(double v) {
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) v += __shfl_down(v, st, 64);
+    return v;
+}
+
 __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
+    static_assert(NT == 256, "two LDS strides, then one wave");
     const int t = threadIdx.x;
     sd[k][t] = v;
     __syncthreads();
-#pragma unroll
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if (t < st) sd[k][t] += sd[k][t + st];
-        __syncthreads();
+    if (t < 128) sd[k][t] += sd[k][t + 128];
+    __syncthreads();
+    if (t < 64) {
+        const double w = wave0_tail(sd[k][t] + sd[k][t + 64]);
+        if (t == 0) sd[k][0] = w;
     }
+    __syncthreads();
     return sd[k][0];
 }
 
@@ -140,14 +153,19 @@ __device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* 
         sd[k][t] = a;
     }
     __syncthreads();
+    if (t < 128) {
 #pragma unroll
-    for (int st = NT / 2; st > 0; st >>= 1) {
-        if (t < st) {
-#pragma unroll
-            for (int k = 0; k < K; ++k) sd[k][t] += sd[k][t + st];
-        }
-        __syncthreads();
+        for (int k = 0; k < K; ++k) sd[k][t] += sd[k][t + 128];
     }
+    __syncthreads();
+    if (t < 64) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double w = wave0_tail(sd[k][t] + sd[k][t + 64]);
+            if (t == 0) sd[k][0] = w;
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = sd[k][0];
     __syncthreads();
