@@ -205,11 +205,14 @@ int heat_time_mrhs(heat_op* op, const float* X, float* Y, int reps, double* avg_
 /* Batched ensembles (new; replaces the reference's one process per ensemble
  * member, scripts/ensemble/run_ensemble.py:53-104, each running
  * src/heat_solver_het.cu's main).  COUNT independent solves, each with its own
- * configuration and fields (fast kernel, colour-ordered scatter, no
- * detectors), advanced together: every PCG iteration of all solves is one
- * pair of launches over all their tiles, and a converged or masked solve's
- * blocks return at once.  Each solve's iterates are bit-identical to the
- * same solve run alone in the fused order. */
+ * configuration and fields (fast kernel, colour-ordered scatter; detectors
+ * on all solves or none), advanced together: every PCG iteration of all
+ * solves is one set of launches over all their tiles, and a converged or
+ * masked solve's blocks return at once.  Each solve's iterates are
+ * bit-identical to the same solve run alone in the fused order.  With
+ * detectors, each solve forms b and its first residual alone, as
+ * heat_build_rhs and heat_pcg_solve do (D1 records, an armed level-R fault),
+ * and its D1 record after a step equals the single solve's. */
 typedef struct heat_batch heat_batch;
 int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
                      const double* const* rhoc, heat_batch** out);

@@ -23,9 +23,11 @@ run, and each process repeats the two clean runs A and B of the field. Here
 the pending runs of a field go to a few driver processes (heat_solver_het
 --campaign), each of which loads the field once, runs A and B once and then
 its share of the fault runs F; each F writes the same files as a single
-invocation with --fi. --jobs of these run at a time on the GPU (small meshes
-leave it mostly idle). --per-run restores one process per run (any binary
-with the reference's command line then works as --bin).
+invocation with --fi. A process advances up to --batch of its fault runs
+together as one batch of solves (heat_batch_step: every PCG iteration of all
+of them in one set of launches), and --jobs processes run at a time on the
+GPU (small meshes leave it mostly idle). --per-run restores one process per
+run (any binary with the reference's command line then works as --bin).
 
 Level H (3) needs two ranks; this solver owns the whole mesh on one GPU, so
 with --mpi >= 2 the level-H rows of the grid are recorded with an error
@@ -184,6 +186,8 @@ def run_field(a, pending, log, part=0):
         with open(spec, "w") as f:
             f.writelines(lines)
         cmd = solver_args(a, g0["fp"], a.out) + ["--campaign", spec]
+        if a.batch > 0:
+            cmd += ["--campaign-batch", str(a.batch)]
         t0 = time.time()
         r = subprocess.run(cmd, capture_output=True, text=True)
         _log(log, cmd, r, time.time() - t0)
@@ -245,6 +249,8 @@ def parse_args(argv=None):
     ap.add_argument("--per-run", action="store_true", help="one solver process per run, as the reference")
     ap.add_argument("--jobs", type=int, default=8,
                     help="solver processes at a time on the GPU (a field's runs are split between them)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="fault runs a driver process advances together as one batch (0: one after another)")
     return ap.parse_args(argv)
 
 

@@ -1501,6 +1501,7 @@ struct heat_batch {
     int *d_tile_off = nullptr, *d_sb_off = nullptr, *d_node_off = nullptr;
     double *sp = nullptr, *tp = nullptr;
     int pcg_chunk = 8, use_graph = 1;
+    int detect = 0;                          // members built with detectors (all or none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_chunk = 0;
@@ -1599,22 +1600,36 @@ int batch_chunk_t(heat_batch* b, const std::vector<FusedView<T>>& views, const F
     const unsigned grid1 = (unsigned)(((total_tiles + NXCD - 1) / NXCD) * NXCD);
     bool pre_sp = false, pre_tp = false;
     for (const auto& v : views) { pre_sp = pre_sp || v.pre_sp; pre_tp = pre_tp || v.pre_tp; }
+    const bool detect = b->detect && sizeof(T) == sizeof(double);   // refined batches run without
     auto body = [&]() -> int {
         for (int i = 0; i < b->pcg_chunk; ++i) {
             if (pre_sp) {
                 k_batch_fin_sp<T><<<count, NT, 0, s>>>(dv);
                 CHECK_LAUNCH();
             }
-            k_batch_dir<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
-            CHECK_LAUNCH();
-            k_batch_tdma<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
-            CHECK_LAUNCH();
-            if (pre_tp) {
-                k_batch_fin_tp<T><<<count, NT, 0, s>>>(dv);
+            if (detect) {
+                k_batch_dir<T, true><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
+                CHECK_LAUNCH();
+                k_batch_tdma<T, true><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
+                CHECK_LAUNCH();
+                if (pre_tp) {
+                    k_batch_fin_tp<T, true><<<count, NT, 0, s>>>(dv);
+                    CHECK_LAUNCH();
+                }
+                k_batch_update<T, true><<<total_sb, NT, 0, s>>>(dv, b->d_sb_off, count);
+                CHECK_LAUNCH();
+            } else {
+                k_batch_dir<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
+                CHECK_LAUNCH();
+                k_batch_tdma<T, false><<<grid1, FTY * FTZ, 0, s>>>(dv, b->d_tile_off, count);
+                CHECK_LAUNCH();
+                if (pre_tp) {
+                    k_batch_fin_tp<T, false><<<count, NT, 0, s>>>(dv);
+                    CHECK_LAUNCH();
+                }
+                k_batch_update<T, false><<<total_sb, NT, 0, s>>>(dv, b->d_sb_off, count);
                 CHECK_LAUNCH();
             }
-            k_batch_update<T, false><<<total_sb, NT, 0, s>>>(dv, b->d_sb_off, count);
-            CHECK_LAUNCH();
         }
         k_batch_all_done<T><<<1, NT, 0, s>>>(dv, count, b->hdone_dev);
         CHECK_LAUNCH();
@@ -1704,7 +1719,8 @@ int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
         const heat_config& c = cfgs[i];
         if (c.kernel != HEAT_KERNEL_FAST || c.scatter != HEAT_SCATTER_COLORED)
             return bail(fail(HEAT_E_ARG, "batch solve %d: the batch runs the colour-ordered fast path only", i));
-        if (c.detect) return bail(fail(HEAT_E_ARG, "batch solve %d: the batch runs without detectors", i));
+        if ((c.detect != 0) != (cfgs[0].detect != 0))
+            return bail(fail(HEAT_E_ARG, "batch solve %d: all solves of a batch have detectors, or none", i));
         heat_op* op = nullptr;
         const int rc = heat_setup(&c, k[i], rhoc[i], &op);
         if (rc) {
@@ -1715,6 +1731,7 @@ int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
         b->ops.push_back(op);
         HIP_TRY_OR(hipStreamSynchronize(op->stream), return bail(HEAT_E_HIP));
     }
+    b->detect = cfgs[0].detect != 0;
     b->tile_off.assign(count + 1, 0);
     b->sb_off.assign(count + 1, 0);
     b->node_off.assign(count + 1, 0);
@@ -1837,12 +1854,36 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
     const int total_sb = b->sb_off[count];
     const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
     for (int step = 0; step < nsteps; ++step) {
-        k_batch_rhs<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->d_node_off, count);
-        CHECK_LAUNCH();
+        if (!b->detect) {
+            k_batch_rhs<<<nblk_nodes, NT, 0, s>>>(b->dviews, b->d_node_off, count);
+            CHECK_LAUNCH();
+        }
         k_batch_params<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, tol, max_iter);
         CHECK_LAUNCH();
-        k_batch_residual<<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
-        CHECK_LAUNCH();
+        if (b->detect) {
+            // with detectors each active solve forms b and r = b - A x0 as
+            // heat_build_rhs and heat_pcg_solve do: the projected applies with
+            // their D1 records (and an armed level-R fault firing in the second,
+            // :999), then the fused start
+            HIP_TRY(hipStreamSynchronize(s));
+            for (int i = 0; i < count; ++i) {
+                if (!b->views[i].active) continue;
+                heat_op* op = b->ops[i];
+                double* Ap = op->vec[HEAT_VEC_AP];
+                int rc = heat_build_rhs(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_B]);
+                if (rc) return rc;
+                rc = apply_impl(op, op->vec[HEAT_VEC_T], Ap, op->cfg.dt, true, true, EPI_D1, 0);
+                if (rc) return rc;
+                // the batch's view: its super-block partials live in the batch's rows
+                k_fused_init<double><<<op->nsb, NT, 0, op->stream>>>(b->views[i]);
+                CHECK_LAUNCH();
+            }
+            for (int i = 0; i < count; ++i)
+                if (b->views[i].active) HIP_TRY(hipStreamSynchronize(b->ops[i]->stream));
+        } else {
+            k_batch_residual<<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
+            CHECK_LAUNCH();
+        }
         const int rc = batch_until_done(b, max_iter, [&]() { return batch_chunk(b); });
         if (rc) return rc;
         k_batch_iters<<<(count + NT - 1) / NT, NT, 0, s>>>(b->dviews, count, b->diters);
@@ -1947,6 +1988,7 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
 int heat_batch_step_mixed(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
                           int* inner_iters) {
     if (!b || nsteps < 0 || max_outer < 0 || max_inner < 0) return fail(HEAT_E_ARG, "bad batch step arguments");
+    if (b->detect) return fail(HEAT_E_STATE, "refined batch steps run without detectors");
     int rc = batch_views32(b);
     if (rc) return rc;
     const int count = (int)b->views.size();

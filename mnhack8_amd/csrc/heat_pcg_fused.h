@@ -991,10 +991,10 @@ __global__ __launch_bounds__(NT) void k_batch_fin_sp(const FusedView<T>* __restr
     if (v.pre_sp) fused_fin_sp_body<T>(v);
 }
 
-template <typename T>
+template <typename T, bool DETECT>
 __global__ __launch_bounds__(NT) void k_batch_fin_tp(const FusedView<T>* __restrict__ views) {
     const FusedView<T>& v = views[blockIdx.x];
-    if (v.pre_tp) fused_fin_tp_body<T, false>(v);
+    if (v.pre_tp) fused_fin_tp_body<T, DETECT>(v);
 }
 
 // ---- FP32-refined solves of a batch (each solve as heat_pcg_solve_mixed) ----

@@ -12,7 +12,8 @@
 // --precision fp64|fp32 (FP32-refined solve), --inner-tol X,
 // --colored-impl gather|sweep|march|tile|tdma, --pcg-chunk K, --no-graph,
 // --pcg-order fused|reference (summation order of the PCG inner products),
-// --campaign FILE (many fault runs of one field in one process, below).
+// --campaign FILE (many fault runs of one field in one process, below),
+// --campaign-batch B (B of those runs advanced together as one batch).
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -61,6 +62,7 @@ struct Opts {
     int colored_impl = -1, pcg_chunk = 8, use_graph = 1, pcg_order = HEAT_PCG_FUSED;
     std::string campaign;
     std::vector<CampaignRun> campaign_runs;
+    int campaign_batch = 0;
 };
 
 // Lines "DIR LEVEL STEP TARGET BIT"; DIR is everything before the last four
@@ -106,7 +108,7 @@ void usage(const char* prog) {
             "   [--diag-cg FILE]   (one CG solve, logs iteration,rel_residual to FILE, then exits)\n"
             "   [--tol X] [--max-iter N] [--source-const Q] [--precision fp64|fp32] [--inner-tol X]\n"
             "   [--colored-impl gather|sweep|march|tile|tdma] [--pcg-chunk K] [--no-graph]\n"
-            "   [--pcg-order fused|reference] [--campaign FILE]\n"
+            "   [--pcg-order fused|reference] [--campaign FILE] [--campaign-batch B]\n"
             "  LEVEL: 1=G global memory, 2=R accumulator, 3=H halo staging buffer\n"
             "  BIT:   0-51 mantissa, 52-62 exponent, 63 sign\n",
             prog);
@@ -491,6 +493,7 @@ int parse(int argc, char** argv, Opts& o) {
         } else if (!strcmp(a, "--pcg-chunk") && has(1)) { o.pcg_chunk = atoi(argv[++i]);
         } else if (!strcmp(a, "--no-graph")) { o.use_graph = 0;
         } else if (!strcmp(a, "--campaign") && has(1)) { o.campaign = argv[++i];
+        } else if (!strcmp(a, "--campaign-batch") && has(1)) { o.campaign_batch = atoi(argv[++i]);
         } else if (!strcmp(a, "--qrank") || !strcmp(a, "--qblock")) {
             fprintf(stderr, "%s: the hybrid quantum-rank mode is not part of this solver\n", a);
             return 1;
@@ -560,6 +563,159 @@ int run_campaign(Solver& S, int n_steps, const CaseLabels& lab, const std::vecto
         printf("[campaign] %zu/%zu %s  %.1f s\n", i + 1, runs.size(), r.dir.c_str(), (now_ms() - t0) / 1e3);
         fflush(stdout);
     }
+    return status;
+}
+
+// One member of a batched campaign: a batch handle and its vectors.
+struct Member {
+    heat_op* op;
+    double *T, *b, *r, *Ap;
+};
+
+double member_norm2(heat_op* op, const double* v) {
+    double s = 0;
+    API(heat_norm2(op, v, &s));
+    return s;
+}
+
+// run_case for the fault runs RUNS[0..m) on the first m members of BATCH:
+// the same per-step sequence (the flip or the arming, b = M T + dt F and the
+// PCG solve, the detectors, the snapshots against the clean run A), with
+// the solves of all members advanced by one heat_batch_step.  A member's
+// solve_ms is the batch step's wall time divided among the members.
+void run_batch_cases(Solver& S, heat_batch* batch, std::vector<Member>& mem, int m, const CampaignRun* const* runs,
+                     int n_steps, const std::vector<std::vector<double>>& snapA, const RunResult& rA,
+                     std::vector<RunResult>& res) {
+    const Opts& o = S.o;
+    const int B = (int)mem.size();
+    std::vector<int> active(B, 0);
+    for (int i = 0; i < m; ++i) active[i] = 1;
+    API(heat_batch_set_active(batch, active.data()));
+    res.assign(m, RunResult());
+    for (int i = 0; i < m; ++i) {
+        API(heat_initial_condition(mem[i].op, mem[i].T, o.ic_mode, 0.0, S.stats[0] / S.stats[2]));
+        API(heat_disarm_fault(mem[i].op));
+    }
+    std::vector<int> iters(B, 0);
+    std::vector<double> tmp;
+    size_t si = 0;
+    for (int step = 0; step < n_steps; ++step) {
+        for (int i = 0; i < m; ++i) {
+            const CampaignRun& r = *runs[i];
+            if (step != r.step) continue;
+            if (r.level == 1) {
+                if (o.fi_rank == 0) API(heat_flip_bit(mem[i].op, mem[i].T, r.target % S.nn, r.bit));
+            } else {
+                API(heat_arm_fault(mem[i].op, r.level, r.target, r.bit));
+            }
+        }
+        for (int i = 0; i < m; ++i) API(heat_synchronize(mem[i].op));
+        const double t0 = now_ms();
+        API(heat_batch_step(batch, 1, o.max_iter, o.tol, iters.data()));
+        for (int i = 0; i < m && o.reproject; ++i) API(heat_set_dirichlet(mem[i].op, mem[i].T, 0.0));
+        for (int i = 0; i < m; ++i) API(heat_synchronize(mem[i].op));
+        const double share = (now_ms() - t0) / m;
+        const bool snap = (o.snap_every > 0 && step % o.snap_every == 0) || step == n_steps - 1;
+        for (int i = 0; i < m; ++i) {
+            RunResult& rr = res[i];
+            heat_op* op = mem[i].op;
+            rr.solve_ms += share;
+            API(heat_disarm_fault(op));
+            rr.cg_iters.push_back(iters[i]);
+            if (o.detect) {
+                double d1 = 0;
+                API(heat_abft_take(op, &d1, nullptr));
+                API(heat_apply_operator(op, mem[i].T, mem[i].Ap, o.dt, 0, 1));
+                API(heat_copy(op, mem[i].b, mem[i].r));
+                API(heat_axpy(op, -1.0, mem[i].Ap, mem[i].r));
+                double bn = member_norm2(op, mem[i].b);
+                if (bn < 1e-300) bn = 1.0;
+                const double d2 = member_norm2(op, mem[i].r) / bn;
+                double tmin = 0, tmax = 0;
+                API(heat_minmax(op, mem[i].T, &tmin, &tmax));
+                bool d3_fire = !std::isfinite(tmin) || !std::isfinite(tmax);
+                if (step < (int)rA.d3_tmax.size()) {
+                    const double ref_max = rA.d3_tmax[step], ref_min = rA.d3_tmin[step];
+                    const double margin = 1e-6 * std::fmax(std::fabs(ref_max), 1e-300) + 1e-12;
+                    if (tmax > ref_max + margin || tmin < ref_min - margin) d3_fire = true;
+                }
+                rr.d1_abft.push_back(d1);
+                rr.d2_resid.push_back(d2);
+                rr.d3_tmin.push_back(tmin);
+                rr.d3_tmax.push_back(tmax);
+                if (!(d1 <= S.chk_tol) && rr.d1_first < 0) rr.d1_first = step;
+                if (!(d2 <= 1e-6) && rr.d2_first < 0) rr.d2_first = step;
+                if (d3_fire && rr.d3_first < 0) rr.d3_first = step;
+                rr.d1_max = std::isfinite(d1) ? std::fmax(rr.d1_max, d1) : 1e300;
+                rr.d2_max = std::isfinite(d2) ? std::fmax(rr.d2_max, d2) : 1e300;
+                API(heat_abft_take(op, nullptr, nullptr));
+            }
+            if (snap) {
+                tmp.resize(S.nn);
+                API(heat_download(op, tmp.data(), mem[i].T, S.nn));
+                double sp = 0;
+                const double E = global_rel_err(tmp, snapA[si], &sp);
+                if (!std::isfinite(E)) rr.nonfinite = 1;
+                rr.E.push_back(E);
+                rr.spread.push_back(sp);
+                rr.snap_steps.push_back(step);
+            }
+        }
+        if (snap) ++si;
+    }
+}
+
+// --campaign with --campaign-batch B: the fault runs in groups of B members
+// of one batch of handles (same field and configuration as the clean runs).
+int run_campaign_batched(Solver& S, const heat_config& cfg, const std::vector<double>& hk,
+                         const std::vector<double>& hr, int n_steps, const CaseLabels& lab,
+                         const std::vector<std::vector<double>>& snapA, const RunResult& rA, const RunResult& rB) {
+    const std::vector<CampaignRun>& runs = S.o.campaign_runs;
+    int status = 0;
+    std::vector<const CampaignRun*> todo;
+    for (const CampaignRun& r : runs) {
+        if ((r.level != 1 && r.level != 2) || r.bit < 0 || r.bit > 63) {
+            fprintf(stderr, "[campaign] %s: level %d bit %d cannot run on one rank\n", r.dir.c_str(), r.level, r.bit);
+            status = 1;
+            continue;
+        }
+        todo.push_back(&r);
+    }
+    const int B = std::min<int>(S.o.campaign_batch, (int)todo.size());
+    if (B <= 0) return status;
+    std::vector<heat_config> cfgs(B, cfg);
+    std::vector<const double*> kp(B, hk.data()), rp(B, hr.data());
+    heat_batch* batch = nullptr;
+    API(heat_batch_setup(B, cfgs.data(), kp.data(), rp.data(), &batch));
+    API(heat_batch_set_tuning(batch, S.o.pcg_chunk, S.o.use_graph));
+    std::vector<Member> mem(B);
+    for (int i = 0; i < B; ++i) {
+        API(heat_batch_instance(batch, i, &mem[i].op));
+        API(heat_vector(mem[i].op, HEAT_VEC_T, &mem[i].T));
+        API(heat_vector(mem[i].op, HEAT_VEC_B, &mem[i].b));
+        API(heat_vector(mem[i].op, HEAT_VEC_R, &mem[i].r));
+        API(heat_vector(mem[i].op, HEAT_VEC_AP, &mem[i].Ap));
+    }
+    const double t0 = now_ms();
+    for (size_t first = 0; first < todo.size(); first += B) {
+        const int m = (int)std::min<size_t>(B, todo.size() - first);
+        std::vector<RunResult> res;
+        run_batch_cases(S, batch, mem, m, todo.data() + first, n_steps, snapA, rA, res);
+        for (int i = 0; i < m; ++i) {
+            const CampaignRun& r = *todo[first + i];
+            Opts oc = S.o;
+            oc.campaign_runs.clear();
+            oc.fi_level = r.level;
+            oc.fi_step = r.step;
+            oc.fi_target = r.target;
+            oc.fi_bit = r.bit;
+            if (!write_case(oc, r.dir, lab, rA, rB, res[i], target_on_boundary(oc))) status = 1;
+        }
+        printf("[campaign] %zu/%zu runs (batches of %d)  %.1f s\n", first + m, todo.size(), B,
+               (now_ms() - t0) / 1e3);
+        fflush(stdout);
+    }
+    API(heat_batch_cleanup(batch));
     return status;
 }
 
@@ -682,7 +838,12 @@ int run(const Opts& o) {
         run_case(S, n_steps, 0, -1, 0, 0, nullptr, &snapA, rB, "cleanB", &rA);
         int on_bnd = -1;
         if (!o.campaign.empty()) {
-            status = run_campaign(S, n_steps, lab, snapA, rA, rB);
+            // the batch runs the fused FP64 colour-ordered solve; other solves
+            // take the runs one after another
+            const bool batched = o.campaign_batch > 0 && o.colored && o.kernel == HEAT_KERNEL_FAST && !o.mixed &&
+                                 o.pcg_order == HEAT_PCG_FUSED;
+            status = batched ? run_campaign_batched(S, cfg, hk, hr, n_steps, lab, snapA, rA, rB)
+                             : run_campaign(S, n_steps, lab, snapA, rA, rB);
         } else if (o.fi_level > 0 && !status) {
             on_bnd = target_on_boundary(o);
             run_case(S, n_steps, o.fi_level, o.fi_step, o.fi_target, o.fi_bit, nullptr, &snapA, rF, "fault", &rA);

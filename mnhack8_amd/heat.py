@@ -356,8 +356,9 @@ class HeatBatch:
     """An ensemble of independent solves advanced together (heat_batch_*).
 
     members: list of dicts with keys n, dt, k, rhoc and optionally source,
-    source_value, no_source, verify.  ops[i] is a HeatOperator view of solve
-    i (upload T^0, download T)."""
+    source_value, no_source, verify, detect (all members or none).  ops[i] is
+    a HeatOperator view of solve i (upload T^0, download T, arm faults, read
+    its D1 record)."""
 
     def __init__(self, members):
         lib = load_library()
@@ -377,7 +378,7 @@ class HeatBatch:
             kp[i], rp[i] = k.ctypes.data, r.ctypes.data
             cfgs[i] = HeatConfig(n, float(m["dt"]), KERNEL["fast"], SCATTER["colored"], int(m.get("verify", 0)),
                                  int(m.get("no_source", 0)), SOURCE[m.get("source", "gaussian")],
-                                 float(m.get("source_value", 0.0)), 0)
+                                 float(m.get("source_value", 0.0)), int(bool(m.get("detect", False))))
         h = C.c_void_p()
         _check(lib.heat_batch_setup(count, cfgs, kp, rp, C.byref(h)))
         self.h = h

@@ -125,3 +125,26 @@ def test_dump_snaps_files(tmp_path):
         assert (e > 0) == (s >= 4)
     assert np.array_equal(snap("cleanA", 19), np.fromfile(str(tmp_path / "T_final_sm_clean_rank0.bin")))
     assert np.array_equal(snap("fault", 19), np.fromfile(str(tmp_path / "T_final_sm_fault_rank0.bin")))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch", [5, 64])
+def test_batched_campaign_equals_sequential(tmp_path, batch):
+    """Fault runs advanced together as one batch of handles (--campaign-batch)
+    write the rows and traces of the same runs taken one after another."""
+    common = ["--N", "12", "--t", "0.01", "--dt", "5e-4", "--sigmas", "1.0,2.0", "--seeds", "3", "--reps", "1",
+              "--targets", "2", "--detect", "--mpi", "1", "--jobs", "2"]
+    seq, bat = str(tmp_path / "seq"), str(tmp_path / "bat")
+    print(f"[case] campaign N=12 sequential, then batches of {batch}", flush=True)
+    assert campaign.main(["--out", seq] + common) == 0
+    assert campaign.main(["--out", bat, "--batch", str(batch)] + common) == 0
+    a, b = _rows(seq), _rows(bat)
+    assert set(a) == set(b) and len(a) == 2 * 2 * 4 * 3 * 2
+    for rid, ra in a.items():
+        rb = b[rid]
+        assert ra["status"] == rb["status"] == "ok"
+        diff = {k: (ra[k], rb[k]) for k in ra if k not in TIMING and ra[k] != rb[k]}
+        assert not diff, (rid, diff)
+        ta = open(os.path.join(seq, "runs", rid, "fi_trace_e.csv")).read()
+        tb = open(os.path.join(bat, "runs", rid, "fi_trace_e.csv")).read()
+        assert ta == tb, rid

@@ -844,3 +844,58 @@ def test_stream_ceiling_kernel(n):
     assert ms > 0
     assert relerr(y, want) < 1e-15
     op.cleanup()
+
+
+def test_batch_with_detectors_matches_single_solves():
+    """A batch of handles built with detectors: each member's step (its first
+    residual with the D1 record and any armed level-R fault, then the batched
+    iterations with D1 on every apply) gives the same iterations, T bits and
+    D1 record (largest ratio, number of checks) as that solve run alone, also
+    with a level-R fault armed in one member and a level-G flip in another."""
+    print("[case] batch with detectors vs single solves", flush=True)
+    h = heat()
+    sizes, seeds = [8, 11, 14, 8], [3, 4, 5, 6]
+    dt, max_iter, tol = 1e-3, 300, 1e-10
+    members, singles, T0s = [], [], []
+    for n, sd in zip(sizes, seeds):
+        k, rhoc = make_field(n, seed=sd)
+        members.append({"n": n, "dt": dt, "k": k, "rhoc": rhoc, "source": "const", "source_value": 1.0,
+                        "detect": True})
+        singles.append(h.HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="colored", source="const",
+                                      source_value=1.0, detect=True))
+        T0s.append(np.random.default_rng(sd).standard_normal((n + 1) ** 3) * 0.1)
+    batch = h.HeatBatch(members)
+    for op, s_op, T0 in zip(batch.ops, singles, T0s):
+        op.upload("T", T0)
+        s_op.upload("T", T0)
+        op.abft_take()
+        s_op.abft_take()
+    e1 = validations_elem(6, 5, 4, sizes[1])
+    g2 = (4 * (sizes[2] + 1) + 7) * (sizes[2] + 1) + 6
+    for step in range(3):
+        if step == 0:
+            batch.ops[1].arm_fault(2, e1, 55)
+            singles[1].arm_fault(2, e1, 55)
+        if step == 1:
+            batch.ops[2].flip_bit("T", g2, 54)
+            singles[2].flip_bit("T", g2, 54)
+        its = batch.step(1, max_iter, tol)[0]
+        for i, (op, s_op) in enumerate(zip(batch.ops, singles)):
+            s_op.build_rhs("T", "b")
+            it = s_op.pcg_solve("b", "T", max_iter, tol)
+            op.disarm_fault()
+            s_op.disarm_fault()
+            d1b, cb = op.abft_take()
+            d1s, cs = s_op.abft_take()
+            print(f"  step {step} member {i}: iterations {its[i]} / {it}, D1 {d1b:.3e} / {d1s:.3e}, "
+                  f"checks {cb} / {cs}", flush=True)
+            assert its[i] == it and cb == cs
+            assert d1b == d1s or (np.isnan(d1b) and np.isnan(d1s))
+            assert np.array_equal(op.download("T"), s_op.download("T"))
+    batch.cleanup()
+    for s_op in singles:
+        s_op.cleanup()
+
+
+def validations_elem(ix, iy, iz, n):
+    return (ix * n + iy) * n + iz
