@@ -234,6 +234,16 @@ int heat_batch_vector(heat_batch* b, int i, int which, double** ptr);
  * Jacobi-PCG from x0 = T (src/heat_solver_het.cu:1308-1311).  ITERS
  * (optional) receives NSTEPS x COUNT iteration counts (-1: masked). */
 int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* iters);
+/* The detectors of every active member of a batch with detectors after a
+ * step (new, no counterpart in the reference, which reads them for its one
+ * solve, src/heat_solver_het.cu:1202-1234): D1[i] = the member's largest D1
+ * ratio since the last reset (then reset), D2[i] = ||b - P A T|| / ||b|| from a
+ * fresh projected apply (||b|| below 1e-300 taken as 1), TMIN[i] and TMAX[i]
+ * = the extremes of T; the D1 record that the D2 apply makes is reset.  The
+ * same values as heat_abft_take, heat_apply_operator, heat_copy, heat_axpy,
+ * heat_norm2 and heat_minmax called in that order on the member, with all
+ * members' work queued before one wait.  Masked members get NaN. */
+int heat_batch_detect(heat_batch* b, double* d1, double* d2, double* tmin, double* tmax);
 /* NSTEPS backward-Euler steps of every active solve of a batch of handles
  * (heat_batch_setup), each solve as heat_pcg_solve_mixed (FP32 inner
  * Jacobi-PCG in fused order inside FP64 defect correction to TOL, inner

@@ -1896,6 +1896,58 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
     return HEAT_OK;
 }
 
+__global__ void k_abft_save(DevState* st, int slot) { st->res[slot] = st->abft_max; }
+
+int heat_batch_detect(heat_batch* b, double* d1, double* d2, double* tmin, double* tmax) {
+    if (!b || !d1 || !d2 || !tmin || !tmax) return fail(HEAT_E_ARG, "null argument");
+    if (!b->detect) return fail(HEAT_E_STATE, "the batch's solves have no detectors");
+    const int count = (int)b->views.size();
+    // every member's sequence on its own stream, then one wait for all:
+    // the D1 step maximum and its reset (heat_abft_take), the D2 apply and
+    // residual (heat_apply_operator, heat_copy, heat_axpy, two heat_norm2),
+    // the extremes of T (heat_minmax), the reset of the D1 record the D2 apply
+    // made; the results land in res[0], res[1], res[2], res[4], res[5]
+    for (int i = 0; i < count; ++i) {
+        if (!b->views[i].active) continue;
+        heat_op* op = b->ops[i];
+        hipStream_t s = op->stream;
+        const int nn = op->g.nn;
+        double *T = op->vec[HEAT_VEC_T], *B = op->vec[HEAT_VEC_B], *R = op->vec[HEAT_VEC_R];
+        double* Ap = op->vec[HEAT_VEC_AP];
+        k_abft_save<<<1, 1, 0, s>>>(op->st, 0);
+        k_abft_reset<<<1, 1, 0, s>>>(op->st);
+        int rc = heat_apply_operator(op, T, Ap, op->cfg.dt, 0, 1);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(R, B, sizeof(double) * (size_t)nn, hipMemcpyDeviceToDevice, s));
+        k_axpy<<<op->nbn, NT, 0, s>>>(-1.0, Ap, R, nn);
+        k_dot<<<op->nbn, NT, 0, s>>>(B, B, nn, op->partials, op->st, 1);
+        k_fin_dot<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 1);
+        k_dot<<<op->nbn, NT, 0, s>>>(R, R, nn, op->partials, op->st, 2);
+        k_fin_dot<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 2);
+        k_minmax<<<op->nbn, NT, 0, s>>>(T, nn, op->partials, op->st, 4);
+        k_fin_minmax<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 4);
+        k_abft_reset<<<1, 1, 0, s>>>(op->st);
+        CHECK_LAUNCH();
+        HIP_TRY(hipMemcpyAsync(op->hst, op->st, sizeof(DevState), hipMemcpyDeviceToHost, s));
+    }
+    for (int i = 0; i < count; ++i) {
+        if (!b->views[i].active) {
+            d1[i] = d2[i] = tmin[i] = tmax[i] = std::nan("");
+            continue;
+        }
+        heat_op* op = b->ops[i];
+        HIP_TRY(hipStreamSynchronize(op->stream));
+        const DevState* h = op->hst;
+        d1[i] = h->res[0];
+        double bn = std::sqrt(h->res[1]);                    // heat_norm2(b), clamped as the driver does
+        if (bn < 1e-300) bn = 1.0;
+        d2[i] = std::sqrt(h->res[2]) / bn;
+        tmin[i] = h->res[4];
+        tmax[i] = h->res[5];
+    }
+    return HEAT_OK;
+}
+
 int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner, double tol, double tol_inner,
                          int* inner_iters, double* d1max) {
     if (!b || nsteps < 0 || max_outer < 0 || max_inner < 0) return fail(HEAT_E_ARG, "bad batch step arguments");

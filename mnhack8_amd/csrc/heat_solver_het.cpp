@@ -572,12 +572,6 @@ struct Member {
     double *T, *b, *r, *Ap;
 };
 
-double member_norm2(heat_op* op, const double* v) {
-    double s = 0;
-    API(heat_norm2(op, v, &s));
-    return s;
-}
-
 // run_case for the fault runs RUNS[0..m) on the first m members of BATCH:
 // the same per-step sequence (the flip or the arming, b = M T + dt F and the
 // PCG solve, the detectors, the snapshots against the clean run A), with
@@ -597,6 +591,7 @@ void run_batch_cases(Solver& S, heat_batch* batch, std::vector<Member>& mem, int
         API(heat_disarm_fault(mem[i].op));
     }
     std::vector<int> iters(B, 0);
+    std::vector<double> det_d1(B), det_d2(B), det_tmin(B), det_tmax(B);
     std::vector<double> tmp;
     size_t si = 0;
     for (int step = 0; step < n_steps; ++step) {
@@ -616,6 +611,9 @@ void run_batch_cases(Solver& S, heat_batch* batch, std::vector<Member>& mem, int
         for (int i = 0; i < m; ++i) API(heat_synchronize(mem[i].op));
         const double share = (now_ms() - t0) / m;
         const bool snap = (o.snap_every > 0 && step % o.snap_every == 0) || step == n_steps - 1;
+        // the detectors of all members at once: the values of run_case's
+        // sequence (heat_batch_detect)
+        if (o.detect) API(heat_batch_detect(batch, det_d1.data(), det_d2.data(), det_tmin.data(), det_tmax.data()));
         for (int i = 0; i < m; ++i) {
             RunResult& rr = res[i];
             heat_op* op = mem[i].op;
@@ -623,16 +621,7 @@ void run_batch_cases(Solver& S, heat_batch* batch, std::vector<Member>& mem, int
             API(heat_disarm_fault(op));
             rr.cg_iters.push_back(iters[i]);
             if (o.detect) {
-                double d1 = 0;
-                API(heat_abft_take(op, &d1, nullptr));
-                API(heat_apply_operator(op, mem[i].T, mem[i].Ap, o.dt, 0, 1));
-                API(heat_copy(op, mem[i].b, mem[i].r));
-                API(heat_axpy(op, -1.0, mem[i].Ap, mem[i].r));
-                double bn = member_norm2(op, mem[i].b);
-                if (bn < 1e-300) bn = 1.0;
-                const double d2 = member_norm2(op, mem[i].r) / bn;
-                double tmin = 0, tmax = 0;
-                API(heat_minmax(op, mem[i].T, &tmin, &tmax));
+                const double d1 = det_d1[i], d2 = det_d2[i], tmin = det_tmin[i], tmax = det_tmax[i];
                 bool d3_fire = !std::isfinite(tmin) || !std::isfinite(tmax);
                 if (step < (int)rA.d3_tmax.size()) {
                     const double ref_max = rA.d3_tmax[step], ref_min = rA.d3_tmin[step];
@@ -648,7 +637,6 @@ void run_batch_cases(Solver& S, heat_batch* batch, std::vector<Member>& mem, int
                 if (d3_fire && rr.d3_first < 0) rr.d3_first = step;
                 rr.d1_max = std::isfinite(d1) ? std::fmax(rr.d1_max, d1) : 1e300;
                 rr.d2_max = std::isfinite(d2) ? std::fmax(rr.d2_max, d2) : 1e300;
-                API(heat_abft_take(op, nullptr, nullptr));
             }
             if (snap) {
                 tmp.resize(S.nn);

@@ -93,6 +93,7 @@ SIGNATURES = {
     "heat_batch_set_active": (_I, [_P, _P]),
     "heat_batch_set_tuning": (_I, [_P, _I, _I]),
     "heat_batch_step": (_I, [_P, _I, _I, _D, _P]),
+    "heat_batch_detect": (_I, [_P, _P, _P, _P, _P]),
     "heat_batch_step_mixed": (_I, [_P, _I, _I, _I, _D, _D, _P]),
     "heat_batch_step_mrhs": (_I, [_P, _I, _I, _I, _D, _D, _P, _P]),
     "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
@@ -425,6 +426,13 @@ class HeatBatch:
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_batch_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))
+
+    def detect(self):
+        """Per-member detectors after a step of a batch with detectors:
+        (d1, d2, tmin, tmax) arrays (NaN for masked members)."""
+        out = [np.empty(self.count) for _ in range(4)]
+        _check(self._lib.heat_batch_detect(self.h, *[o.ctypes.data for o in out]))
+        return tuple(out)
 
     def step(self, nsteps=1, max_iter=500, tol=1e-8):
         """nsteps backward-Euler steps of every active solve; returns the

@@ -851,7 +851,8 @@ def test_batch_with_detectors_matches_single_solves():
     residual with the D1 record and any armed level-R fault, then the batched
     iterations with D1 on every apply) gives the same iterations, T bits and
     D1 record (largest ratio, number of checks) as that solve run alone, also
-    with a level-R fault armed in one member and a level-G flip in another."""
+    with a level-R fault armed in one member and a level-G flip in another;
+    and heat_batch_detect gives the driver's D1, D2 and D3 values."""
     print("[case] batch with detectors vs single solves", flush=True)
     h = heat()
     sizes, seeds = [8, 11, 14, 8], [3, 4, 5, 6]
@@ -880,18 +881,35 @@ def test_batch_with_detectors_matches_single_solves():
             batch.ops[2].flip_bit("T", g2, 54)
             singles[2].flip_bit("T", g2, 54)
         its = batch.step(1, max_iter, tol)[0]
+        # step 1: the detectors of all members at once (heat_batch_detect)
+        det = batch.detect() if step == 1 else None
         for i, (op, s_op) in enumerate(zip(batch.ops, singles)):
             s_op.build_rhs("T", "b")
             it = s_op.pcg_solve("b", "T", max_iter, tol)
             op.disarm_fault()
             s_op.disarm_fault()
-            d1b, cb = op.abft_take()
-            d1s, cs = s_op.abft_take()
-            print(f"  step {step} member {i}: iterations {its[i]} / {it}, D1 {d1b:.3e} / {d1s:.3e}, "
-                  f"checks {cb} / {cs}", flush=True)
-            assert its[i] == it and cb == cs
-            assert d1b == d1s or (np.isnan(d1b) and np.isnan(d1s))
+            assert its[i] == it
             assert np.array_equal(op.download("T"), s_op.download("T"))
+            if det is None:
+                d1b, cb = op.abft_take()
+                d1s, cs = s_op.abft_take()
+                print(f"  step {step} member {i}: iterations {its[i]} / {it}, D1 {d1b:.3e} / {d1s:.3e}, "
+                      f"checks {cb} / {cs}", flush=True)
+                assert cb == cs
+                assert d1b == d1s or (np.isnan(d1b) and np.isnan(d1s))
+            else:
+                # the driver's sequence on the single solve
+                d1s, _ = s_op.abft_take()
+                s_op.apply_operator("T", "Ap", dt, allow_fi=False, project=True)
+                s_op.copy("b", "r")
+                s_op.axpy(-1.0, "Ap", "r")
+                bn = s_op.norm2("b")
+                d2s = s_op.norm2("r") / (bn if bn >= 1e-300 else 1.0)
+                tmin, tmax = s_op.minmax("T")
+                s_op.abft_take()
+                print(f"  step {step} member {i}: D1 {det[0][i]:.3e} / {d1s:.3e}, D2 {det[1][i]:.3e} / {d2s:.3e}",
+                      flush=True)
+                assert (det[0][i], det[1][i], det[2][i], det[3][i]) == (d1s, d2s, tmin, tmax)
     batch.cleanup()
     for s_op in singles:
         s_op.cleanup()
