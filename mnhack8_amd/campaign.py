@@ -247,9 +247,9 @@ def parse_args(argv=None):
     ap.add_argument("--shard", default="0/1", help="i/n: run only rows with index %% n == i")
     ap.add_argument("--rng", type=int, default=12345)
     ap.add_argument("--per-run", action="store_true", help="one solver process per run, as the reference")
-    ap.add_argument("--jobs", type=int, default=8,
+    ap.add_argument("--jobs", type=int, default=4,
                     help="solver processes at a time on the GPU (a field's runs are split between them)")
-    ap.add_argument("--batch", type=int, default=64,
+    ap.add_argument("--batch", type=int, default=128,
                     help="fault runs a driver process advances together as one batch (0: one after another)")
     return ap.parse_args(argv)
 
