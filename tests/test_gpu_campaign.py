@@ -1,13 +1,14 @@
-"""The one-process-per-field fault campaign on the GPU against the reference's
-one-process-per-run protocol (the same driver started once per run with --fi)
-and against the reference's recorded N=32 campaign; and the driver's
-per-snapshot field dumps.
+"""The fault campaign on the GPU (driver processes with --campaign, fault runs
+batched by default) against the reference's one-process-per-run protocol (the
+same driver started once per run with --fi), against the same runs taken one
+after another, and against the reference's recorded N=32 campaign; and the
+driver's per-snapshot field dumps.
 
 Each fault run of `heat_solver_het --campaign` must write the fi_summary row
 and fi_trace file that a single invocation with --fi writes, character for
 character, apart from the wall-clock columns (solve_ms_A/B/F): the clean runs
-A and B, run once per field, are the same runs that every single invocation
-repeats.
+A and B, run once per process, are the same runs that every single invocation
+repeats, and a batched member's step is the single solve's step.
 """
 import csv
 import os
