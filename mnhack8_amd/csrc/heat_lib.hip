@@ -1510,7 +1510,7 @@ struct heat_batch {
     int graph_chunk32 = 0;
     heat_op* mesh = nullptr;                 // shared-mesh batch: the borrowed handle
     // 16-column refined solve of a shared-mesh batch (node-major FP32 columns)
-    float *mrE = nullptr, *mrR = nullptr, *mrZ = nullptr, *mrP = nullptr, *mrAP = nullptr;
+    float *mrE = nullptr, *mrR = nullptr, *mrP = nullptr, *mrAP = nullptr;
     double* mrpart = nullptr;
     double* mrtile = nullptr;                // the apply's tile partials (4 x 16 x tiles)
     MrState* mrst = nullptr;
@@ -1528,7 +1528,7 @@ void batch_free(heat_batch* b) {
     if (b->gexec32) (void)hipGraphExecDestroy(b->gexec32);
     if (b->graph32) (void)hipGraphDestroy(b->graph32);
     (void)hipFree(b->dviews32);
-    (void)hipFree(b->mrE); (void)hipFree(b->mrR); (void)hipFree(b->mrZ); (void)hipFree(b->mrP); (void)hipFree(b->mrAP);
+    (void)hipFree(b->mrE); (void)hipFree(b->mrR); (void)hipFree(b->mrP); (void)hipFree(b->mrAP);
     (void)hipFree(b->mrpart); (void)hipFree(b->mrtile); (void)hipFree(b->mrst);
     if (b->hmrst) (void)hipHostFree(b->hmrst);
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
@@ -1966,7 +1966,6 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
         const size_t bytes = sizeof(float) * (size_t)g.nn * MRC;
         HIP_TRY(hipMalloc(&b->mrE, bytes));
         HIP_TRY(hipMalloc(&b->mrR, bytes));
-        HIP_TRY(hipMalloc(&b->mrZ, bytes));
         HIP_TRY(hipMalloc(&b->mrP, bytes));
         HIP_TRY(hipMalloc(&b->mrAP, bytes));
         HIP_TRY(hipMalloc(&b->mrpart, sizeof(double) * MRK * MRC * MRB));
@@ -1991,7 +1990,7 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
         k_mr_params<<<1, 1, 0, s>>>(b->mrst, tol, tol_inner, max_inner, detect ? m->nwA : 0.0);
         CHECK_LAUNCH();
         for (int nout = 0;; ++nout) {
-            k_mr_init<<<MRB, NT, 0, s>>>(b->dviews, m->cr, m->ck, m->d32, g, b->mrE, b->mrR, b->mrZ, b->mrP,
+            k_mr_init<<<MRB, NT, 0, s>>>(b->dviews, m->cr, m->ck, m->d32, g, b->mrE, b->mrR, b->mrP,
                                          b->mrpart, b->mrst);
             CHECK_LAUNCH();
             k_mr_fin_init<<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
@@ -2008,12 +2007,12 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
                     if (detect) k_mr_fin_pap<true><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
                     else k_mr_fin_pap<false><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
                     CHECK_LAUNCH();
-                    k_mr_update<<<MRB, NT, 0, s>>>(b->mrE, b->mrP, b->mrR, b->mrAP, b->mrZ, m->d32, g, b->mrpart,
+                    k_mr_update<<<MRB, NT, 0, s>>>(b->mrE, b->mrP, b->mrR, b->mrAP, m->d32, g, b->mrpart,
                                                    b->mrst);
                     CHECK_LAUNCH();
                     k_mr_fin_update<<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
                     CHECK_LAUNCH();
-                    k_mr_dir<<<MRB, NT, 0, s>>>(b->mrZ, b->mrP, g, b->mrst);
+                    k_mr_dir<<<MRB, NT, 0, s>>>(b->mrR, m->d32, b->mrP, g, b->mrst);
                     CHECK_LAUNCH();
                 }
                 k_mr_flags<<<1, 1, 0, s>>>(b->mrst);

@@ -12,8 +12,9 @@
 //     residual meets the outer tolerance (or is not finite);
 //   inner iteration: Ap = P A p (matrix cores; p.Ap [and the D1 sums] per
 //     tile in the same kernel); alpha;
-//     e += alpha p, r -= alpha Ap, z = r / d; r.r, r.z; convergence, beta;
-//     p = z + beta p;
+//     e += alpha p, r -= alpha Ap; r.r, r.z with z = r / d; convergence, beta;
+//     p = r / d + beta p (z is formed again there from r and d, not stored:
+//     one node vector of 16 columns less to write and read per iteration);
 //   then x += e for every column still open.
 // Vectors are node-major, right-hand-side-minor (V[node * 16 + c]), the
 // layout of the multi-RHS apply; x and b stay the batch members' FP64 vectors.
@@ -86,10 +87,11 @@ __device__ __forceinline__ void mr_column_sums(double (*sd)[NT], const double* _
 }
 
 // Outer pass start: FP64 residual of every open column at every node, the
-// FP32 inner start vectors and the sums [r32.r32, r32.z32, b.b, r.r].
+// FP32 inner start vectors (e = 0, r32, p = z32 = r32 / d) and the sums
+// [r32.r32, r32.z32, b.b, r.r].
 __global__ __launch_bounds__(NT) void k_mr_init(const FusedView<double>* __restrict__ views, const double* __restrict__ cr,
                                                 const double* __restrict__ ck, const float* __restrict__ d32, Grid g,
-                                                float* __restrict__ E, float* __restrict__ R, float* __restrict__ Z,
+                                                float* __restrict__ E, float* __restrict__ R,
                                                 float* __restrict__ P, double* __restrict__ part,
                                                 const MrState* __restrict__ st) {
     __shared__ double sd[MRK][NT];
@@ -110,7 +112,6 @@ __global__ __launch_bounds__(NT) void k_mr_init(const FusedView<double>* __restr
             const size_t i = (size_t)n * MRC + c;
             E[i] = 0.0f;
             R[i] = rf;
-            Z[i] = zf;
             P[i] = zf;
             a[0] += (double)rf * (double)rf;
             a[1] += (double)rf * (double)zf;
@@ -172,10 +173,10 @@ __global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ pa
     }
 }
 
-// e += alpha p, r -= alpha Ap, z = r / d of every open column; sums r.r, r.z
+// e += alpha p, r -= alpha Ap of every open column; sums r.r, r.z (z = r / d)
 __global__ __launch_bounds__(NT) void k_mr_update(float* __restrict__ E, const float* __restrict__ P,
                                                   float* __restrict__ R, const float* __restrict__ AP,
-                                                  float* __restrict__ Z, const float* __restrict__ d32, Grid g,
+                                                  const float* __restrict__ d32, Grid g,
                                                   double* __restrict__ part, const MrState* __restrict__ st) {
     __shared__ double sd[2][NT];
     const int c = threadIdx.x & (MRC - 1);
@@ -188,7 +189,6 @@ __global__ __launch_bounds__(NT) void k_mr_update(float* __restrict__ E, const f
             const float ri = (float)fma(-alpha, (double)AP[i], (double)R[i]);
             const float zi = ri / d32[n];
             R[i] = ri;
-            Z[i] = zi;
             a[0] += (double)ri * (double)ri;
             a[1] += (double)ri * (double)zi;
         }
@@ -215,15 +215,17 @@ __global__ __launch_bounds__(NT) void k_mr_fin_update(const double* __restrict__
     }
 }
 
-// p = z + beta p of every open column; then the all-done flag
-__global__ __launch_bounds__(NT) void k_mr_dir(const float* __restrict__ Z, float* __restrict__ P, Grid g,
-                                               const MrState* __restrict__ st) {
+// p = z + beta p of every open column, z = r / d formed as k_mr_update forms
+// it (the same float division of the same values)
+__global__ __launch_bounds__(NT) void k_mr_dir(const float* __restrict__ R, const float* __restrict__ d32,
+                                               float* __restrict__ P, Grid g, const MrState* __restrict__ st) {
     const int c = threadIdx.x & (MRC - 1);
     if (st->done[c]) return;
     const double beta = st->beta[c];
     for (int n = blockIdx.x * (NT / MRC) + (threadIdx.x >> 4); n < g.nn; n += MRB * (NT / MRC)) {
         const size_t i = (size_t)n * MRC + c;
-        P[i] = (float)fma(beta, (double)P[i], (double)Z[i]);
+        const float zi = R[i] / d32[n];
+        P[i] = (float)fma(beta, (double)P[i], (double)zi);
     }
 }
 
