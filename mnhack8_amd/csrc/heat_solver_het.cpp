@@ -566,10 +566,10 @@ int run_campaign(Solver& S, int n_steps, const CaseLabels& lab, const std::vecto
     return status;
 }
 
-// One member of a batched campaign: a batch handle and its vectors.
+// One member of a batched campaign: a batch handle and its temperature.
 struct Member {
     heat_op* op;
-    double *T, *b, *r, *Ap;
+    double* T;
 };
 
 // run_case for the fault runs RUNS[0..m) on the first m members of BATCH:
@@ -680,9 +680,6 @@ int run_campaign_batched(Solver& S, const heat_config& cfg, const std::vector<do
     for (int i = 0; i < B; ++i) {
         API(heat_batch_instance(batch, i, &mem[i].op));
         API(heat_vector(mem[i].op, HEAT_VEC_T, &mem[i].T));
-        API(heat_vector(mem[i].op, HEAT_VEC_B, &mem[i].b));
-        API(heat_vector(mem[i].op, HEAT_VEC_R, &mem[i].r));
-        API(heat_vector(mem[i].op, HEAT_VEC_AP, &mem[i].Ap));
     }
     const double t0 = now_ms();
     for (size_t first = 0; first < todo.size(); first += B) {
