@@ -2007,12 +2007,12 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
                     if (detect) k_mr_fin_pap<true><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
                     else k_mr_fin_pap<false><<<MRC, NT, 0, s>>>(b->mrtile, ntiles_mr, b->mrst);
                     CHECK_LAUNCH();
-                    k_mr_update<<<MRB, NT, 0, s>>>(b->mrE, b->mrP, b->mrR, b->mrAP, m->d32, g, b->mrpart,
+                    k_mr_update<<<MRB, NT, 0, s>>>(b->mrR, b->mrAP, m->d32, g, b->mrpart,
                                                    b->mrst);
                     CHECK_LAUNCH();
                     k_mr_fin_update<<<MRC, NT, 0, s>>>(b->mrpart, b->mrst);
                     CHECK_LAUNCH();
-                    k_mr_dir<<<MRB, NT, 0, s>>>(b->mrR, m->d32, b->mrP, g, b->mrst);
+                    k_mr_dir<<<MRB, NT, 0, s>>>(b->mrE, b->mrR, m->d32, b->mrP, g, b->mrst);
                     CHECK_LAUNCH();
                 }
                 k_mr_flags<<<1, 1, 0, s>>>(b->mrst);

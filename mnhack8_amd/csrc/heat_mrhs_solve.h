@@ -12,9 +12,10 @@
 //     residual meets the outer tolerance (or is not finite);
 //   inner iteration: Ap = P A p (matrix cores; p.Ap [and the D1 sums] per
 //     tile in the same kernel); alpha;
-//     e += alpha p, r -= alpha Ap; r.r, r.z with z = r / d; convergence, beta;
-//     p = r / d + beta p (z is formed again there from r and d, not stored:
-//     one node vector of 16 columns less to write and read per iteration);
+//     r -= alpha Ap; r.r, r.z with z = r / d; convergence, beta;
+//     e += alpha p and p = r / d + beta p in one pass (z is formed again there
+//     from r and d, not stored, and p is read once for both: two node vectors
+//     of 16 columns less to move per iteration);
 //   then x += e for every column still open.
 // Vectors are node-major, right-hand-side-minor (V[node * 16 + c]), the
 // layout of the multi-RHS apply; x and b stay the batch members' FP64 vectors.
@@ -34,6 +35,7 @@ struct MrState {
     double rz[MRC], alpha[MRC], beta[MRC], bnorm[MRC];
     double bnorm_outer[MRC], rnorm_outer[MRC], d1max[MRC];
     int done[MRC], outer_done[MRC], iter[MRC], inner_total[MRC];
+    int stepped[MRC];                // 1: the column took this iteration's step (e += alpha p still due)
     double tol, tol_outer, nwA;
     int max_inner, all_done, all_outer_done, pad;
 };
@@ -157,6 +159,7 @@ __global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ pa
     double o[MRK] = {0.0, 0.0, 0.0, 0.0};
     if (DETECT) mr_column_sums<MRK>(sd, part, c, o, ntiles);
     else mr_column_sums<1>(sd, part, c, o, ntiles);
+    if (threadIdx.x == 0) st->stepped[c] = 0;
     if (threadIdx.x == 0 && !st->done[c]) {
         if (DETECT) {
             // D1 of this apply, recorded before the breakdown check (as abft_record)
@@ -169,13 +172,13 @@ __global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ pa
             st->alpha[c] = 0.0;
         } else {
             st->alpha[c] = st->rz[c] / o[0];
+            st->stepped[c] = 1;
         }
     }
 }
 
-// e += alpha p, r -= alpha Ap of every open column; sums r.r, r.z (z = r / d)
-__global__ __launch_bounds__(NT) void k_mr_update(float* __restrict__ E, const float* __restrict__ P,
-                                                  float* __restrict__ R, const float* __restrict__ AP,
+// r -= alpha Ap of every open column; sums r.r, r.z (z = r / d)
+__global__ __launch_bounds__(NT) void k_mr_update(float* __restrict__ R, const float* __restrict__ AP,
                                                   const float* __restrict__ d32, Grid g,
                                                   double* __restrict__ part, const MrState* __restrict__ st) {
     __shared__ double sd[2][NT];
@@ -185,7 +188,6 @@ __global__ __launch_bounds__(NT) void k_mr_update(float* __restrict__ E, const f
         const double alpha = st->alpha[c];
         for (int n = blockIdx.x * (NT / MRC) + (threadIdx.x >> 4); n < g.nn; n += MRB * (NT / MRC)) {
             const size_t i = (size_t)n * MRC + c;
-            E[i] = (float)fma(alpha, (double)P[i], (double)E[i]);
             const float ri = (float)fma(-alpha, (double)AP[i], (double)R[i]);
             const float zi = ri / d32[n];
             R[i] = ri;
@@ -215,17 +217,24 @@ __global__ __launch_bounds__(NT) void k_mr_fin_update(const double* __restrict__
     }
 }
 
-// p = z + beta p of every open column, z = r / d formed as k_mr_update forms
-// it (the same float division of the same values)
-__global__ __launch_bounds__(NT) void k_mr_dir(const float* __restrict__ R, const float* __restrict__ d32,
-                                               float* __restrict__ P, Grid g, const MrState* __restrict__ st) {
+// e += alpha p of every column that took this iteration's step, then
+// p = z + beta p of every column still open (z = r / d formed as k_mr_update
+// forms it: the same float division of the same values)
+__global__ __launch_bounds__(NT) void k_mr_dir(float* __restrict__ E, const float* __restrict__ R,
+                                               const float* __restrict__ d32, float* __restrict__ P, Grid g,
+                                               const MrState* __restrict__ st) {
     const int c = threadIdx.x & (MRC - 1);
-    if (st->done[c]) return;
-    const double beta = st->beta[c];
+    if (!st->stepped[c]) return;
+    const bool open = !st->done[c];
+    const double alpha = st->alpha[c], beta = st->beta[c];
     for (int n = blockIdx.x * (NT / MRC) + (threadIdx.x >> 4); n < g.nn; n += MRB * (NT / MRC)) {
         const size_t i = (size_t)n * MRC + c;
-        const float zi = R[i] / d32[n];
-        P[i] = (float)fma(beta, (double)P[i], (double)zi);
+        const float pi = P[i];
+        E[i] = (float)fma(alpha, (double)pi, (double)E[i]);
+        if (open) {
+            const float zi = R[i] / d32[n];
+            P[i] = (float)fma(beta, (double)pi, (double)zi);
+        }
     }
 }
 
@@ -260,6 +269,7 @@ __global__ void k_mr_params(MrState* st, double tol, double tol_inner, int max_i
     for (int c = 0; c < MRC; ++c) {
         st->outer_done[c] = 0;
         st->done[c] = 0;
+        st->stepped[c] = 0;
         st->inner_total[c] = 0;
         st->iter[c] = 0;
         st->d1max[c] = 0.0;
