@@ -149,3 +149,11 @@ def test_driver_campaign_flag_checks(tmp_path):
     spec.write_text(f"{tmp_path}/x 1 4 123\n")
     r = subprocess.run(base[:6] + ["--campaign", str(spec)], capture_output=True, text=True)
     assert r.returncode == 1 and "every line must read DIR LEVEL STEP TARGET BIT" in r.stderr
+
+
+def test_aggregate_figures(tmp_path):
+    """The three figures of the reference's aggregate step, when matplotlib is there."""
+    pytest.importorskip("matplotlib")
+    aggregate.aggregate(os.path.join(GOLD, "ensemble_N32", "ensemble.csv"), str(tmp_path), figures=True)
+    for fn in ("sensitivity.png", "healing.png", "contrast.png"):
+        assert (tmp_path / fn).stat().st_size > 1000, fn
