@@ -822,6 +822,57 @@ def test_fused_pcg_large_mesh_finalized_sums():
     assert np.array_equal(op.download("T"), xmo)
 
 
+def test_full_size_fp64_workload():
+    """The plain FP64 workload at its full size: N=128 (2,146,689 nodes),
+    kappa = exp(g), g ~ N(0,1) clipped to [-3,3], rho*c = 1, f = 1, u0 = 0,
+    dt = 1e-4, 50 backward-Euler steps, PCG to 1e-10 with the checksums on,
+    colour-ordered and atomic scatter.  Checks that hold at any size: the
+    first solve is bit-identical to the oracle's; every step converges; D1
+    stays under its threshold on every apply of the clean run; the last
+    iterate's true residual through the oracle's apply meets the tolerance;
+    and the two scatters, which sum in different orders, agree to the
+    tolerance of the solves."""
+    n, dt, steps, tol, cap = 128, 1e-4, 50, 1e-10, 500
+    k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=1)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0, threads=16)
+    finals = {}
+    for scatter in ("colored", "atomic"):
+        print(f"[case] full-size fp64 n={n} {scatter}, {steps} steps to {tol}", flush=True)
+        op = heat().HeatOperator(n, dt, k, rhoc, scatter=scatter, source="const", source_value=1.0,
+                                 detect=True)
+        if scatter == "colored":
+            b0 = orc.build_rhs(np.zeros(op.nn))
+            op.upload("b", b0)
+            op.upload("T", np.zeros(op.nn))
+            it0 = op.pcg_solve("b", "T", cap, tol)
+            ito, xo, _ = orc.pcg_fused(b0, np.zeros(op.nn), cap, tol)
+            assert it0 == ito
+            assert np.array_equal(op.download("T"), xo)
+        thr = op.abft_threshold()
+        op.upload("T", np.zeros(op.nn))
+        op.abft_take()
+        iters = []
+        for _ in range(steps):
+            op.build_rhs("T", "b")
+            iters.append(op.pcg_solve("b", "T", cap, tol))
+        d1, calls = op.abft_take()
+        print(f"  iterations {min(iters)}..{max(iters)}, D1 max {d1:.3g} (threshold {thr:.3g}) "
+              f"over {calls} applies", flush=True)
+        assert max(iters) < cap
+        assert calls == sum(iters) + 2 * steps      # build_rhs + initial residual + one per iteration
+        assert d1 <= thr
+        b, T = op.download("b"), op.download("T")
+        rel = np.linalg.norm(b - orc.apply(T, dt)) / np.linalg.norm(b)
+        print(f"  true relative residual of the last step {rel:.3g}", flush=True)
+        assert rel < 2 * tol
+        finals[scatter] = T
+        op.cleanup()
+    diff = relerr(finals["atomic"], finals["colored"])
+    print(f"  atomic vs colour-ordered final field: {diff:.3g}", flush=True)
+    assert diff < 1e-8
+
+
 @pytest.mark.parametrize("n", [3, 8])
 def test_stream_ceiling_kernel(n):
     """heat_time_stream's kernel writes y = x + (0.5 rho*c*m3 + 0.25 dt*kappa*k2)
