@@ -873,6 +873,73 @@ def test_full_size_fp64_workload():
     assert diff < 1e-8
 
 
+def test_full_size_refined_workload():
+    """The FP32-refined solve on the same N=128 mesh and steps: FP32 operator
+    and inner PCG to 1e-4 inside FP64 defect correction to 1e-10.  Every
+    step's correction loop converges, its first solve is bit-identical to the
+    oracle's, D1 stays under the threshold of each apply's precision, the last
+    iterate's true residual through the oracle's FP64 apply meets the
+    tolerance, and the final field agrees with the FP64 solve's to the
+    tolerance of the solves."""
+    n, dt, steps, tol, tol_in = 128, 1e-4, 50, 1e-10, 1e-4
+    print(f"[case] full-size refined n={n}, {steps} steps to {tol} (inner {tol_in})", flush=True)
+    k, rhoc = fields.iid_lognormal(n, sigma=1.0, clip=3.0, seed=1)
+    k, rhoc = k.ravel(), rhoc.ravel()
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0, threads=16)
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0, detect=True)
+    b0 = orc.build_rhs(np.zeros(op.nn))
+    op.upload("b", b0)
+    op.upload("T", np.zeros(op.nn))
+    got = op.pcg_solve_mixed("b", "T", 50, 500, tol, tol_in)
+    oo, io, xo = orc.pcg_mixed_fused(b0, np.zeros(op.nn), 50, 500, tol, tol_in)
+    assert got == (oo, io)
+    assert np.array_equal(op.download("T"), xo)
+    thr = op.abft_threshold()
+    finals = {}
+    for refined in (True, False):
+        op.upload("T", np.zeros(op.nn))
+        op.abft_take()
+        counts = []
+        for _ in range(steps):
+            op.build_rhs("T", "b")
+            if refined:
+                counts.append(op.pcg_solve_mixed("b", "T", 50, 500, tol, tol_in))
+            else:
+                counts.append((0, op.pcg_solve("b", "T", 500, tol)))
+        d1, calls = op.abft_take()
+        finals[refined] = T = op.download("T")
+        b = op.download("b")
+        rel = np.linalg.norm(b - orc.apply(T, dt)) / np.linalg.norm(b)
+        print(f"  {'refined' if refined else 'fp64'}: (outer, inner) per step {min(counts)}..{max(counts)}, "
+              f"D1 max {d1:.3g} over {calls} applies, true relative residual {rel:.3g}", flush=True)
+        assert max(c[0] for c in counts) < 50 and max(c[1] for c in counts) < (50 * 500 if refined else 500)
+        # the refined solve's record includes its FP32 inner applies, checked
+        # against the FP32 threshold: 10 x their clean floor on this mesh (4.76e-9)
+        assert calls > 0 and d1 <= (4.76e-8 if refined else thr)
+        # the refined loop stops on the FP64 residual itself (summed in another order here)
+        assert rel < (tol * (1 + 1e-6) if refined else 2 * tol)
+    diff = relerr(finals[True], finals[False])
+    print(f"  refined vs FP64 final field: {diff:.3g}", flush=True)
+    assert diff < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["refined", "fp64"])
+def test_full_size_ensemble(precision):
+    """The batched ensemble at its full size: 96 members, N drawn from
+    {16,24,32,48,64,96}, log-normal kappa with sigma from {0.5,1,2}, f = 1,
+    u0 = 0, dt = 1e-3, 20 steps to 1e-8 (FP32-refined with inner 1e-4, or
+    FP64).  Every member's final field and the iteration total equal those
+    of the same members solved one after another through the single-solve
+    path, which the smaller cases pin to the oracle bit for bit."""
+    from mnhack8_amd import ensemble
+    print(f"[case] full-size ensemble {precision}: 96 members, 20 steps to 1e-8", flush=True)
+    out = ensemble.run(count=96, steps=20, tol=1e-8, precision=precision, log=lambda s: print(" ", s, flush=True))
+    assert out["members"] == 96 and out["sizes"] == [16, 24, 32, 48, 64, 96]
+    assert out["sequential_matches_batch_bitwise"]
+    assert out["iterations_total"] == out["sequential_iterations_total"]
+    assert max(out["iterations_max_per_step"]) < (50 * 500 if precision == "refined" else 500)
+
+
 @pytest.mark.parametrize("n", [3, 8])
 def test_stream_ceiling_kernel(n):
     """heat_time_stream's kernel writes y = x + (0.5 rho*c*m3 + 0.25 dt*kappa*k2)
