@@ -1883,6 +1883,25 @@ int heat_batch_step(heat_batch* b, int nsteps, int max_iter, double tol, int* it
         } else {
             k_batch_residual<<<total_sb, NT, 0, s>>>(b->dviews, b->d_sb_off, count);
             CHECK_LAUNCH();
+            // a member with a level-R fault armed forms its first residual
+            // alone, as heat_pcg_solve does, so that the fault fires in that
+            // apply (:999): the batch's residual is replaced by the member's
+            bool armed = false;
+            for (size_t i = 0; i < b->ops.size(); ++i)
+                armed = armed || (b->views[i].active && b->ops[i]->fi.armed && b->ops[i]->fi_level == 2);
+            if (armed) {
+                HIP_TRY(hipStreamSynchronize(s));
+                for (size_t i = 0; i < b->ops.size(); ++i) {
+                    heat_op* op = b->ops[i];
+                    if (!b->views[i].active || !op->fi.armed || op->fi_level != 2) continue;
+                    int rc = apply_impl(op, op->vec[HEAT_VEC_T], op->vec[HEAT_VEC_AP], op->cfg.dt, true, true,
+                                        EPI_NONE, 0);
+                    if (rc) return rc;
+                    k_fused_init<double><<<op->nsb, NT, 0, op->stream>>>(b->views[i]);
+                    CHECK_LAUNCH();
+                    HIP_TRY(hipStreamSynchronize(op->stream));
+                }
+            }
         }
         const int rc = batch_until_done(b, max_iter, [&]() { return batch_chunk(b); });
         if (rc) return rc;

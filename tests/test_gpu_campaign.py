@@ -129,18 +129,24 @@ def test_dump_snaps_files(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("batch", [5, 64])
-def test_batched_campaign_equals_sequential(tmp_path, batch):
+@pytest.mark.parametrize("batch,detect", [(5, True), (64, True), (64, False)])
+def test_batched_campaign_equals_sequential(tmp_path, batch, detect):
     """Fault runs advanced together as one batch of handles (--campaign-batch)
-    write the rows and traces of the same runs taken one after another."""
+    write the rows and traces of the same runs taken one after another
+    (--batch 0), with and without the detectors.  Without them the batch
+    forms its first residuals in one launch, and a member with a level-R
+    fault armed must still take that residual through the apply in which the
+    fault fires: every level-R run then differs from its clean runs."""
     common = ["--N", "12", "--t", "0.01", "--dt", "5e-4", "--sigmas", "1.0,2.0", "--seeds", "3", "--reps", "1",
-              "--targets", "2", "--detect", "--mpi", "1", "--jobs", "2"]
+              "--targets", "2", "--mpi", "1", "--jobs", "2"] + (["--detect"] if detect else [])
     seq, bat = str(tmp_path / "seq"), str(tmp_path / "bat")
-    print(f"[case] campaign N=12 sequential, then batches of {batch}", flush=True)
-    assert campaign.main(["--out", seq] + common) == 0
+    print(f"[case] campaign N=12 sequential, then batches of {batch}, detect={detect}", flush=True)
+    assert campaign.main(["--out", seq, "--batch", "0"] + common) == 0
     assert campaign.main(["--out", bat, "--batch", str(batch)] + common) == 0
     a, b = _rows(seq), _rows(bat)
     assert set(a) == set(b) and len(a) == 2 * 2 * 4 * 3 * 2
+    fired = [r for r in b.values() if r["fi_level"] == "2" and float(r["E_fault_max"]) > 0]
+    assert len(fired) >= 8, "level-R faults of the batch did not fire"
     for rid, ra in a.items():
         rb = b[rid]
         assert ra["status"] == rb["status"] == "ok"
