@@ -293,6 +293,22 @@ int heat_autotune(heat_op* op, int reps, int* impl, int* planes, int* shape, dou
 /* The D1 threshold (chk_tol, src/heat_solver_het.cu:637) used by the driver. */
 int heat_abft_threshold(heat_op* op, double* chk_tol);
 
+/* heat_dot without a handle and without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, such as an external CG loop built on heat_apply_operator,
+ * tight serving loops, harnesses that forbid hipMalloc).  *out (a DEVICE
+ * address) receives sum_i x[i] y[i] over the n device values of x and y, in
+ * heat_dot's fixed order (256-value block partials, then a 1024-way strided
+ * sum and tree), so the bits equal heat_dot's for the same vectors.  The block
+ * partials live in a file-scope device array of HEAT_STATIC_DOT_MAX_BLOCKS
+ * doubles (256 KiB, held by each GPU the library runs on until the process
+ * ends): n must be in 0..256 * HEAT_STATIC_DOT_MAX_BLOCKS (8,388,608 values;
+ * N <= 202), else HEAT_E_ARG.  Queued on STREAM (a hipStream_t; NULL: the
+ * null stream) with no host wait; no hipMalloc, hipFree or memset.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+#define HEAT_STATIC_DOT_MAX_BLOCKS 32768
+int heat_dot_static(const double* x, const double* y, int64_t n, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1414,6 +1414,14 @@ __global__ __launch_bounds__(FT) void k_fin_dot(const double* __restrict__ parti
     if (threadIdx.x == 0) st->res[slot] = out[0];
 }
 
+// the same second stage with the result stored at a device address
+__global__ __launch_bounds__(FT) void k_fin_dot_to(const double* __restrict__ partials, int nb,
+                                                   double* __restrict__ result) {
+    double out[1];
+    final_tree<1>(partials, nb, out);
+    if (threadIdx.x == 0) *result = out[0];
+}
+
 __global__ __launch_bounds__(FT) void k_fin_minmax(const double* __restrict__ partials, int nb,
                                                    DevState* st, int slot) {
     __shared__ double smin[FT], smax[FT];

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -811,6 +812,32 @@ int ensure_mixed(heat_op* op) {
     return HEAT_OK;
 }
 
+// Block partials of heat_dot_static: a file-scope device array, so that the
+// entry point allocates nothing.  Each GPU holds its own copy until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (static_scratch_ptr).
+static __device__ __align__(256) double scratch[HEAT_STATIC_DOT_MAX_BLOCKS];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int static_scratch_ptr(double** out) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(HEAT_E_STATE, "device %d outside 0..%d", dev, MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(HEAT_E_HIP, "hipGetSymbolAddress returned a null address");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HEAT_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -965,6 +992,23 @@ int heat_norm2(heat_op* op, const double* x, double* out) {
     int rc = heat_dot(op, x, x, &d2);
     if (rc) return rc;
     *out = std::sqrt(d2);
+    return HEAT_OK;
+}
+
+int heat_dot_static(const double* x, const double* y, int64_t n, double* out, void* stream) {
+    if (!x || !y || !out) return fail(HEAT_E_ARG, "null argument");
+    if (n < 0 || n > (int64_t)NT * HEAT_STATIC_DOT_MAX_BLOCKS)
+        return fail(HEAT_E_ARG, "n = %lld outside 0..%lld (the static partials hold %d blocks of %d)",
+                    (long long)n, (long long)NT * HEAT_STATIC_DOT_MAX_BLOCKS, HEAT_STATIC_DOT_MAX_BLOCKS, NT);
+    double* part = nullptr;
+    int rc = static_scratch_ptr(&part);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nb = (int)blocks(n, NT);
+    // every block writes its partial, so nothing of the last call's contents is read
+    if (nb > 0) k_dot<<<nb, NT, 0, s>>>(x, y, (int)n, part, nullptr, 0);
+    k_fin_dot_to<<<1, FT, 0, s>>>(part, nb, out);
+    CHECK_LAUNCH();
     return HEAT_OK;
 }
 

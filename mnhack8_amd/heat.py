@@ -101,6 +101,7 @@ SIGNATURES = {
     "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
     "heat_time_stream": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
+    "heat_dot_static": (_I, [_P, _P, _I64, _P, _P]),
 }
 
 _lib = None
