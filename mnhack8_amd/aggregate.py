@@ -125,56 +125,62 @@ def _write(path, header, table):
 
 
 def _figures(out, rows, P, by_sigma):
+    """Three figures under the reference's file names: per-level bars of
+    P(not benign) by bit class with their Wilson intervals; the cumulative
+    distribution of the healing ratio E_final / E_max per outcome class; and
+    P(not benign) and the clean CG work against the field's contrast."""
     import matplotlib
     matplotlib.use("Agg")
     import matplotlib.pyplot as plt
 
-    M = np.full((len(LEVEL_ORDER), len(CLASSES)), np.nan)
-    for (lv, bc), (p, _, _, _) in P.items():
-        M[LEVEL_ORDER.index(lv), CLASSES.index(bc)] = p
-    fig, ax = plt.subplots(figsize=(6, 3.6))
-    im = ax.imshow(M, vmin=0, vmax=1, cmap="magma")
+    levels = [lv for lv in LEVEL_ORDER if any(k[0] == lv for k in P)]
+    fig, ax = plt.subplots(figsize=(7, 3.2))
+    width = 0.8 / max(1, len(levels))
+    for j, lv in enumerate(levels):
+        xs, ps, err = [], [], [[], []]
+        for i, bc in enumerate(CLASSES):
+            if (lv, bc) in P:
+                p, lo, hi, _ = P[(lv, bc)]
+                xs.append(i + (j - (len(levels) - 1) / 2) * width)
+                ps.append(p)
+                err[0].append(p - lo)
+                err[1].append(hi - p)
+        ax.bar(xs, ps, width, yerr=err, capsize=2, label=f"level {lv}")
     ax.set_xticks(range(len(CLASSES)), CLASSES)
-    ax.set_yticks(range(len(LEVEL_ORDER)), ["G state", "R accum", "H halo"])
-    for (lv, bc), (p, lo, hi, n) in P.items():
-        ax.text(CLASSES.index(bc), LEVEL_ORDER.index(lv), f"{p:.2f}\n[{lo:.2f},{hi:.2f}]\nn={n}", ha="center",
-                va="center", fontsize=7, color="white" if p < 0.6 else "black")
-    fig.colorbar(im, label="P(not benign)")
-    ax.set_title("SEU sensitivity map (Wilson 95% CI)")
+    ax.set_ylim(0, 1.25)
+    ax.set_ylabel("P(not benign), 95% interval")
+    ax.legend(fontsize=8, ncol=3, loc="upper center")
     fig.tight_layout()
-    fig.savefig(os.path.join(out, "sensitivity.png"), dpi=150)
+    fig.savefig(os.path.join(out, "sensitivity.png"), dpi=120)
     plt.close(fig)
 
-    fig, ax = plt.subplots(figsize=(5, 4))
-    colours = {"benign": "tab:green", "transient": "tab:blue", "persistent": "tab:red", "detected": "k"}
-    for c, col in colours.items():
-        rs = [r for r in rows if r["class"] == c]
-        if rs:
-            ax.loglog([max(fnum(r["E_fault_max"]), 1e-18) for r in rs],
-                      [max(fnum(r["E_fault_final"]), 1e-18) for r in rs], ".", label=c, color=col, alpha=0.6)
-    ax.plot([1e-18, 1e12], [1e-18, 1e12], "k:", lw=0.5)
-    ax.set_xlabel("E_max")
-    ax.set_ylabel("E_final")
-    ax.legend()
-    ax.set_title("healing: below the diagonal = diffusion removed error")
+    fig, ax = plt.subplots(figsize=(5, 3.2))
+    for c in ("benign", "transient", "persistent"):
+        ratio = sorted(np.log10(max(fnum(r["E_fault_final"]), 1e-300) / fnum(r["E_fault_max"]))
+                       for r in rows if r["class"] == c and fnum(r["E_fault_max"]) > 0)
+        if ratio:
+            ax.step(ratio, np.arange(1, len(ratio) + 1) / len(ratio), where="post", label=f"{c} ({len(ratio)})")
+    ax.set_xlabel("log10(E_final / E_max)")
+    ax.set_ylabel("fraction of runs")
+    ax.legend(fontsize=8)
     fig.tight_layout()
-    fig.savefig(os.path.join(out, "healing.png"), dpi=150)
+    fig.savefig(os.path.join(out, "healing.png"), dpi=120)
     plt.close(fig)
 
     sig = sorted(by_sigma)
+    contrast = [np.mean([fnum(r["kmax"]) / fnum(r["kmin"]) for r in by_sigma[s]]) for s in sig]
     stats = [wilson(sum(r["class"] != "benign" for r in by_sigma[s]), len(by_sigma[s])) for s in sig]
     cg = [np.mean([fnum(r["mean_cg_iters_A"]) for r in by_sigma[s]]) for s in sig]
-    fig, ax1 = plt.subplots(figsize=(5, 3.4))
-    ax2 = ax1.twinx()
-    ax1.errorbar(sig, [p for p, _, _ in stats], yerr=[[p - lo for p, lo, _ in stats], [hi - p for p, _, hi in stats]],
-                 fmt="o-", color="tab:red")
-    ax1.set_ylabel("P(not benign)", color="tab:red")
-    ax2.plot(sig, cg, "s--", color="tab:blue")
-    ax2.set_ylabel("mean CG iterations", color="tab:blue")
-    ax1.set_xlabel("sigma of ln k (material contrast)")
-    ax1.set_title("contrast dependence")
+    fig, (top, bottom) = plt.subplots(2, 1, sharex=True, figsize=(4.5, 4.2))
+    top.fill_between(contrast, [lo for _, lo, _ in stats], [hi for _, _, hi in stats], alpha=0.3)
+    top.plot(contrast, [p for p, _, _ in stats], "o-")
+    top.set_ylabel("P(not benign)")
+    bottom.plot(contrast, cg, "s-")
+    bottom.set_ylabel("CG iterations per step")
+    bottom.set_xscale("log")
+    bottom.set_xlabel("kmax / kmin")
     fig.tight_layout()
-    fig.savefig(os.path.join(out, "contrast.png"), dpi=150)
+    fig.savefig(os.path.join(out, "contrast.png"), dpi=120)
     plt.close(fig)
 
 

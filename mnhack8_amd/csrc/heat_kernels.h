@@ -874,52 +874,118 @@ __device__ __forceinline__ bool resolve_element(const Grid& g, int tid, int colo
 
 __constant__ int c_off[8][3] = {{0,0,0},{1,0,0},{1,1,0},{0,1,0},{0,0,1},{1,0,1},{1,1,1},{0,1,1}};
 
-__device__ __forceinline__ void shape_func_grad(double xi, double eta, double zeta, double N[8],
-                                                double dN[8][3]) {
-    const int sg[8][3] = {{-1,-1,-1},{1,-1,-1},{1,1,-1},{-1,1,-1},
-                          {-1,-1,1},{1,-1,1},{1,1,1},{-1,1,1}};
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        double sx = sg[i][0], sy = sg[i][1], sz = sg[i][2];
-        double a = 1 + sx * xi, b = 1 + sy * eta, c = 1 + sz * zeta;
-        N[i] = 0.125 * a * b * c;
-        dN[i][0] = 0.125 * sx * b * c;
-        dN[i][1] = 0.125 * a * sy * c;
-        dN[i][2] = 0.125 * a * b * sz;
-    }
-}
+// The two-point Gauss rule on the unit interval: abscissae t0 = (1 - 1/sqrt(3))/2
+// and t1 = 1 - t0, weights 1/2.  At t_p the two linear hats phi_0 = 1 - t and
+// phi_1 = t take GW[p][a]; their derivatives are -1 and +1 (per unit length).
+constexpr double GT0 = 0.21132486540518713, GT1 = 0.7886751345948129;
+__constant__ double GW[2][2] = {{GT1, GT0}, {GT0, GT1}};
 
-// elem_apply_gauss, src/heat_solver_het.cu:310-338
+// Element operator (rhoc M + dt k K) x_e by quadrature at the 2x2x2 Gauss
+// points, evaluated by sum factorisation: the nodal values u[a][b][c] (a, b,
+// c = the node's x, y, z offsets) are carried to the points one axis at a time
+// (values: weights GW; the derivative along an axis: u[..1..] - u[..0..],
+// constant along that axis), the point terms rhoc*u and dt*k*grad(u) are
+// formed, and the transposed one-axis contractions carry them back to the
+// nodes.  On a cube the rule integrates both matrices exactly, so this is
+// the operator of the closed-form rows up to rounding (the reference's
+// quadrature kernel, elem_apply_gauss at src/heat_solver_het.cu:310-338,
+// computes the same sums as a loop over the eight points).
 __device__ __forceinline__ void elem_gauss(const double xe[8], double ye[8], double k_e,
                                            double rhoc_e, double dt, double h) {
-    const double det_J = h * h * h / 8.0;
-    const double inv_J = 2.0 / h;
-    const double gp = 1.0 / 1.7320508075688772;
-    const double pts[2] = {-gp, gp};
+    double u[2][2][2];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) ye[i] = 0.0;
-    for (int gi = 0; gi < 2; ++gi)
-    for (int gj = 0; gj < 2; ++gj)
-    for (int gk = 0; gk < 2; ++gk) {
-        double N[8], dNref[8][3];
-        shape_func_grad(pts[gi], pts[gj], pts[gk], N, dNref);
-        double dNx[8], dNy[8], dNz[8];
+    for (int i = 0; i < 8; ++i) u[c_off[i][0]][c_off[i][1]][c_off[i][2]] = xe[i];
+    // along z: values at the two z points, and the z difference
+    double cz[2][2][2], dz[2][2];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            dNx[i] = inv_J * dNref[i][0];
-            dNy[i] = inv_J * dNref[i][1];
-            dNz[i] = inv_J * dNref[i][2];
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) cz[a][b][r] = GW[r][0] * u[a][b][0] + GW[r][1] * u[a][b][1];
+            dz[a][b] = u[a][b][1] - u[a][b][0];
+        }
+    // along y
+    double cyz[2][2][2], dy[2][2], dzy[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) cyz[a][q][r] = GW[q][0] * cz[a][0][r] + GW[q][1] * cz[a][1][r];
+            dzy[a][q] = GW[q][0] * dz[a][0] + GW[q][1] * dz[a][1];
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            double Mx = 0.0, Kx = 0.0;
+        for (int r = 0; r < 2; ++r) dy[a][r] = cz[a][1][r] - cz[a][0][r];
+    }
+    // along x: the value at every point, and the three differences
+    double val[2][2][2], gx[2][2], gy[2][2], gz[2][2];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                Mx += rhoc_e * N[i] * N[j] * xe[j];
-                Kx += k_e * (dNx[i] * dNx[j] + dNy[i] * dNy[j] + dNz[i] * dNz[j]) * xe[j];
-            }
-            ye[i] += det_J * (Mx + dt * Kx);
+    for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) val[p][q][r] = GW[p][0] * cyz[0][q][r] + GW[p][1] * cyz[1][q][r];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) gy[p][r] = GW[p][0] * dy[0][r] + GW[p][1] * dy[1][r];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) gz[p][q] = GW[p][0] * dzy[0][q] + GW[p][1] * dzy[1][q];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) gx[q][r] = cyz[1][q][r] - cyz[0][q][r];
+    // back to the nodes: the mass term through all three axes; each gradient
+    // term through the two axes across it (summed over the two points along
+    // it, where it is constant: the factor 2)
+    double mv[2][2][2], kx[2][2], ky[2][2], kz[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+        double t1[2][2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 2; ++r) t1[q][r] = GW[0][a] * val[0][q][r] + GW[1][a] * val[1][q][r];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            double t2[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) t2[r] = GW[0][b] * t1[0][r] + GW[1][b] * t1[1][r];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) mv[a][b][c] = GW[0][c] * t2[0] + GW[1][c] * t2[1];
         }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double hx0 = GW[0][b] * gx[0][0] + GW[1][b] * gx[1][0];
+            const double hx1 = GW[0][b] * gx[0][1] + GW[1][b] * gx[1][1];
+            kx[b][c] = GW[0][c] * hx0 + GW[1][c] * hx1;
+        }
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double hy0 = GW[0][a] * gy[0][0] + GW[1][a] * gy[1][0];
+            const double hy1 = GW[0][a] * gy[0][1] + GW[1][a] * gy[1][1];
+            ky[a][c] = GW[0][c] * hy0 + GW[1][c] * hy1;
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const double hz0 = GW[0][a] * gz[0][0] + GW[1][a] * gz[1][0];
+            const double hz1 = GW[0][a] * gz[0][1] + GW[1][a] * gz[1][1];
+            kz[a][b] = GW[0][b] * hz0 + GW[1][b] * hz1;
+        }
+    }
+    // point weights: (h/2)^3 per point; gradients carry 1/h each
+    const double wq = h * h * h / 8.0;
+    const double ks = 2.0 * dt * k_e / (h * h);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int a = c_off[i][0], b = c_off[i][1], c = c_off[i][2];
+        const double stiff = (a ? kx[b][c] : -kx[b][c]) + (b ? ky[a][c] : -ky[a][c]) + (c ? kz[a][b] : -kz[a][b]);
+        ye[i] = wq * (rhoc_e * mv[a][b][c] + ks * stiff);
     }
 }
 
@@ -1235,39 +1301,42 @@ __global__ __launch_bounds__(ET) void k_diag(double* __restrict__ d,
     }
 }
 
-// source_kernel, src/heat_solver_het.cu:380-407 (mode 0 Gaussian, 1 constant)
+// The source vector F_i = integral of Q N_i (source_kernel, src/heat_solver_het.cu:
+// 380-407, which applies the 2x2x2 Gauss rule).  Both source models separate
+// over the axes: Q = 1000 G(x) G(y) G(z) with G(s) = exp(-(s - 1/2)^2 / (2 0.1^2))
+// (source_at, :229-237), or Q constant.  So the element's Gauss sum for the
+// node with offsets (a, b, c) is pref * X_a * Y_b * Z_c with X_a = sum_p
+// phi_a(t_p) G(x_p) over the two points of the element's x interval (and
+// likewise Y, Z), pref = 1000 h^3/8; for constant Q the one-axis sums are 1
+// exactly and F_e = Q h^3/8 at each of the eight nodes.
+__device__ __forceinline__ void source_axis(double lo, double h, double out[2]) {
+    const double s0 = fma(h, GT0, lo) - 0.5, s1 = fma(h, GT1, lo) - 0.5;
+    const double g0 = exp(-(s0 * s0) / (2.0 * 0.1 * 0.1)), g1 = exp(-(s1 * s1) / (2.0 * 0.1 * 0.1));
+    out[0] = fma(GT1, g0, GT0 * g1);
+    out[1] = fma(GT0, g0, GT1 * g1);
+}
+
 template <bool ATOMIC>
 __global__ __launch_bounds__(ET) void k_source(double* __restrict__ F, Grid g, int mode,
                                                double qconst, int color) {
     const int tid = blockIdx.x * ET + threadIdx.x;
     int e, ix, iy, iz;
     if (!resolve_element(g, tid, color, e, ix, iy, iz)) return;
-    const double x0 = ix * g.h, y0 = iy * g.h, z0 = iz * g.h;
-    const double det_J = g.h * g.h * g.h / 8.0;
-    const double gp = 1.0 / 1.7320508075688772;
-    const double pts[2] = {-gp, gp};
-    double Fe[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int gi = 0; gi < 2; ++gi)
-    for (int gj = 0; gj < 2; ++gj)
-    for (int gk = 0; gk < 2; ++gk) {
-        const double xg = x0 + (pts[gi] + 1.0) * 0.5 * g.h;
-        const double yg = y0 + (pts[gj] + 1.0) * 0.5 * g.h;
-        const double zg = z0 + (pts[gk] + 1.0) * 0.5 * g.h;
-        double Q;
-        if (mode == 1) Q = qconst;
-        else {
-            const double r2 = (xg - 0.5) * (xg - 0.5) + (yg - 0.5) * (yg - 0.5) + (zg - 0.5) * (zg - 0.5);
-            Q = 1000.0 * exp(-r2 / (2.0 * 0.1 * 0.1));
-        }
-        double N[8], dN[8][3];
-        shape_func_grad(pts[gi], pts[gj], pts[gk], N, dN);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) Fe[i] += det_J * Q * N[i];
+    const double vol8 = g.h * g.h * g.h / 8.0;
+    double X[2] = {1.0, 1.0}, Y[2] = {1.0, 1.0}, Z[2] = {1.0, 1.0};
+    double pref = qconst * vol8;
+    if (mode != 1) {
+        source_axis(ix * g.h, g.h, X);
+        source_axis(iy * g.h, g.h, Y);
+        source_axis(iz * g.h, g.h, Z);
+        pref = 1000.0 * vol8;
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int nid = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
-        if (ATOMIC) unsafeAtomicAdd(&F[nid], Fe[i]); else F[nid] += Fe[i];
+        const int a = c_off[i][0], b = c_off[i][1], c = c_off[i][2];
+        const double fe = ((pref * X[a]) * Y[b]) * Z[c];
+        const int nid = (ix + a) * g.sx + (iy + b) * g.sy + (iz + c);
+        if (ATOMIC) unsafeAtomicAdd(&F[nid], fe); else F[nid] += fe;
     }
 }
 

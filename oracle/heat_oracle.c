@@ -7,8 +7,9 @@
  * CPU-baseline leg of bench.py use it, as the checker.  Built with
  * -ffp-contract=off so every rounding is the one written here.
  *
- * Parity pins (checked by tests/test_oracle_golden.py against the reference's
- * committed outputs under tests/baseline_validation/v10): V0 eigenmode errors
+ * Parity pins (checked by tests/test_golden_pins.py and tests/test_oracle_math.py
+ * against the reference's committed outputs under tests/baseline_validation/v10
+ * and against closed forms): V0 eigenmode errors
  * (11 digits), per-step CG counts and D1/D2/D3 traces of the N=32 lognormal
  * run, fault classes/detected_by/first-firing steps of the G and R campaign,
  * the V1 quadrants norm, and the closed forms of the reference matrices.
@@ -463,6 +464,39 @@ static void source_element(ho_op* op, double* F, int ix, int iy, int iz) {
     for (int i = 0; i < 8; ++i) F[gid[i]] += Fe[i];
 }
 
+/* The HIP library's separable form of the same Gauss sum (k_source): both
+ * source models factor over the axes, so node (a,b,c) of an element gets
+ * ((pref * X_a) * Y_b) * Z_c with X_a = sum_p phi_a(t_p) G(x_p) over the two
+ * Gauss points t_p of the element's x interval, pref = 1000 h^3/8 (Gaussian)
+ * or Q h^3/8 with X = Y = Z = 1 (constant).  Used with the closed-form
+ * ("struct") element rows, whose arithmetic it shares with the GPU. */
+static const double GT0 = 0.21132486540518713, GT1 = 0.7886751345948129;
+
+static void source_axis_sep(double lo, double h, double out[2]) {
+    const double s0 = fma(h, GT0, lo) - 0.5, s1 = fma(h, GT1, lo) - 0.5;
+    const double g0 = exp(-(s0 * s0) / (2.0 * 0.1 * 0.1)), g1 = exp(-(s1 * s1) / (2.0 * 0.1 * 0.1));
+    out[0] = fma(GT1, g0, GT0 * g1);
+    out[1] = fma(GT0, g0, GT1 * g1);
+}
+
+static void source_element_sep(ho_op* op, double* F, int ix, int iy, int iz) {
+    const ho_grid* g = &op->g;
+    if (op->verify) return;
+    long gid[8];
+    element_nodes(g, ix, iy, iz, gid);
+    const double vol8 = g->h * g->h * g->h / 8.0;
+    double X[2] = {1.0, 1.0}, Y[2] = {1.0, 1.0}, Z[2] = {1.0, 1.0};
+    double pref = op->source_value * vol8;
+    if (op->source_mode != HO_SOURCE_CONST) {
+        source_axis_sep(ix * g->h, g->h, X);
+        source_axis_sep(iy * g->h, g->h, Y);
+        source_axis_sep(iz * g->h, g->h, Z);
+        pref = 1000.0 * vol8;
+    }
+    for (int i = 0; i < 8; ++i)
+        F[gid[i]] += ((pref * X[OFF[i][0]]) * Y[OFF[i][1]]) * Z[OFF[i][2]];
+}
+
 static void diag_element(ho_op* op, double* d, long e, int ix, int iy, int iz) {
     long gid[8];
     element_nodes(&op->g, ix, iy, iz, gid);
@@ -485,6 +519,7 @@ static void element_loop(ho_op* op, double* v, int what) {
             long e; int ix, iy, iz;
             resolve_element(g, t, c, &e, &ix, &iy, &iz);
             if (what == 0) diag_element(op, v, e, ix, iy, iz);
+            else if (op->kernel == HO_KERNEL_STRUCT) source_element_sep(op, v, ix, iy, iz);
             else source_element(op, v, ix, iy, iz);
         }
     }

@@ -119,9 +119,9 @@ def test_apply_gauss(scatter):
     xd, yd = dev(x), dev(np.zeros_like(x))
     op.apply_operator(xd, yd, dt, project=True)
     y = host(op, yd)
+    # the oracle restates the reference's point-by-point quadrature loop; the
+    # GPU evaluates the same rule by sum factorisation (other roundings)
     yo = orc.apply(x, dt)
-    if scatter == "colored":
-        assert np.array_equal(y, yo)
     assert relerr(y, yo) < 1e-14
     assert relerr(op.download("diag"), orc.vector("diag")) < 1e-15
 

@@ -115,6 +115,52 @@ def test_symmetry_and_constant_nullspace(scatter):
     assert abs(a - b) <= 1e-13 * abs(a)
 
 
+def gauss_source_numpy(n, q):
+    """F_i = sum over elements and the 2x2x2 Gauss points of (h/2)^3 Q N_i,
+    assembled with numpy from the point coordinates, Dirichlet rows zeroed."""
+    h = 1.0 / n
+    t = np.array([0.5 - 0.5 / math.sqrt(3.0), 0.5 + 0.5 / math.sqrt(3.0)])
+    F = np.zeros((n + 1,) * 3)
+    e = np.arange(n)
+    for p in range(2):
+        for qq in range(2):
+            for r in range(2):
+                X, Y, Z = np.meshgrid((e + t[p]) * h, (e + t[qq]) * h, (e + t[r]) * h, indexing="ij")
+                Q = q(X, Y, Z)
+                for a in range(2):
+                    for b in range(2):
+                        for c in range(2):
+                            N = ((t[p] if a else 1 - t[p]) * (t[qq] if b else 1 - t[qq])
+                                 * (t[r] if c else 1 - t[r]))
+                            F[a:a + n, b:b + n, c:c + n] += h ** 3 / 8 * Q * N
+    F[0, :, :] = F[-1, :, :] = F[:, 0, :] = F[:, -1, :] = F[:, :, 0] = F[:, :, -1] = 0.0
+    return F.ravel()
+
+
+def test_source_vector_forms():
+    """The separable source (the GPU's arithmetic, with the closed-form rows)
+    against the reference's point-by-point Gauss sum and an independent numpy
+    assembly of the same rule; the constant source against its closed form
+    (every interior node is shared by eight elements: F = Q h^3)."""
+    n = 7
+    ones = np.ones(n ** 3)
+    gauss_q = lambda X, Y, Z: 1000.0 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2) / 0.02)
+    sep = OracleOp(n, 1e-3, ones, ones, kernel="struct").vector("F")
+    lit = OracleOp(n, 1e-3, ones, ones, kernel="fast").vector("F")
+    ref = gauss_source_numpy(n, gauss_q)
+    assert np.max(np.abs(sep - lit)) <= 1e-14 * np.max(np.abs(lit))
+    assert np.max(np.abs(sep - ref)) <= 1e-14 * np.max(np.abs(ref))
+    # a planted slip (Gauss points shifted by a tenth of a cell) moves F far beyond that bar
+    assert np.max(np.abs(gauss_source_numpy(n, lambda X, Y, Z: gauss_q(X + 0.1 / n, Y, Z)) - ref)) > 1e-6 * np.max(ref)
+    Q = 2.5
+    sepc = OracleOp(n, 1e-3, ones, ones, kernel="struct", source="const", source_value=Q).vector("F")
+    litc = OracleOp(n, 1e-3, ones, ones, kernel="fast", source="const", source_value=Q).vector("F")
+    interior = interior_mask(n)
+    assert np.max(np.abs(sepc[interior] - Q / n ** 3)) <= 1e-15 * Q / n ** 3
+    assert np.max(np.abs(litc - sepc)) <= 1e-15 * Q / n ** 3
+    assert (sepc[~interior] == 0.0).all()
+
+
 def test_diag_is_matrix_diagonal():
     n = 4
     rng = np.random.default_rng(2)
