@@ -1411,8 +1411,10 @@ __global__ __launch_bounds__(NT) void k_minmax(const double* __restrict__ x, int
 // ---------------------------------------------------------------------------
 // after an apply pass: alpha = rho / p.Ap (EPI_PCG*) and the D1 record (EPI_*D1)
 template <int EPI>
+// (inner != 0: the inner FP32 solve of a refined one, whose breakdown test is
+// scaled by the squared norm of its right-hand side, as in the fused order)
 __global__ __launch_bounds__(FT) void k_fin_apply(const double* __restrict__ partials, int nb,
-                                                  DevState* st, double nw, int check_done) {
+                                                  DevState* st, double nw, int check_done, int inner = 0) {
     if (check_done && st->done) return;
     constexpr int K = (EPI == EPI_PCG) ? 1 : (EPI == EPI_D1 ? 3 : 4);
     double out[K];
@@ -1421,7 +1423,8 @@ __global__ __launch_bounds__(FT) void k_fin_apply(const double* __restrict__ par
         int o = 0;
         if (EPI == EPI_PCG || EPI == EPI_PCG_D1) {
             const double pAp = out[o++];
-            if (fabs(pAp) < 1e-30) { st->done = 1; st->broke = 1; }   // :1013
+            const double brk = inner ? 1e-30 * st->bnorm * st->bnorm : 1e-30;
+            if (fabs(pAp) < brk) { st->done = 1; st->broke = 1; }   // :1013
             else st->alpha = st->rho / pAp;
         }
         if (EPI == EPI_D1 || EPI == EPI_PCG_D1) abft_record(st, out[o], out[o + 1], out[o + 2], nw);

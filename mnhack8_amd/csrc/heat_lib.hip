@@ -1171,7 +1171,7 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
                     k_apply_post<float, EPI_PCG><<<op->nbn, NT, 0, s>>>(op->p32, op->ap32, g, 0, nullptr, 0.0,
                                                                        op->partials, op->st, 1);
                     CHECK_LAUNCH();
-                    k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1);
+                    k_fin_apply<EPI_PCG><<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, 0.0, 1, 1);
                     CHECK_LAUNCH();
                     k_mixed_update<<<op->nbn, NT, 0, s>>>(op->e32, op->p32, op->r32, op->ap32, op->d32,
                                                            op->z32, g.nn, op->partials, op->st);

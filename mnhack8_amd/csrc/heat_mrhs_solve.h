@@ -167,8 +167,10 @@ __global__ __launch_bounds__(NT) void k_mr_fin_pap(const double* __restrict__ pa
             const double cur = st->d1max[c];
             if (!isnan(cur) && (isnan(rel) || rel > cur)) st->d1max[c] = rel;   // a NaN stays
         }
-        if (fabs(o[0]) < 1e-30) {
-            st->done[c] = 1;                                        // breakdown
+        // breakdown, scaled by the squared norm of the column's inner right-hand
+        // side (the FP64 residual, which shrinks with every correction)
+        if (fabs(o[0]) < 1e-30 * st->bnorm[c] * st->bnorm[c]) {
+            st->done[c] = 1;
             st->alpha[c] = 0.0;
         } else {
             st->alpha[c] = st->rz[c] / o[0];

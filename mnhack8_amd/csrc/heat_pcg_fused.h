@@ -706,7 +706,11 @@ __device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) 
     // D1 of this iteration's apply is recorded before the breakdown check, as
     // the reference's apply_operator does (:908-919 run before :1013)
     if (DETECT && threadIdx.x == 0 && j == 0) abft_record(st, o[1], o[2], o[3], nwA);
-    if (fabs(pAp) < 1e-30) {                         // :1013
+    // breakdown (:1013): the reference's |p.Ap| < 1e-30 for the FP64 solve; an
+    // inner FP32 solve of a refined one solves A e = r for a residual r that
+    // shrinks with every correction, so its test is scaled by |r|^2
+    const double brk = (sizeof(T) == sizeof(float)) ? 1e-30 * fs->bnorm * fs->bnorm : 1e-30;
+    if (fabs(pAp) < brk) {
         if (threadIdx.x == 0 && j == 0) fs->done = 1;
         return;
     }

@@ -952,7 +952,7 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
                 const int hit = fi && total == op->sfi_iter;
                 if (hit && op->sfi_vec == HO_SEU_P) { flip_f32(p, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
                 if (hit && op->sfi_vec == HO_SEU_AP) { flip_f32(ap, op->sfi_idx, op->sfi_bit); op->sfi_fired = 1; }
-                if (fabs(t[0]) < 1e-30) break;
+                if (fabs(t[0]) < 1e-30 * bnorm * bnorm) break;   /* scaled to the inner right-hand side */
                 const double alpha = rho / t[0];
                 for (long i = 0; i < n; ++i) {
                     e[i] = (float)fma(alpha, (double)p[i], (double)e[i]);

@@ -384,6 +384,23 @@ def test_fused_mixed_reaches_fp64_tolerance():
     assert 2 <= outer <= 6 and inner > outer
 
 
+def test_fused_mixed_is_scale_invariant():
+    """The refined solve of 2^-40 b is 2^-40 times the solve of b, bit for bit
+    (a power of two scales every FP32 and FP64 rounding exactly), with the
+    same outer and inner counts: every test inside the solve (the outer and
+    inner tolerances, and the inner breakdown test) is relative to its
+    right-hand side.  An absolute breakdown threshold stops the inner solves
+    of the scaled system at once and leaves it unconverged."""
+    op, b = lognormal_problem(11)
+    s = 2.0 ** -40
+    o1, i1, x1 = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    o2, i2, x2 = op.pcg_mixed_fused(s * b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert (o1, i1) == (o2, i2)
+    assert np.array_equal(x2, s * x1)
+    r = s * b - op.apply(x2, op.dt)
+    assert np.linalg.norm(r) / np.linalg.norm(s * b) < 1e-10
+
+
 def fused_solve_with_fault(op, b, vec, node, bit, it, tol=1e-12):
     op.arm_pcg_fault(vec, node, bit, it)
     out = op.pcg_fused(b, np.zeros(op.nn), 500, tol)
