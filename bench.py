@@ -57,7 +57,10 @@ def cpu_baseline(n, k, rhoc, dt, reps=20, warm=3):
     """The CPU oracle's colour-sweep matvec (same arithmetic), OpenMP threads."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleOp
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    # every core this process may run on, unless OMP_NUM_THREADS names fewer
+    # (the GPU box sets it to the host share of one GPU)
+    avail = len(os.sched_getaffinity(0))
+    threads = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", avail) or avail)))
     orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="colored", source="const",
                    source_value=1.0, threads=threads)
     x = np.random.default_rng(0).standard_normal(orc.nn)
@@ -71,7 +74,9 @@ def cpu_baseline(n, k, rhoc, dt, reps=20, warm=3):
     med = float(np.median(times))
     res = {"value": orc.nn / med / 1e9, "unit": "GDOF/s", "cores": threads, "kind": "port",
            "sample": f"{reps} colour-sweep matvecs of the same N={n} mesh after {warm} warm-ups, median "
-                     f"{med * 1e3:.1f} ms per apply"}
+                     f"{med * 1e3:.1f} ms per apply",
+           "host_cpus": os.cpu_count(), "affinity_cpus": avail,
+           "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
     # the first time step's FP64 PCG to 1e-8 (T^0 = 0, b = dt F) on the same threads
     b = orc.build_rhs(np.zeros(orc.nn))
     t0 = time.perf_counter()
@@ -103,6 +108,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-mrhs", action="store_true", help="skip the 16-RHS N=320 apply measurement")
+    ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-resident N=256 apply measurement")
     ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="run this many axpy passes (2 node vectors read, 1 written) first: the known-bytes "
@@ -170,6 +176,7 @@ def main():
     host_s = time.perf_counter() - t_host0
     gdofs = nn / (avg_ms * 1e-3) / 1e9
     log(f"warm: {avg_ms * 1e3:.2f} us/apply  {gdofs:.1f} GDOF/s  (host wall {host_s * 1e3:.1f} ms)")
+
     cold_ms, cold_min = op.time_matvec("p", "Ap", 20, flush_bytes=1 << 30)
     log(f"cold (1 GiB flush before each): {cold_ms * 1e3:.2f} us/apply")
 
@@ -202,7 +209,12 @@ def main():
                                f"FP64 colour-ordered matvec y = P(M + dt kappa K)x",
                    "n": n, "nodes": nn, "elements": ne, "scatter": "colored (node-centric, colour order)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"k_apply_tdma ({sched_key})", "us_per_launch": round(avg_ms * 1e3, 2),
+                     "timing": "mean of the timed back-to-back launches, HIP events on the library's stream; "
+                               "a kernel trace of the same launches gives the same per-dispatch duration",
+                     "cache_resident": "the N=128 working set (68 MB per apply) stays in the 256 MB Infinity "
+                                       "Cache; cold_cache and hbm_resident give the HBM-bound figures"},
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
                        "achieved_GBs": round(achieved_cold, 1), "frac": round(achieved_cold / HBM_PEAK_GBS, 4)},
         "stream_ceiling": {"kernel": "k_stream_ceiling (x, rho*c*m3, dt*kappa*k2 read; y written; 16-byte units)",
@@ -236,6 +248,34 @@ def main():
                            "max_rel_diff_vs_colour": float(np.max(np.abs(y_at - y_col)) / np.max(np.abs(y_col)))},
     }
     log(f"scatter forms: colour sweep {ms_sweep * 1e3:.1f} us, atomic {ms_atomic * 1e3:.1f} us")
+
+    if not a.no_hbm:
+        # the same apply where its data cannot stay in the Infinity Cache
+        # (N=256: 0.54 GB per apply, twice the cache), beside the stream of
+        # the same bytes at that size
+        n2 = 256
+        log(f"HBM-resident apply at N={n2}")
+        k2_, r2_ = fields.iid_lognormal(n2, sigma=1.0, clip=3.0, seed=a.seed)
+        op2 = HeatOperator(n2, dt, k2_.ravel(), r2_.ravel(), kernel="fast", scatter="colored", source="const",
+                           source_value=1.0)
+        del k2_, r2_
+        op2.upload("p", np.random.default_rng(a.seed).standard_normal(op2.nn))
+        pick2 = op2.autotune(20)
+        op2.time_matvec("p", "Ap", 5)
+        disp2, _ = op2.time_matvec("p", "Ap", 100)
+        op2.time_stream("p", "Ap", 10)
+        st2 = op2.time_stream("p", "Ap", 100)
+        bytes2 = 8 * (2 * op2.nn + 2 * op2.ne)
+        result["hbm_resident"] = {
+            "n": n2, "nodes": op2.nn, "elements": op2.ne, "algorithmic_bytes_per_apply": bytes2,
+            "schedule": f"{pick2[0]} planes={pick2[1]} tile={pick2[2][0]}x{pick2[2][1]}",
+            "us_per_launch": round(disp2 * 1e3, 2), "gdof_s": round(op2.nn / (disp2 * 1e-3) / 1e9, 2),
+            "frac": round(bytes2 / (disp2 * 1e-3) / 8e12, 4),
+            "stream_us": round(st2 * 1e3, 2), "stream_frac": round(bytes2 / (st2 * 1e-3) / 8e12, 4),
+            "apply_vs_stream": round(st2 / disp2, 4)}
+        log(f"N={n2}: {disp2 * 1e3:.1f} us per launch (frac {result['hbm_resident']['frac']}), stream "
+            f"{st2 * 1e3:.1f} us (frac {result['hbm_resident']['stream_frac']})")
+        op2.cleanup()
 
     if not a.no_solve:
         # time to relative residual 1e-8 for the first time step (T^0 = 0, b = dt F)
