@@ -158,6 +158,18 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
                          int max_inner, double tol, double tol_inner,
                          int* outer, int* inner_iters);
 
+/* Inner solver of the fused-order refined solve (new; no reference
+ * counterpart).  HEAT_INNER_ONE_LAUNCH (default): preconditioned CG in the
+ * Chronopoulos-Gear arrangement, one launch per iteration (both inner
+ * products of an iteration reduced together; the same iterates as the
+ * standard PCG in exact arithmetic, other roundings).  HEAT_INNER_STANDARD:
+ * the standard PCG in two kernels per iteration, the order of the batched
+ * refined solves (heat_batch_step_mixed).  A solve with a PCG-state upset
+ * armed always runs the standard iteration. */
+#define HEAT_INNER_ONE_LAUNCH 0
+#define HEAT_INNER_STANDARD   1
+int heat_set_inner_solver(heat_op* op, int mode);
+
 /* Level R: the next apply with allow_fi != 0 flips bit `bit` of element
  * `target`'s accumulator ye[0] before it is scattered, once
  * (src/heat_solver_het.cu:369-374, 902, 1137). */

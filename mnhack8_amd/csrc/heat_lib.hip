@@ -22,6 +22,7 @@
 #include "heat_pcg_fused.h"
 #include "heat_mrhs.h"
 #include "heat_mrhs_solve.h"
+#include "heat_cg1.h"
 
 using namespace heat;
 
@@ -112,6 +113,15 @@ struct heat_op {
     double* fsp = nullptr;        // super-block partials, 5 x nsb
     double* ftp = nullptr;        // fused-apply tile partials, 4 x ntiles
     int nsb = 0, fnby = 0, fnbz = 0, fnch = 0, fntiles = 0;
+    // the one-launch inner iteration of the refined solve (heat_cg1.h)
+    int inner_mode = HEAT_INNER_ONE_LAUNCH;
+    float *c1_r = nullptr, *c1_s[2] = {nullptr, nullptr}, *c1_w[2] = {nullptr, nullptr};
+    double* c1_tp = nullptr;               // 2 x C1ROWS x tiles
+    float* c1_dinv = nullptr;              // FP32 reciprocal of the diagonal
+    int c1_nby = 0, c1_nbz = 0, c1_nch = 0, c1_ntiles = 0;
+    hipGraph_t c1_graph[2] = {nullptr, nullptr};      // a chunk by the parity of its first step
+    hipGraphExec_t c1_exec[2] = {nullptr, nullptr};
+    int c1_key[2] = {0, 0};
     // captured PCG chunk
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -630,6 +640,15 @@ int run_until_done(heat_op* op, int max_iter, LaunchFn launch, const int* dflag 
 void free_all(heat_op* op) {
     if (op->gexec) (void)hipGraphExecDestroy(op->gexec);
     if (op->graph) (void)hipGraphDestroy(op->graph);
+    for (int q = 0; q < 2; ++q) {
+        if (op->c1_exec[q]) (void)hipGraphExecDestroy(op->c1_exec[q]);
+        if (op->c1_graph[q]) (void)hipGraphDestroy(op->c1_graph[q]);
+        (void)hipFree(op->c1_s[q]);
+        (void)hipFree(op->c1_w[q]);
+    }
+    (void)hipFree(op->c1_r);
+    (void)hipFree(op->c1_tp);
+    (void)hipFree(op->c1_dinv);
     for (auto& e : op->ev) if (e) (void)hipEventDestroy(e);
     (void)hipFree(op->kcell); (void)hipFree(op->rhoc); (void)hipFree(op->cr); (void)hipFree(op->ck);
     (void)hipFree(op->cr32); (void)hipFree(op->ck32);
@@ -809,6 +828,112 @@ int ensure_mixed(heat_op* op) {
     CHECK_LAUNCH();
     k_convert<float, double><<<blocks(g.nn, 256), 256, 0, s>>>(op->d32, op->vec[HEAT_VEC_DIAG], g.nn);
     CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+// Buffers and tile geometry of the one-launch inner iteration (made on first use).
+int ensure_cg1(heat_op* op) {
+    int rc = ensure_mixed(op);
+    if (rc) return rc;
+    if (op->c1_r) return HEAT_OK;
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    const size_t nb4 = sizeof(float) * (size_t)g.nn;
+    const int m = g.n - 1 > 0 ? g.n - 1 : 1;
+    op->c1_nby = (m + C1TY - 1) / C1TY;
+    op->c1_nbz = (m + C1TZ - 1) / C1TZ;
+    op->c1_nch = cg1_chunks(g.n, op->c1_nby * op->c1_nbz);
+    op->c1_ntiles = op->c1_nby * op->c1_nbz * op->c1_nch;
+    HIP_TRY(hipMalloc(&op->c1_r, nb4));
+    for (int q = 0; q < 2; ++q) {
+        HIP_TRY(hipMalloc(&op->c1_s[q], nb4));
+        HIP_TRY(hipMalloc(&op->c1_w[q], nb4));
+    }
+    HIP_TRY(hipMalloc(&op->c1_tp, sizeof(double) * 2 * C1ROWS * (size_t)op->c1_ntiles));
+    HIP_TRY(hipMalloc(&op->c1_dinv, nb4));
+    k_recip<<<blocks(g.nn, NT), NT, 0, s>>>(op->c1_dinv, op->d32, g.nn);
+    CHECK_LAUNCH();
+    // the correction is added to x at every node: zero where the walk never
+    // writes it (the boundary)
+    k_fill<float><<<blocks(g.nn, NT), NT, 0, s>>>(op->e32, g.nn, 0.0f);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+Cg1View make_cg1_view(heat_op* op, const double* b, const double* Ax) {
+    Cg1View v;
+    v.e = op->e32;
+    v.p = op->p32;
+    v.r[0] = op->r32;
+    v.r[1] = op->c1_r;
+    for (int q = 0; q < 2; ++q) {
+        v.s[q] = op->c1_s[q];
+        v.w[q] = op->c1_w[q];
+        v.tp[q] = op->c1_tp + (size_t)q * C1ROWS * op->c1_ntiles;
+    }
+    v.d = op->c1_dinv;
+    v.cr = op->cr32;
+    v.ck = op->ck32;
+    v.b = b;
+    v.Ax = Ax;
+    v.wA = op->vec[HEAT_VEC_WA];
+    v.g = op->g;
+    v.ntiles = op->c1_ntiles;
+    v.nby = op->c1_nby;
+    v.nbz = op->c1_nbz;
+    v.nch = op->c1_nch;
+    v.pre = op->c1_ntiles > FUSED_INBLOCK_MAX;
+    v.fs = op->fst;
+    v.st = op->st;
+    v.nwA = op->nwA;
+    v.hflag = op->hdone_dev;
+    return v;
+}
+
+// one step of parity par (with its one-block fold for many tiles)
+int cg1_launch_step(heat_op* op, const Cg1View& v, int par, bool first_of_pass) {
+    hipStream_t s = op->stream;
+    const bool detect = op->cfg.detect != 0;
+    const unsigned grid = (unsigned)(((v.ntiles + NXCD - 1) / NXCD) * NXCD);
+    if (v.pre) {
+        if (detect) k_cg1_fin<true><<<1, NT, 0, s>>>(v, par, first_of_pass ? 1 : 0);
+        else k_cg1_fin<false><<<1, NT, 0, s>>>(v, par, first_of_pass ? 1 : 0);
+        CHECK_LAUNCH();
+    }
+    if (detect) k_cg1_step<true><<<grid, C1NT, 0, s>>>(v, par);
+    else k_cg1_step<false><<<grid, C1NT, 0, s>>>(v, par);
+    CHECK_LAUNCH();
+    return HEAT_OK;
+}
+
+// pcg_chunk steps starting with parity par0 (the pass's later steps: never the
+// first of the pass), captured once per starting parity as a hipGraph
+int cg1_launch_chunk(heat_op* op, const Cg1View& v, int par0) {
+    auto body = [&]() -> int {
+        for (int i = 0; i < op->pcg_chunk; ++i) {
+            const int rc = cg1_launch_step(op, v, (par0 + i) & 1, false);
+            if (rc) return rc;
+        }
+        return HEAT_OK;
+    };
+    if (!op->use_graph) return body();
+    const int key = 1 + op->pcg_chunk * 4 + (op->cfg.detect ? 2 : 0) + (v.pre ? 1 : 0);
+    hipGraphExec_t& ex = op->c1_exec[par0];
+    hipGraph_t& gr = op->c1_graph[par0];
+    if (!ex || op->c1_key[par0] != key) {
+        if (ex) { (void)hipGraphExecDestroy(ex); ex = nullptr; }
+        if (gr) { (void)hipGraphDestroy(gr); gr = nullptr; }
+        HIP_TRY(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = body();
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(op->stream, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        gr = graph;
+        HIP_TRY(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        op->c1_key[par0] = key;
+    }
+    HIP_TRY(hipGraphLaunch(ex, op->stream));
     return HEAT_OK;
 }
 
@@ -1202,6 +1327,63 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
                                   op->pcg_fi.iter);
     CHECK_LAUNCH();
     const bool detect = op->cfg.detect != 0;
+    if (op->inner_mode == HEAT_INNER_ONE_LAUNCH && !op->pcg_fi.armed) {
+        // the inner solve in one launch per iteration (heat_cg1.h); an armed
+        // upset of the inner state runs the standard two-kernel iteration below
+        rc = ensure_cg1(op);
+        if (rc) return rc;
+        double* Ax = op->vec[HEAT_VEC_AP];
+        const Cg1View v = make_cg1_view(op, b, Ax);
+        const unsigned grid = (unsigned)(((v.ntiles + NXCD - 1) / NXCD) * NXCD);
+        const int saved_chunk = op->pcg_chunk;
+        op->pcg_chunk = std::min(saved_chunk, MIXED_CHUNK);
+        int nout = 0;
+        for (;;) {
+            rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
+            if (rc) break;
+            if (detect) k_cg1_start<true><<<grid, C1NT, 0, s>>>(v);
+            else k_cg1_start<false><<<grid, C1NT, 0, s>>>(v);
+            if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
+            // the first step makes the outer test: after a correction it goes
+            // alone and is read back, so a solve the correction finished stops
+            int launched = 0;
+            bool finished = false;
+            if (nout > 0) {
+                rc = cg1_launch_step(op, v, 0, true);
+                if (rc == HEAT_OK) rc = read_fused(op);
+                if (rc) break;
+                launched = 1;
+                finished = op->hfst->done != 0;
+            }
+            if (!finished) {
+                rc = run_until_done(op, max_inner + 1, [&]() {
+                    int r2 = HEAT_OK;
+                    if (launched == 0) {
+                        // the pass's first step folds the start's partials (8 rows)
+                        r2 = cg1_launch_step(op, v, 0, true);
+                        launched = 1;
+                        if (r2) return r2;
+                    }
+                    r2 = cg1_launch_chunk(op, v, launched & 1);
+                    launched += op->pcg_chunk;
+                    return r2;
+                }, &op->fst->done, op->hdone);
+                if (rc == HEAT_OK) rc = read_fused(op);
+                if (rc) break;
+            }
+            if (op->hfst->outer_done == 3) { rc = fail(HEAT_E_STATE, "inner iteration launched out of step"); break; }
+            if (op->hfst->outer_done || nout >= max_outer) break;
+            k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
+            if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
+            ++nout;
+        }
+        op->pcg_chunk = saved_chunk;
+        if (rc) return rc;
+        op->pcg_fi_fired = 0;
+        if (outer) *outer = nout;
+        if (inner_iters) *inner_iters = op->hfst->inner_total;
+        return HEAT_OK;
+    }
     int nout = 0;
     // Inner passes poll in chunks of at most MIXED_CHUNK iterations: every
     // pass ends with the rest of its last chunk and one more chunk of
@@ -1301,6 +1483,13 @@ int heat_time_mrhs(heat_op* op, const float* X, float* Y, int reps, double* avg_
     (void)hipEventDestroy(e1);
     *avg_ms = ms / reps;
     return rc;
+}
+
+int heat_set_inner_solver(heat_op* op, int mode) {
+    if (!op || (mode != HEAT_INNER_ONE_LAUNCH && mode != HEAT_INNER_STANDARD))
+        return fail(HEAT_E_ARG, "inner solver must be HEAT_INNER_ONE_LAUNCH or HEAT_INNER_STANDARD");
+    op->inner_mode = mode;
+    return HEAT_OK;
 }
 
 int heat_set_pcg_mode(heat_op* op, int mode) {

@@ -77,6 +77,12 @@ struct FusedState {
     // sums of the other kernel's partials, when a one-block finalize launch
     // forms them (large meshes) instead of every block of K1 / K2
     double red_sp[5], red_tp[4];
+    // the one-launch inner iteration (heat_cg1.h), slots by iteration parity:
+    // gamma = r.u and alpha of the step, the index of the next step, and the
+    // folded partials when a one-block launch folds them
+    double cg_gamma[2], cg_alpha[2];
+    double cg_red[2][8];
+    int cg_it[2];
 };
 
 enum { SEU_U = 0, SEU_R = 1, SEU_P = 2, SEU_Z = 3, SEU_AP = 4 };

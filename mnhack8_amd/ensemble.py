@@ -57,6 +57,8 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
 
     def solve_one(op):
         if refined:
+            # the batch's refined steps run the standard inner iteration
+            op.set_inner_solver("standard")
             return op.pcg_solve_mixed("b", "T", max_outer, max_iter, tol, tol_inner)[1]
         return op.pcg_solve("b", "T", max_iter, tol)
 

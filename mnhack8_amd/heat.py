@@ -102,6 +102,7 @@ SIGNATURES = {
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
     "heat_time_stream": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
     "heat_dot_static": (_I, [_P, _P, _I64, _P, _P]),
+    "heat_set_inner_solver": (_I, [_P, _I]),
 }
 
 _lib = None
@@ -224,6 +225,10 @@ class HeatOperator:
 
     def set_tile_shape(self, shape):
         _check(self._lib.heat_set_tile_shape(self.h, int(shape)))
+
+    def set_inner_solver(self, mode):
+        """Inner solve of the refined solve: "one_launch" (default) or "standard"."""
+        _check(self._lib.heat_set_inner_solver(self.h, {"one_launch": 0, "standard": 1}[mode]))
 
     def set_pcg_mode(self, mode):
         """"fused" (three launches per iteration) or "reference" (the reference's dot order)."""

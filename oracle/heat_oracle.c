@@ -987,6 +987,187 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
     return 0;
 }
 
+/* ------------------------------------------------------------------------- */
+/* The one-launch inner iteration of the refined solve (mnhack8_amd/csrc/     */
+/* heat_cg1.h): preconditioned CG in the Chronopoulos-Gear arrangement, its   */
+/* tiles and its summation order.                                            */
+/* ------------------------------------------------------------------------- */
+enum { C1_TY = 8, C1_TZ = 32, C1_XW = C1_TZ + 2, C1_XN = (C1_TZ + 2) * (C1_TY + 2),
+       C1_XR = (C1_XN + F_NT - 1) / F_NT };
+
+typedef struct { int m, nby, nbz, nch; long ntiles; } c1_geom;
+
+/* tiles of 8 x 32 interior nodes by chunks of the n - 1 interior planes */
+static c1_geom cg1_geom(const ho_grid* g) {
+    c1_geom q;
+    q.m = g->n - 1 > 0 ? g->n - 1 : 1;
+    q.nby = (q.m + C1_TY - 1) / C1_TY;
+    q.nbz = (q.m + C1_TZ - 1) / C1_TZ;
+    q.nch = fused_chunks(q.m, q.nby * q.nbz);
+    q.ntiles = (long)q.nby * q.nbz * q.nch;
+    return q;
+}
+
+/* A row of tile sums: the term A[i] * B[i] (or A[i] when B is NULL) over the
+ * tile's own interior nodes, per thread either in slot order (ORDER 0: the
+ * thread's nodes of the 34 x 10 halo tile, plane by plane, its two slots in
+ * order) or in stencil order (ORDER 1: the thread's node (y0 + tid / 32,
+ * z0 + tid % 32), plane by plane), then the 256-wide tree, then the fold of
+ * the tiles' partials. */
+typedef struct { int order; const double* a; const double* b; } c1_row;
+
+static void cg1_sums(const ho_grid* g, const c1_geom* q, const c1_row* rows, int nrows, double* part,
+                     double* out) {
+#pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
+    for (long t = 0; t < q->ntiles; ++t) {
+        const int bz = (int)(t % q->nbz), by = (int)((t / q->nbz) % q->nby), chunk = (int)(t / ((long)q->nbz * q->nby));
+        const int ix0 = 1 + (int)(((long long)chunk * q->m) / q->nch);
+        const int ix1 = 1 + (int)(((long long)(chunk + 1) * q->m) / q->nch);
+        const int y0 = 1 + by * C1_TY, z0 = 1 + bz * C1_TZ;
+        for (int k = 0; k < nrows; ++k) {
+            double s[F_NT];
+            for (int tid = 0; tid < F_NT; ++tid) {
+                double acc = 0.0;
+                for (int xp = ix0; xp < ix1; ++xp) {
+                    if (rows[k].order == 0) {
+                        for (int j = 0; j < C1_XR; ++j) {
+                            const int i = tid + j * F_NT;
+                            if (i >= C1_XN) continue;
+                            const int r = i / C1_XW, c = i % C1_XW;
+                            const int yy = y0 - 1 + r, zz = z0 - 1 + c;
+                            if (r < 1 || r > C1_TY || c < 1 || c > C1_TZ || yy > q->m || zz > q->m) continue;
+                            const long id = lnid(g, xp, yy, zz);
+                            acc += rows[k].b ? rows[k].a[id] * rows[k].b[id] : rows[k].a[id];
+                        }
+                    } else {
+                        const int yy = y0 + tid / C1_TZ, zz = z0 + tid % C1_TZ;
+                        if (yy > q->m || zz > q->m) continue;
+                        const long id = lnid(g, xp, yy, zz);
+                        acc += rows[k].b ? rows[k].a[id] * rows[k].b[id] : rows[k].a[id];
+                    }
+                }
+                s[tid] = acc;
+            }
+            part[(long)k * q->ntiles + t] = tree_256(s);
+        }
+    }
+    for (int k = 0; k < nrows; ++k) out[k] = fold_partials(part + (long)k * q->ntiles, q->ntiles);
+}
+
+/* The refined solve with the one-launch inner iteration: as
+ * ho_pcg_solve_mixed_fused (FP64 residual, correction, the same stopping
+ * rules), with the inner solve
+ *   start   r0 = (float)(b - A x), u0 = r0 * dinv (dinv = 1/d in FP32), w0 = A u0 (FP32 apply)
+ *   step i  beta = gamma_i / gamma_{i-1}, alpha = gamma_i / fma(-beta, gamma_i / alpha_{i-1}, delta_i)
+ *           (step 0: beta = 0, alpha = gamma_0 / delta_0), then in FP32 with alpha, beta rounded:
+ *           s = w + beta s, r -= alpha s, p = r_old dinv + beta p, e += alpha p, u = r dinv, w = A u,
+ *           gamma = r.u, delta = w.u, r.r in the tile order of cg1_sums;
+ * the D1 check of each inner apply is taken at the next step's head. */
+int ho_pcg_solve_mixed_cg1(ho_op* op, const double* b, double* x, int max_outer, int max_inner,
+                           double tol, double tol_inner, int* outer, int* inner_total) {
+    const ho_grid* g = &op->g;
+    const long n = g->nn;
+    const c1_geom q = cg1_geom(g);
+    float* f = (float*)malloc(sizeof(float) * 7 * n);
+    float *e = f, *r = f + n, *s = f + 2 * n, *p = f + 3 * n, *u = f + 4 * n, *w = f + 5 * n, *dinv = f + 6 * n;
+    double* dv = (double*)malloc(sizeof(double) * 6 * n);
+    double *ax = dv, *r64 = dv + n, *dr = dv + 2 * n, *du = dv + 3 * n, *dw = dv + 4 * n, *bcp = dv + 5 * n;
+    double* part = (double*)malloc(sizeof(double) * 8 * (size_t)q.ntiles);
+    unsigned char* in = (unsigned char*)malloc((size_t)n);
+    for (long i = 0; i < n; ++i) {
+        dinv[i] = 1.0f / (float)op->diag[i];
+        in[i] = !node_is_boundary(g, i);
+        e[i] = 0.0f;
+    }
+    int nout = 0, total = 0;
+    op->sfi_armed = 0;
+    op->sfi_fired = 0;
+    for (;;) {
+        apply_projected(op, x, ax);
+        for (long i = 0; i < n; ++i) {
+            if (in[i]) {
+                r64[i] = fma(-1.0, ax[i], b[i]);
+                r[i] = (float)r64[i];
+                u[i] = r[i] * dinv[i];
+                e[i] = 0.0f;
+                bcp[i] = b[i];
+            } else {
+                r64[i] = 0.0; r[i] = 0.0f; u[i] = 0.0f; bcp[i] = 0.0;
+            }
+            dr[i] = r[i]; du[i] = u[i];
+        }
+        ho_apply_operator_f32(op, u, w, op->dt, 1);
+        for (long i = 0; i < n; ++i) dw[i] = w[i];
+        double t[8];
+        const c1_row start_rows[8] = {{0, dr, du}, {1, dw, du}, {0, dr, dr}, {1, dw, NULL},
+                                      {1, op->detect ? op->wA : du, du}, {1, du, du}, {0, bcp, bcp}, {0, r64, r64}};
+        cg1_sums(g, &q, start_rows, 8, part, t);
+        double bn = 1.0, gamma_prev = 0.0, alpha_prev = 0.0;
+        int outer_done = 0;
+        for (int it = 0;; ++it) {
+            if (op->detect) {
+                double rel = fabs(t[3] - t[4]) / (op->norm_wA * sqrt(t[5]) + 1e-300);
+                if (!isfinite(rel)) rel = 1e300;
+                op->abft_step_max = fmax(op->abft_step_max, rel);
+                op->abft_step_calls++;
+            }
+            double beta, den;
+            int stop;
+            if (it == 0) {
+                double bo = sqrt(t[6]);
+                if (bo < 1e-30) bo = 1.0;
+                const double rn = sqrt(t[7]);
+                outer_done = (rn / bo < tol) ? 1 : (isfinite(rn) ? 0 : 2);
+                bn = sqrt(t[2]);
+                if (bn < 1e-30) bn = 1.0;
+                beta = 0.0;
+                den = t[1];
+                stop = outer_done != 0 || max_inner <= 0 || fabs(den) < 1e-30 * bn * bn;
+            } else {
+                const int conv = sqrt(t[2]) / bn < tol_inner;
+                beta = t[0] / gamma_prev;
+                den = fma(-beta, t[0] / alpha_prev, t[1]);
+                stop = conv || it >= max_inner || fabs(den) < 1e-30 * bn * bn;
+            }
+            if (stop) break;
+            const double alpha = t[0] / den;
+            gamma_prev = t[0];
+            alpha_prev = alpha;
+            const int first = it == 0;
+            const float af = (float)alpha, bf = (float)beta;
+            for (long i = 0; i < n; ++i) {
+                if (!in[i]) continue;
+                const float sn = first ? w[i] : fmaf(bf, s[i], w[i]);
+                const float rn = fmaf(-af, sn, r[i]);
+                const float uo = r[i] * dinv[i];
+                const float pn = first ? uo : fmaf(bf, p[i], uo);
+                e[i] = fmaf(af, pn, e[i]);
+                p[i] = pn;
+                s[i] = sn;
+                r[i] = rn;
+                u[i] = rn * dinv[i];
+                dr[i] = r[i]; du[i] = u[i];
+            }
+            ho_apply_operator_f32(op, u, w, op->dt, 1);
+            for (long i = 0; i < n; ++i) dw[i] = w[i];
+            const c1_row step_rows[6] = {{0, dr, du}, {1, dw, du}, {0, dr, dr}, {1, dw, NULL},
+                                         {1, op->detect ? op->wA : du, du}, {1, du, du}};
+            cg1_sums(g, &q, step_rows, op->detect ? 6 : 3, part, t);
+            ++total;
+        }
+        if (outer_done || nout >= max_outer) break;
+        for (long i = 0; i < n; ++i) x[i] += (double)e[i];
+        ++nout;
+    }
+    if (outer) *outer = nout;
+    if (inner_total) *inner_total = total;
+    free(f);
+    free(dv);
+    free(part);
+    free(in);
+    return 0;
+}
+
 /* the solve the command line runs, in the order chosen by --pcg-order */
 static int solve_cli(ho_op* op, const double* b, double* x, int max_iter, double tol, FILE* diag_log) {
     /* the GPU library runs the fused iteration on the colour-ordered fast path only */

@@ -87,6 +87,8 @@ void   ho_fused_tile_sums(ho_op* op, const double* p, const double* Ap, const do
 /* FP32-refined solve (FP64 outer residual, FP32 inner Jacobi-PCG) in that order. */
 int    ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_outer, int max_inner,
                                 double tol, double tol_inner, int* outer, int* inner_total);
+int    ho_pcg_solve_mixed_cg1  (ho_op* op, const double* b, double* x, int max_outer, int max_inner,
+                                double tol, double tol_inner, int* outer, int* inner_total);
 
 /* initial conditions, src/heat_solver_het.cu:556-584.  mode 1 gaussian, 2 sine mode at t */
 void   ho_initial_condition(ho_op* op, double* v, int mode, double t, double alpha);

@@ -36,6 +36,7 @@ _SIG = {
     "ho_pcg_solve": (_I, [_P, _P, _P, _I, _D, _P]),
     "ho_pcg_solve_fused": (_I, [_P, _P, _P, _I, _D, _P]),
     "ho_pcg_solve_mixed_fused": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
+    "ho_pcg_solve_mixed_cg1": (_I, [_P, _P, _P, _I, _I, _D, _D, _P, _P]),
     "ho_fused_sb_dot": (_D, [_P, _P, _P]),
     "ho_set_vector": (_I, [_P, _I, _P]),
     "ho_arm_pcg_fault": (None, [_P, _I, _L, _I, _I]),
@@ -164,6 +165,15 @@ class OracleOp:
         o, i = C.c_int(), C.c_int()
         lib().ho_pcg_solve_mixed_fused(self.h, _p(b), _p(x), max_outer, max_inner, tol, tol_inner,
                                        C.byref(o), C.byref(i))
+        return o.value, i.value, x
+
+    def pcg_mixed_cg1(self, b, x, max_outer=50, max_inner=500, tol=1e-8, tol_inner=1e-4):
+        """The refined solve with the GPU library's one-launch inner iteration."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x, dtype=np.float64, copy=True)
+        o, i = C.c_int(), C.c_int()
+        lib().ho_pcg_solve_mixed_cg1(self.h, _p(b), _p(x), max_outer, max_inner, tol, tol_inner,
+                                     C.byref(o), C.byref(i))
         return o.value, i.value, x
 
     def arm_pcg_fault(self, vec, node, bit, iteration):

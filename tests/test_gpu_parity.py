@@ -334,23 +334,55 @@ def test_fp32_operator_bitwise(dt_eff):
     assert relerr(y, y64) < 1e-5
 
 
-@pytest.mark.parametrize("n", [7, 24, 33])
-def test_mixed_fused_bitwise(n):
+@pytest.mark.parametrize("inner", ["standard", "one_launch"])
+@pytest.mark.parametrize("n", [2, 7, 24, 33, 40])
+def test_mixed_fused_bitwise(n, inner):
     """FP32-refined solve (fused order) bit-identical to the oracle's
-    restatement: outer and inner counts exact, FP64 iterate identical."""
+    restatement of its inner iteration (the standard two-kernel PCG, or the
+    one-launch Chronopoulos-Gear arrangement with its tile order): outer and
+    inner counts exact, FP64 iterate identical, D1 record identical."""
     dt = 1e-4
-    print(f"[case] mixed fused n={n}", flush=True)
+    print(f"[case] mixed fused n={n} inner={inner}", flush=True)
     k, rhoc = make_field(n, seed=40 + n)
-    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
-    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0, detect=True)
+    op.set_inner_solver(inner)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0, detect=True)
+    solve = orc.pcg_mixed_fused if inner == "standard" else orc.pcg_mixed_cg1
     b = orc.build_rhs(np.zeros(op.nn))
     for tol_inner in (1e-4, 1e-2):
         op.upload("b", b)
         op.upload("T", np.zeros(op.nn))
-        outer, inner = op.pcg_solve_mixed("b", "T", 50, 500, 1e-10, tol_inner)
-        oo, io, xo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, tol_inner)
-        print(f"  tol_inner={tol_inner}: outer {outer} inner {inner} (oracle {oo} {io})", flush=True)
-        assert (outer, inner) == (oo, io)
+        op.abft_take()
+        orc.abft_take()
+        outer, inner_its = op.pcg_solve_mixed("b", "T", 50, 500, 1e-10, tol_inner)
+        oo, io, xo = solve(b, np.zeros(op.nn), 50, 500, 1e-10, tol_inner)
+        print(f"  tol_inner={tol_inner}: outer {outer} inner {inner_its} (oracle {oo} {io})", flush=True)
+        assert (outer, inner_its) == (oo, io)
+        assert np.array_equal(op.download("T"), xo)
+        d1, calls = op.abft_take()
+        d1o, callso = orc.abft_take()
+        assert calls == callso and d1 == d1o
+
+
+@pytest.mark.parametrize("chunk,graph", [(1, True), (3, True), (4, False), (8, True)])
+def test_mixed_one_launch_chunks(chunk, graph):
+    """The one-launch inner iteration launched in chunks of 1, 3, 4 and 8
+    steps (graphs captured per starting parity, or eager): the same bits as
+    the oracle, solve after solve on one handle."""
+    n, dt = 19, 1e-4
+    print(f"[case] one-launch inner chunk={chunk} graph={graph}", flush=True)
+    k, rhoc = make_field(n, seed=61, sigma=2.0)
+    op = heat().HeatOperator(n, dt, k, rhoc, source="const", source_value=1.0)
+    op.set_tuning(chunk, graph)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", source="const", source_value=1.0)
+    b = orc.build_rhs(np.zeros(op.nn))
+    for rep in range(3):
+        x0 = np.random.default_rng(rep).standard_normal(op.nn) * (b != 0)
+        op.upload("b", b)
+        op.upload("T", x0)
+        got = op.pcg_solve_mixed("b", "T", 50, 500, 1e-10, 1e-4)
+        oo, io, xo = orc.pcg_mixed_cg1(b, x0, 50, 500, 1e-10, 1e-4)
+        assert got == (oo, io), (rep, got, (oo, io))
         assert np.array_equal(op.download("T"), xo)
 
 
@@ -513,8 +545,10 @@ def test_upset_and_clean_solves_share_a_handle():
                     op.arm_pcg_fault("p", node, bit, site)
                     orc.arm_pcg_fault("p", node, bit, site)
                 if refined:
+                    # an armed upset runs the standard inner iteration, otherwise the one-launch one
                     got = op.pcg_solve_mixed("b", "T", 8, 60, 1e-10, 1e-4)
-                    want = orc.pcg_mixed_fused(b, np.zeros(op.nn), 8, 60, 1e-10, 1e-4)
+                    solve = orc.pcg_mixed_fused if armed else orc.pcg_mixed_cg1
+                    want = solve(b, np.zeros(op.nn), 8, 60, 1e-10, 1e-4)
                     assert tuple(got) == tuple(want[:2])
                     xo = want[2]
                 else:
@@ -863,7 +897,7 @@ def test_fused_pcg_large_mesh_finalized_sums():
     # FP32-refined at the same size: outer/inner counts and iterate bit-identical
     op.upload("T", np.zeros(op.nn))
     outer, inner = op.pcg_solve_mixed("b", "T", 10, 200, 1e-8, 1e-3)
-    oo, io, xmo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 10, 200, 1e-8, 1e-3)
+    oo, io, xmo = orc.pcg_mixed_cg1(b, np.zeros(op.nn), 10, 200, 1e-8, 1e-3)
     assert (outer, inner) == (oo, io)
     assert np.array_equal(op.download("T"), xmo)
 
@@ -937,7 +971,7 @@ def test_full_size_refined_workload():
     op.upload("b", b0)
     op.upload("T", np.zeros(op.nn))
     got = op.pcg_solve_mixed("b", "T", 50, 500, tol, tol_in)
-    oo, io, xo = orc.pcg_mixed_fused(b0, np.zeros(op.nn), 50, 500, tol, tol_in)
+    oo, io, xo = orc.pcg_mixed_cg1(b0, np.zeros(op.nn), 50, 500, tol, tol_in)
     assert got == (oo, io)
     assert np.array_equal(op.download("T"), xo)
     thr = op.abft_threshold()

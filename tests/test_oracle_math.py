@@ -401,6 +401,25 @@ def test_fused_mixed_is_scale_invariant():
     assert np.linalg.norm(r) / np.linalg.norm(s * b) < 1e-10
 
 
+@pytest.mark.parametrize("n", [2, 5, 11, 20])
+def test_one_launch_inner_iteration(n):
+    """The refined solve with the one-launch inner iteration (Chronopoulos-
+    Gear arrangement): the same Krylov iterates as the standard PCG in exact
+    arithmetic, so the same outer and inner counts here, the FP64 tolerance in
+    the true residual, iterates within rounding of the standard solve's, and
+    exact scaling by 2^-40 (every test in it is relative)."""
+    op, b = lognormal_problem(n)
+    o1, i1, x1 = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    o2, i2, x2 = op.pcg_mixed_cg1(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert (o1, i1) == (o2, i2)
+    r = b - op.apply(x2, op.dt)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10
+    assert np.linalg.norm(x2 - x1) <= 1e-9 * np.linalg.norm(x1)
+    s = 2.0 ** -40
+    o3, i3, x3 = op.pcg_mixed_cg1(s * b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert (o3, i3) == (o2, i2) and np.array_equal(x3, s * x2)
+
+
 def fused_solve_with_fault(op, b, vec, node, bit, it, tol=1e-12):
     op.arm_pcg_fault(vec, node, bit, it)
     out = op.pcg_fused(b, np.zeros(op.nn), 500, tol)
