@@ -165,7 +165,8 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
  * standard PCG in exact arithmetic, other roundings).  HEAT_INNER_STANDARD:
  * the standard PCG in two kernels per iteration, the order of the batched
  * refined solves (heat_batch_step_mixed).  A solve with a PCG-state upset
- * armed always runs the standard iteration. */
+ * armed, on a mesh above N = 644 or with max_inner above 2^20 - 3 runs the
+ * standard iteration. */
 #define HEAT_INNER_ONE_LAUNCH 0
 #define HEAT_INNER_STANDARD   1
 int heat_set_inner_solver(heat_op* op, int mode);

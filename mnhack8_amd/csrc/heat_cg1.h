@@ -49,6 +49,17 @@ __host__ __device__ inline int cg1_chunks(int n, int ncol) {
     return fused_chunks(n - 1 > 0 ? n - 1 : 1, ncol);
 }
 
+// The pinned status word every step's head stores: the pass (mod 1024), the
+// step of the pass and its status (0 running, 1 the inner solve ended, 2 the
+// outer test ended the solve, 3 a launch out of step).
+constexpr int CG1_MAX_STEP = (1 << 20) - 1;
+__host__ __device__ constexpr int cg1_flag(int pass, int step, int status) {
+    return ((pass & 1023) << 22) | ((step & CG1_MAX_STEP) << 2) | status;
+}
+__host__ __device__ constexpr int cg1_flag_pass(int f) { return (f >> 22) & 1023; }
+__host__ __device__ constexpr int cg1_flag_step(int f) { return (f >> 2) & CG1_MAX_STEP; }
+__host__ __device__ constexpr int cg1_flag_status(int f) { return f & 3; }
+
 struct Cg1View {
     float *e, *p;                 // iterate (the correction) and direction
     float *r[2], *s[2], *w[2];    // by iteration parity
@@ -62,7 +73,7 @@ struct Cg1View {
     FusedState* fs;
     DevState* st;
     double nwA;
-    int* hflag;                   // pinned host word: done after each launch's decision (or null)
+    int* hflag;                   // pinned host word: cg1_flag at each step's head (or null)
 };
 
 // Partial rows: 0 r.u, 1 w.u, 2 r.r of the launch's outputs; with the
@@ -70,16 +81,6 @@ struct Cg1View {
 // launch adds 6 b.b and 7 r64.r64 (the FP64 residual of the outer pass).
 template <bool DETECT, bool START>
 constexpr int cg1_rows() { return START ? 8 : (DETECT ? 6 : 3); }
-
-template <int K>
-__device__ __forceinline__ void cg1_fold(const Cg1View& v, int par, double (*sd)[NT], double o[K]) {
-    if (v.pre) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) o[k] = v.fs->cg_red[par][k];
-    } else {
-        reduce_partials<K>(sd, v.tp[par], v.ntiles, v.ntiles, o);
-    }
-}
 
 // One-block fold of the partials of parity PAR into fs->cg_red[par] (many tiles).
 template <bool DETECT>
@@ -98,23 +99,39 @@ __global__ __launch_bounds__(NT) void k_cg1_fin(Cg1View v, int par, int start) {
 
 // Head of step i (parity par): the tests in the order of the standard solve
 // (outer residual at i = 0, then convergence and the iteration limit), D1
-// of the apply that formed w_i, then alpha_i and beta_i.  False: the solve is
-// done (the same decision in every block).
-template <bool DETECT>
-__device__ __forceinline__ bool cg1_head(const Cg1View& v, int par, double (*sd)[NT], double& alpha, double& beta,
-                                         bool& first) {
+// of the apply that formed w_i, then alpha_i and beta_i.  x: this thread's
+// first four tile partials of the first KX rows (partials_load4), loaded
+// before the head so that their round trip overlaps the walk's first loads;
+// for many tiles (pre) k_cg1_fin has folded them into fs->cg_red.  False:
+// the solve is done (the same decision in every block).
+template <bool DETECT, int KX>
+__device__ __forceinline__ bool cg1_head(const Cg1View& v, int par, double (*sd)[NT], const double (&x)[KX][4],
+                                         double& alpha, double& beta, bool& first) {
     FusedState* fs = v.fs;
     const bool b0 = blockIdx.x == 0 && threadIdx.x == 0;
     if (fs->done) {
-        if (b0 && v.hflag) __hip_atomic_store(v.hflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (b0 && v.hflag)
+            __hip_atomic_store(v.hflag, cg1_flag(fs->cg_pass, fs->iter, fs->cg_status), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         return false;
     }
     const int it = fs->cg_it[par];
     double o[8];
     first = (it == 0);
-    if (first && fs->check_outer) cg1_fold<8>(v, par, sd, o);
-    else if (DETECT) cg1_fold<6>(v, par, sd, o);
-    else cg1_fold<3>(v, par, sd, o);
+    constexpr int KS = DETECT ? 6 : 3;
+    const bool need8 = first && fs->check_outer;
+    const double* part = v.tp[par];
+    const int nt = v.ntiles;
+    if (v.pre) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = fs->cg_red[par][k];
+    } else if (need8) {
+        if constexpr (KX >= 8) partials_fold<8, KX>(sd, x, part, nt, nt, o);
+        else reduce_partials<8>(sd, part, nt, nt, o);     // a first step launched without the 8-row load
+    } else {
+        if constexpr (KX >= KS) partials_fold<KS, KX>(sd, x, part, nt, nt, o);
+        else reduce_partials<KS>(sd, part, nt, nt, o);
+    }
     if (DETECT && b0) abft_record(v.st, o[3], o[4], o[5], v.nwA);
     bool stop = false;
     double den = 0.0;
@@ -144,11 +161,13 @@ __device__ __forceinline__ bool cg1_head(const Cg1View& v, int par, double (*sd)
         stop = conv || it >= fs->max_iter || fabs(den) < 1e-30 * bn * bn;
     }
     alpha = stop ? 0.0 : o[0] / den;
+    const int status = !stop ? 0 : (outer == 3 ? 3 : (outer != 0 ? 2 : 1));
     if (b0) {
         if (first) fs->outer_done = outer;
         if (outer == 3) fs->outer_done = 3;
         if (stop) {
             fs->done = 1;
+            fs->cg_status = status;
         } else {
             fs->cg_gamma[par] = o[0];
             fs->cg_alpha[par] = alpha;
@@ -156,7 +175,8 @@ __device__ __forceinline__ bool cg1_head(const Cg1View& v, int par, double (*sd)
             fs->iter = it + 1;
             fs->inner_total += 1;
         }
-        if (v.hflag) __hip_atomic_store(v.hflag, stop ? 1 : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (v.hflag)
+            __hip_atomic_store(v.hflag, cg1_flag(fs->cg_pass, it, status), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     return !stop;
 }
@@ -192,17 +212,24 @@ __device__ __forceinline__ void c1_stf(float v, __amdgpu_buffer_rsrc_t r, unsign
 // The walk of tile b.  START: the pass's first launch (u0, w0 and their
 // sums from b and A x; par = 1, outputs in slot 0).  Otherwise step i with
 // par = i & 1: inputs r[par], w[par], s[par ^ 1], outputs r[par ^ 1],
-// s[par], w[par ^ 1], partials tp[par ^ 1].  Vector updates are FP32 with
-// alpha and beta rounded to FP32 once; u = r * (1/d) with the FP32
-// reciprocal diagonal dinv.  Each plane's values are loaded two planes ahead.
-template <bool DETECT, bool START>
-__device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, double alpha, double beta, bool first,
-                                         float (*us)[C1XN], float (*er)[C1EN], float (*ek)[C1EN],
-                                         double (*sd)[NT]) {
+// s[par], w[par ^ 1], partials tp[par ^ 1]; FOLD8: the pass's first step,
+// which folds the start launch's eight partial rows.  Vector updates are
+// FP32 with alpha and beta rounded to FP32 once; u = r * (1/d) with the FP32
+// reciprocal diagonal dinv.
+//
+// Every load and store of the walk is issued on every path: one that falls
+// off the tile's planes or the mesh gets an out-of-range offset instead of a
+// branch (a load reads 0, a store is dropped), and the walk always takes an
+// even number of plane steps (a last step past the chunk stores nothing and
+// adds nothing).  So the memory-counter waits are the same on every path,
+// and each plane's loads, issued two planes ahead (one in the start
+// launch), are waited for only when that plane is formed.
+template <bool DETECT, bool START, bool FOLD8>
+__device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float (*us)[C1XN],
+                                         float (*er)[C1EN], float (*ek)[C1EN], double (*sd)[NT]) {
     const int out = par ^ 1;
     const Grid g = v.g;
     const int n = g.n, S = g.sx, nn1 = g.nn1, nsq = n * n, m = n - 1;
-    const int PB = S * (int)sizeof(float);                 // bytes of an FP32 node plane
     const int nby = v.nby, nbz = v.nbz;
     const int bz = b % nbz, by = (b / nbz) % nby, chunk = b / (nbz * nby);
     const int ix0 = 1 + (int)(((long long)chunk * m) / v.nch);
@@ -214,31 +241,36 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, doubl
     const bool node_in = iy <= m && iz <= m;
     const int py = iy & 1, pz = iz & 1;
     const int sy = py ? -1 : 1, sz = pz ? -1 : 1;
-    // byte offsets of this thread's slots: the u tile (interior nodes, else
-    // past the end), its own nodes, the element tile, and its stencil node
-    unsigned xo[C1XR], xw[C1XR], xd[C1XR];
-    bool xown[C1XR];
+    // the head's tile partials, requested first (round trip shared with the walk's first loads)
+    constexpr int KX = START ? 1 : (FOLD8 ? 8 : (DETECT ? 6 : 3));
+    double xpart[KX][4];
+    if constexpr (!START) partials_load4<KX>(v.tp[par], v.pre ? 0 : v.ntiles, v.ntiles, xpart);
+    // in-plane node offsets (elements) of this thread's slots of the u tile,
+    // whether each is an interior node and one of this tile's own, and the
+    // element tile's; every array is one buffer resource, and a slot's byte
+    // offset is formed once per plane (C1_OOB off the interior)
+    int xq[C1XR];
+    bool xin[C1XR], xown[C1XR];
 #pragma unroll
     for (int k = 0; k < C1XR; ++k) {
         const int i = tid + k * C1NT;
         const int r = i / C1XW, c = i - (i / C1XW) * C1XW;
         const int yy = y0 - 1 + r, zz = z0 - 1 + c;
-        const bool in = i < C1XN && yy >= 1 && yy <= m && zz >= 1 && zz <= m;
-        xown[k] = in && r >= 1 && r <= C1TY && c >= 1 && c <= C1TZ;
-        xo[k] = in ? (unsigned)((yy * nn1 + zz) * 4) : C1_OOB;
-        xw[k] = xown[k] ? xo[k] : C1_OOB;
-        xd[k] = in ? (unsigned)((yy * nn1 + zz) * 8) : C1_OOB;
+        xin[k] = i < C1XN && yy >= 1 && yy <= m && zz >= 1 && zz <= m;
+        xown[k] = xin[k] && r >= 1 && r <= C1TY && c >= 1 && c <= C1TZ;
+        xq[k] = yy * nn1 + zz;
     }
-    unsigned eo[C1ER];
+    int eq[C1ER];
+    bool eok[C1ER];
 #pragma unroll
     for (int k = 0; k < C1ER; ++k) {
         const int i = tid + k * C1NT;
         const int r = i / C1EW, c = i - (i / C1EW) * C1EW;
         const int ey = y0 - 1 + r, ez = z0 - 1 + c;
-        eo[k] = (i < C1EN && ey < n && ez < n) ? (unsigned)((ey * n + ez) * 4) : C1_OOB;
+        eok[k] = i < C1EN && ey < n && ez < n;
+        eq[k] = ey * n + ez;
     }
-    const unsigned wo = node_in ? (unsigned)((iy * nn1 + iz) * 4) : C1_OOB;
-    const unsigned wod = node_in ? (unsigned)((iy * nn1 + iz) * 8) : C1_OOB;
+    const int wq = iy * nn1 + iz;
     int eidx[2][2], eoff[2][2];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
@@ -256,90 +288,100 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, doubl
         float r[C1XR], w[C1XR], s[C1XR], d[C1XR], e[C1XR], p[C1XR];
         double bb[C1XR], ax[C1XR];
         float cr[C1ER], ck[C1ER];
+        double wa;                                      // D1 weight at the own node of the plane behind
     };
-    // plane P (0..n) of an FP32 node array; planes off the interior (0 and n)
-    // read as zeros: a zero-size resource (the walk never stores there)
-    auto plane = [&](const float* a, int P) __attribute__((always_inline)) {
-        return c1_rsrc(a + (size_t)P * S, (P >= 1 && P <= m) ? PB : 0);
-    };
+    // whole-array resources (the host keeps every byte offset below 2^31)
+    const int NB = g.nn * 4, EB = g.ne * 4;
+    const __amdgpu_buffer_rsrc_t R_d = c1_rsrc(v.d, NB), R_rin = c1_rsrc(v.r[par], NB), R_win = c1_rsrc(v.w[par], NB);
+    const __amdgpu_buffer_rsrc_t R_sin = c1_rsrc(v.s[out], NB), R_e = c1_rsrc(v.e, NB), R_p = c1_rsrc(v.p, NB);
+    const __amdgpu_buffer_rsrc_t R_rout = c1_rsrc(v.r[out], NB), R_sout = c1_rsrc(v.s[par], NB);
+    const __amdgpu_buffer_rsrc_t R_wout = c1_rsrc(v.w[out], NB);
+    const __amdgpu_buffer_rsrc_t R_cr = c1_rsrc(v.cr, EB), R_ck = c1_rsrc(v.ck, EB);
+    const __amdgpu_buffer_rsrc_t R_b = c1_rsrc(v.b, 2 * NB), R_ax = c1_rsrc(v.Ax, 2 * NB);
+    const __amdgpu_buffer_rsrc_t R_wA = c1_rsrc(v.wA, 2 * NB);
+    // plane P's loads: u-tile slots (planes 0 and n, and planes past the
+    // walk, read as zeros), the own nodes' e and p, element plane P - 1 and,
+    // with the detectors, the D1 weight of plane P - 1's own node
     auto fetch = [&](int P, Stage& St) __attribute__((always_inline)) {
-        const __amdgpu_buffer_rsrc_t rd = plane(v.d, P);
+        const bool pin = P >= 1 && P <= m && P <= ix1;
+        const bool pown = P >= ix0 && P < ix1;
+        const int pq = P * S;
 #pragma unroll
-        for (int k = 0; k < C1XR; ++k) St.d[k] = c1_ldf(rd, xo[k]);
-        if (START) {
-            const int dbytes = (P >= 1 && P <= m) ? 2 * PB : 0;
-            const __amdgpu_buffer_rsrc_t rb = c1_rsrc(v.b + (size_t)P * S, dbytes);
-            const __amdgpu_buffer_rsrc_t ra = c1_rsrc(v.Ax + (size_t)P * S, dbytes);
-#pragma unroll
-            for (int k = 0; k < C1XR; ++k) {
-                St.bb[k] = c1_ldd(rb, xd[k]);
-                St.ax[k] = c1_ldd(ra, xd[k]);
-            }
-        } else {
-            const __amdgpu_buffer_rsrc_t rr = plane(v.r[par], P), rw = plane(v.w[par], P), rs = plane(v.s[out], P);
-            const __amdgpu_buffer_rsrc_t re = plane(v.e, P), rp = plane(v.p, P);
-#pragma unroll
-            for (int k = 0; k < C1XR; ++k) {
-                St.r[k] = c1_ldf(rr, xo[k]);
-                St.w[k] = c1_ldf(rw, xo[k]);
-                St.s[k] = c1_ldf(rs, xo[k]);
-                St.e[k] = c1_ldf(re, xw[k]);
-                St.p[k] = c1_ldf(rp, xw[k]);
+        for (int k = 0; k < C1XR; ++k) {
+            const unsigned o4 = (pin && xin[k]) ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
+            St.d[k] = c1_ldf(R_d, o4);
+            if (START) {
+                const unsigned o8 = (pin && xin[k]) ? (unsigned)(pq + xq[k]) * 8u : C1_OOB;
+                St.bb[k] = c1_ldd(R_b, o8);
+                St.ax[k] = c1_ldd(R_ax, o8);
+            } else {
+                const unsigned ow = (pown && xown[k]) ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
+                St.r[k] = c1_ldf(R_rin, o4);
+                St.w[k] = c1_ldf(R_win, o4);
+                St.s[k] = c1_ldf(R_sin, o4);
+                St.e[k] = c1_ldf(R_e, ow);
+                St.p[k] = c1_ldf(R_p, ow);
             }
         }
-        // element plane P - 1 (none below plane 1: a zero-size resource)
         const int ep = P - 1;
-        const bool epin = ep >= 0 && ep < n;
-        const __amdgpu_buffer_rsrc_t rcr = c1_rsrc(v.cr + (size_t)(epin ? ep : 0) * nsq, epin ? nsq * 4 : 0);
-        const __amdgpu_buffer_rsrc_t rck = c1_rsrc(v.ck + (size_t)(epin ? ep : 0) * nsq, epin ? nsq * 4 : 0);
+        const bool epin = ep >= 0 && ep < n && P <= ix1;
+        const int eqp = (epin ? ep : 0) * nsq;
 #pragma unroll
         for (int k = 0; k < C1ER; ++k) {
-            St.cr[k] = c1_ldf(rcr, eo[k]);
-            St.ck[k] = c1_ldf(rck, eo[k]);
+            const unsigned o = (epin && eok[k]) ? (unsigned)(eqp + eq[k]) * 4u : C1_OOB;
+            St.cr[k] = c1_ldf(R_cr, o);
+            St.ck[k] = c1_ldf(R_ck, o);
+        }
+        if (DETECT) {
+            const bool xact = ep >= ix0 && ep < ix1;
+            St.wa = c1_ldd(R_wA, (xact && node_in) ? (unsigned)(ep * S + wq) * 8u : C1_OOB);
         }
     };
+    // two stages two planes ahead; the start launch, whose stages hold the
+    // FP64 right-hand side and A x, keeps one stage one plane ahead
+    Stage SA, SB;
+    fetch(ix0 - 1, SA);
+    if constexpr (!START) fetch(ix0, SB);
+    double alpha = 0.0, beta = 0.0;
+    bool first = true;
+    if constexpr (!START) {
+        if (!cg1_head<DETECT, KX>(v, par, sd, xpart, alpha, beta, first)) return;
+    }
     double a_g = 0.0, a_rr = 0.0, a_bb = 0.0, a_r64 = 0.0;
     double a_dl = 0.0, a_1w = 0.0, a_c = 0.0, a_uu = 0.0;
     const float af = (float)alpha, bf = (float)beta;
     // u of plane P (zero off the interior), own nodes updated and stored
     auto form = [&](int P, int slot, const Stage& St) __attribute__((always_inline)) {
         const bool own = P >= ix0 && P < ix1;           // uniform over the block
-        const __amdgpu_buffer_rsrc_t ro = plane(v.r[out], P);
-        const __amdgpu_buffer_rsrc_t eo2 = plane(v.e, P);
+        const int pq = P * S;
 #pragma unroll
         for (int k = 0; k < C1XR; ++k) {
             const int i = tid + k * C1NT;
+            const bool mine = own && xown[k];
+            const unsigned ow = mine ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
             float un;
             if (START) {
                 const double r64 = fma(-1.0, St.ax[k], St.bb[k]);
                 const float rf = (float)r64;
                 un = rf * St.d[k];
-                if (own) {
-                    c1_stf(rf, ro, xw[k]);
-                    c1_stf(0.0f, eo2, xw[k]);
-                    if (xown[k]) {
-                        a_g += (double)rf * (double)un;
-                        a_rr += (double)rf * (double)rf;
-                        a_bb += St.bb[k] * St.bb[k];
-                        a_r64 += r64 * r64;
-                    }
-                }
+                c1_stf(rf, R_rout, ow);
+                c1_stf(0.0f, R_e, ow);
+                a_g += mine ? (double)rf * (double)un : 0.0;
+                a_rr += mine ? (double)rf * (double)rf : 0.0;
+                a_bb += mine ? St.bb[k] * St.bb[k] : 0.0;
+                a_r64 += mine ? r64 * r64 : 0.0;
             } else {
                 const float sn = first ? St.w[k] : fmaf(bf, St.s[k], St.w[k]);
                 const float rn = fmaf(-af, sn, St.r[k]);
                 un = rn * St.d[k];
-                if (own) {
-                    const float uo = St.r[k] * St.d[k];
-                    const float pn = first ? uo : fmaf(bf, St.p[k], uo);
-                    c1_stf(fmaf(af, pn, St.e[k]), eo2, xw[k]);
-                    c1_stf(pn, plane(v.p, P), xw[k]);
-                    c1_stf(sn, plane(v.s[par], P), xw[k]);
-                    c1_stf(rn, ro, xw[k]);
-                    if (xown[k]) {
-                        a_g += (double)rn * (double)un;
-                        a_rr += (double)rn * (double)rn;
-                    }
-                }
+                const float uo = St.r[k] * St.d[k];
+                const float pn = first ? uo : fmaf(bf, St.p[k], uo);
+                c1_stf(fmaf(af, pn, St.e[k]), R_e, ow);
+                c1_stf(pn, R_p, ow);
+                c1_stf(sn, R_sout, ow);
+                c1_stf(rn, R_rout, ow);
+                a_g += mine ? (double)rn * (double)un : 0.0;
+                a_rr += mine ? (double)rn * (double)rn : 0.0;
             }
             if (i < C1XN) us[slot][i] = un;
         }
@@ -366,12 +408,15 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, doubl
         for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz] = Coef<float>{0.0f, 0.0f};
     const int par_yz = (py << 1) | pz;
     const FiArm nofi{0, 0, 0};
-    // one incoming plane P: form it, load plane P + 2 into the same stage,
-    // then the stencil of plane P - 1
+    // one incoming plane P: form it, load plane P + AHEAD into the same
+    // stage, then the stencil of plane P - 1 (stored and summed if it is one
+    // of the chunk's planes)
+    constexpr int AHEAD = START ? 1 : 2;
     auto step = [&](int P, Stage& St) __attribute__((always_inline)) {
         const int slot = P & 1;
         form(P, slot, St);
-        if (P + 2 <= ix1) fetch(P + 2, St);
+        const double wa = DETECT ? St.wa : 0.0;
+        fetch(P + AHEAD, St);
         __syncthreads();
 #pragma unroll
         for (int b2 = 0; b2 < 3; ++b2)
@@ -385,20 +430,18 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, doubl
                 Enew[cy][cz].ck = ek[slot][eidx[cy][cz]];
             }
         const int X = P - 1;                            // the plane whose stencil is now complete
-        if (X >= ix0) {                                 // uniform over the block
-            const float wn = (X & 1) ? march_node<float, false, 1>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi)
-                                     : march_node<float, false, 0>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi);
-            // u is 0 at a position outside the mesh, so such a lane adds exact zeros
-            const float uc = W[1][1][1];
-            c1_stf(wn, plane(v.w[out], X), wo);
-            const double wd = (double)wn, ud = (double)uc;
-            a_dl += wd * ud;
-            if (DETECT) {
-                const double wa = c1_ldd(c1_rsrc(v.wA + (size_t)X * S, 2 * PB), wod);
-                a_1w += node_in ? wd : 0.0;
-                a_c += wa * ud;
-                a_uu += ud * ud;
-            }
+        const bool xact = X >= ix0 && X < ix1;          // uniform over the block
+        const float wn = (X & 1) ? march_node<float, false, 1>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi)
+                                 : march_node<float, false, 0>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi);
+        // u is 0 at a position outside the mesh, so such a lane adds exact zeros
+        const float uc = W[1][1][1];
+        c1_stf(wn, R_wout, (xact && node_in) ? (unsigned)(X * S + wq) * 4u : C1_OOB);
+        const double wd = (double)wn, ud = (double)uc;
+        a_dl += xact ? wd * ud : 0.0;
+        if (DETECT) {
+            a_1w += (xact && node_in) ? wd : 0.0;
+            a_c += xact ? wa * ud : 0.0;
+            a_uu += xact ? ud * ud : 0.0;
         }
 #pragma unroll
         for (int b2 = 0; b2 < 3; ++b2)
@@ -412,38 +455,40 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, doubl
 #pragma unroll
             for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz];
     };
-    Stage SA, SB;
-    fetch(ix0 - 1, SA);
-    fetch(ix0, SB);
-    for (int P = ix0 - 1; P <= ix1; P += 2) {
-        step(P, SA);
-        if (P + 1 <= ix1) step(P + 1, SB);
+    // planes ix0 - 1 .. ix1 (and one more when their count is odd)
+    if constexpr (START) {
+        for (int P = ix0 - 1; P <= ix1; ++P) step(P, SA);
+    } else {
+        for (int P = ix0 - 1; P <= ix1; P += 2) {
+            step(P, SA);
+            step(P + 1, SB);
+        }
     }
-    double* tpo = v.tp[out];
-    const int nt = v.ntiles;
-    const double t0 = tree256(sd, 0, a_g), t1 = tree256(sd, 1, a_dl), t2 = tree256(sd, 2, a_rr);
-    double t3 = 0.0, t4 = 0.0, t5 = 0.0, t6 = 0.0, t7 = 0.0;
-    if (DETECT || START) {
-        t3 = tree256(sd, 3, a_1w);
-        t4 = tree256(sd, 4, a_c);
-        t5 = tree256(sd, 5, a_uu);
-    }
+    double t[8];
     if (START) {
-        t6 = tree256(sd, 6, a_bb);
-        t7 = tree256(sd, 7, a_r64);
+        const double a8[8] = {a_g, a_dl, a_rr, a_1w, a_c, a_uu, a_bb, a_r64};
+        tree_rows<8>(sd, a8, t);
+    } else if (DETECT) {
+        const double a6[6] = {a_g, a_dl, a_rr, a_1w, a_c, a_uu};
+        tree_rows<6>(sd, a6, t);
+    } else {
+        const double a3[3] = {a_g, a_dl, a_rr};
+        tree_rows<3>(sd, a3, t);
     }
     if (tid == 0) {
-        tpo[b] = t0;
-        tpo[nt + b] = t1;
-        tpo[2 * nt + b] = t2;
+        double* tpo = v.tp[out];
+        const int nt = v.ntiles;
+        tpo[b] = t[0];
+        tpo[nt + b] = t[1];
+        tpo[2 * nt + b] = t[2];
         if (DETECT || START) {
-            tpo[3 * nt + b] = t3;
-            tpo[4 * nt + b] = t4;
-            tpo[5 * nt + b] = t5;
+            tpo[3 * nt + b] = t[3];
+            tpo[4 * nt + b] = t[4];
+            tpo[5 * nt + b] = t[5];
         }
         if (START) {
-            tpo[6 * nt + b] = t6;
-            tpo[7 * nt + b] = t7;
+            tpo[6 * nt + b] = t[6];
+            tpo[7 * nt + b] = t[7];
         }
     }
 }
@@ -456,7 +501,7 @@ __global__ void k_recip(float* __restrict__ dinv, const float* __restrict__ d, i
 
 // The pass's first launch: r0, u0, w0 and their sums (outputs of parity 0).
 template <bool DETECT>
-__global__ __launch_bounds__(C1NT) void k_cg1_start(Cg1View v) {
+__global__ __launch_bounds__(C1NT) void k_cg1_start(Cg1View v, int pass) {
     __shared__ float us[2][C1XN];
     __shared__ float er[2][C1EN], ek[2][C1EN];
     __shared__ double sd[C1ROWS][NT];
@@ -466,24 +511,23 @@ __global__ __launch_bounds__(C1NT) void k_cg1_start(Cg1View v) {
         fs->iter = 0;
         fs->done = 0;
         fs->cg_it[0] = 0;
+        fs->cg_pass = pass;
+        fs->cg_status = 0;
         fs->inner_base = fs->inner_total;
     }
     if (b >= v.ntiles) return;
-    cg1_body<DETECT, true>(v, b, 1, 0.0, 0.0, true, us, er, ek, sd);
+    cg1_body<DETECT, true, false>(v, b, 1, us, er, ek, sd);
 }
 
-// Step i of the inner solve (par = i & 1).
-template <bool DETECT>
+// Step i of the inner solve (par = i & 1); FOLD8: the pass's first step.
+template <bool DETECT, bool FOLD8>
 __global__ __launch_bounds__(C1NT) void k_cg1_step(Cg1View v, int par) {
     __shared__ float us[2][C1XN];
     __shared__ float er[2][C1EN], ek[2][C1EN];
     __shared__ double sd[C1ROWS][NT];
     const int b = xcd_slab_block(v.ntiles);
     if (b >= v.ntiles) return;
-    double alpha, beta;
-    bool first;
-    if (!cg1_head<DETECT>(v, par, sd, alpha, beta, first)) return;
-    cg1_body<DETECT, false>(v, b, par, alpha, beta, first, us, er, ek, sd);
+    cg1_body<DETECT, false, FOLD8>(v, b, par, us, er, ek, sd);
 }
 
 }  // namespace heat

@@ -119,9 +119,6 @@ struct heat_op {
     double* c1_tp = nullptr;               // 2 x C1ROWS x tiles
     float* c1_dinv = nullptr;              // FP32 reciprocal of the diagonal
     int c1_nby = 0, c1_nbz = 0, c1_nch = 0, c1_ntiles = 0;
-    hipGraph_t c1_graph[2] = {nullptr, nullptr};      // a chunk by the parity of its first step
-    hipGraphExec_t c1_exec[2] = {nullptr, nullptr};
-    int c1_key[2] = {0, 0};
     // captured PCG chunk
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -641,8 +638,6 @@ void free_all(heat_op* op) {
     if (op->gexec) (void)hipGraphExecDestroy(op->gexec);
     if (op->graph) (void)hipGraphDestroy(op->graph);
     for (int q = 0; q < 2; ++q) {
-        if (op->c1_exec[q]) (void)hipGraphExecDestroy(op->c1_exec[q]);
-        if (op->c1_graph[q]) (void)hipGraphDestroy(op->c1_graph[q]);
         (void)hipFree(op->c1_s[q]);
         (void)hipFree(op->c1_w[q]);
     }
@@ -900,40 +895,14 @@ int cg1_launch_step(heat_op* op, const Cg1View& v, int par, bool first_of_pass) 
         else k_cg1_fin<false><<<1, NT, 0, s>>>(v, par, first_of_pass ? 1 : 0);
         CHECK_LAUNCH();
     }
-    if (detect) k_cg1_step<true><<<grid, C1NT, 0, s>>>(v, par);
-    else k_cg1_step<false><<<grid, C1NT, 0, s>>>(v, par);
-    CHECK_LAUNCH();
-    return HEAT_OK;
-}
-
-// pcg_chunk steps starting with parity par0 (the pass's later steps: never the
-// first of the pass), captured once per starting parity as a hipGraph
-int cg1_launch_chunk(heat_op* op, const Cg1View& v, int par0) {
-    auto body = [&]() -> int {
-        for (int i = 0; i < op->pcg_chunk; ++i) {
-            const int rc = cg1_launch_step(op, v, (par0 + i) & 1, false);
-            if (rc) return rc;
-        }
-        return HEAT_OK;
-    };
-    if (!op->use_graph) return body();
-    const int key = 1 + op->pcg_chunk * 4 + (op->cfg.detect ? 2 : 0) + (v.pre ? 1 : 0);
-    hipGraphExec_t& ex = op->c1_exec[par0];
-    hipGraph_t& gr = op->c1_graph[par0];
-    if (!ex || op->c1_key[par0] != key) {
-        if (ex) { (void)hipGraphExecDestroy(ex); ex = nullptr; }
-        if (gr) { (void)hipGraphDestroy(gr); gr = nullptr; }
-        HIP_TRY(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = body();
-        hipGraph_t graph = nullptr;
-        const hipError_t ec = hipStreamEndCapture(op->stream, &graph);
-        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-        if (ec != hipSuccess) return fail(HEAT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
-        gr = graph;
-        HIP_TRY(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
-        op->c1_key[par0] = key;
+    if (first_of_pass) {
+        if (detect) k_cg1_step<true, true><<<grid, C1NT, 0, s>>>(v, par);
+        else k_cg1_step<false, true><<<grid, C1NT, 0, s>>>(v, par);
+    } else {
+        if (detect) k_cg1_step<true, false><<<grid, C1NT, 0, s>>>(v, par);
+        else k_cg1_step<false, false><<<grid, C1NT, 0, s>>>(v, par);
     }
-    HIP_TRY(hipGraphLaunch(ex, op->stream));
+    CHECK_LAUNCH();
     return HEAT_OK;
 }
 
@@ -1266,6 +1235,9 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
 
 // iterations per polled chunk of the refined solve's inner passes (at most)
 constexpr int MIXED_CHUNK = 4;
+// steps of the one-launch inner iteration queued beyond the one the device
+// last reported (each returns at once after the pass's end)
+constexpr int CG1_AHEAD = 2;
 
 int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer, int max_inner,
                          double tol, double tol_inner, int* outer, int* inner_iters) {
@@ -1327,7 +1299,12 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
                                   op->pcg_fi.iter);
     CHECK_LAUNCH();
     const bool detect = op->cfg.detect != 0;
-    if (op->inner_mode == HEAT_INNER_ONE_LAUNCH && !op->pcg_fi.armed) {
+    // the one-launch walk addresses every array with 32-bit byte offsets:
+    // meshes up to N = 644 (nn * 8 bytes below 2^31), and its status word
+    // counts steps below 2^20; larger meshes or limits run the standard
+    // iteration
+    const bool cg1_fits = (long long)g.nn * 8 < (1LL << 31) && max_inner < CG1_MAX_STEP - 1;
+    if (op->inner_mode == HEAT_INNER_ONE_LAUNCH && !op->pcg_fi.armed && cg1_fits) {
         // the inner solve in one launch per iteration (heat_cg1.h); an armed
         // upset of the inner state runs the standard two-kernel iteration below
         rc = ensure_cg1(op);
@@ -1335,49 +1312,66 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         double* Ax = op->vec[HEAT_VEC_AP];
         const Cg1View v = make_cg1_view(op, b, Ax);
         const unsigned grid = (unsigned)(((v.ntiles + NXCD - 1) / NXCD) * NXCD);
-        const int saved_chunk = op->pcg_chunk;
-        op->pcg_chunk = std::min(saved_chunk, MIXED_CHUNK);
-        int nout = 0;
-        for (;;) {
-            rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
-            if (rc) break;
-            if (detect) k_cg1_start<true><<<grid, C1NT, 0, s>>>(v);
-            else k_cg1_start<false><<<grid, C1NT, 0, s>>>(v);
-            if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
-            // the first step makes the outer test: after a correction it goes
-            // alone and is read back, so a solve the correction finished stops
-            int launched = 0;
-            bool finished = false;
-            if (nout > 0) {
-                rc = cg1_launch_step(op, v, 0, true);
-                if (rc == HEAT_OK) rc = read_fused(op);
-                if (rc) break;
-                launched = 1;
-                finished = op->hfst->done != 0;
+        // The passes follow each other on the device, one plain launch per
+        // step (a graph launch would add a gap of its own per chunk): the
+        // head of every step stores cg1_flag(pass, step, status) in a pinned
+        // word, and the host keeps CG1_AHEAD steps queued beyond the step
+        // the device last reported.  When a step reports the pass's end the
+        // host queues the correction and the next pass behind the steps
+        // still in flight, which return at once; no event, no read-back.
+        int nout = 0, pass = 1, launched = 0, status = 0;
+        auto start_pass = [&]() -> int {
+            int r2 = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
+            if (r2) return r2;
+            if (detect) k_cg1_start<true><<<grid, C1NT, 0, s>>>(v, pass);
+            else k_cg1_start<false><<<grid, C1NT, 0, s>>>(v, pass);
+            CHECK_LAUNCH();
+            launched = 0;
+            return HEAT_OK;
+        };
+        volatile int* hflag = op->hdone;
+        // pass 0 is no pass of this solve: the fused solves' done flag, the
+        // word's other writer, stores pass 0, and every earlier one-launch
+        // solve ended with a read-back, so none of its steps is in flight
+        *hflag = 0;
+        rc = start_pass();
+        for (long spins = 0; rc == HEAT_OK; ++spins) {
+            const int f = *hflag;
+            const bool mine = cg1_flag_pass(f) == cg1_flag_pass(cg1_flag(pass, 0, 0));
+            if (mine && cg1_flag_status(f) != 0) {
+                status = cg1_flag_status(f);
+                if (status != 1 || nout >= max_outer) break;
+                // the inner solve of this pass ended and the outer test was not met: correct, next pass
+                k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
+                if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
+                ++nout;
+                ++pass;
+                rc = start_pass();
+                continue;
             }
-            if (!finished) {
-                rc = run_until_done(op, max_inner + 1, [&]() {
-                    int r2 = HEAT_OK;
-                    if (launched == 0) {
-                        // the pass's first step folds the start's partials (8 rows)
-                        r2 = cg1_launch_step(op, v, 0, true);
-                        launched = 1;
-                        if (r2) return r2;
-                    }
-                    r2 = cg1_launch_chunk(op, v, launched & 1);
-                    launched += op->pcg_chunk;
-                    return r2;
-                }, &op->fst->done, op->hdone);
-                if (rc == HEAT_OK) rc = read_fused(op);
-                if (rc) break;
+            const int running = mine ? cg1_flag_step(f) : -1;
+            if (launched - running <= CG1_AHEAD && launched <= max_inner + 1) {
+                // the pass's first step folds the start's eight partial rows
+                rc = cg1_launch_step(op, v, launched & 1, launched == 0);
+                ++launched;
+                continue;
             }
-            if (op->hfst->outer_done == 3) { rc = fail(HEAT_E_STATE, "inner iteration launched out of step"); break; }
-            if (op->hfst->outer_done || nout >= max_outer) break;
-            k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
-            if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
-            ++nout;
+            if ((spins & 1023) == 0) {
+                // a fault ends the wait; so does a stream that ran dry without the pass's end
+                const hipError_t q = hipStreamQuery(s);
+                if (q == hipSuccess && *hflag == f) {
+                    rc = fail(HEAT_E_STATE, "inner solve did not report completion");
+                    break;
+                }
+                if (q != hipSuccess && q != hipErrorNotReady) {
+                    rc = fail(HEAT_E_HIP, "stream query failed: %s", hipGetErrorString(q));
+                    break;
+                }
+            }
+            __builtin_ia32_pause();
         }
-        op->pcg_chunk = saved_chunk;
+        if (rc == HEAT_OK) rc = read_fused(op);
+        if (rc == HEAT_OK && status == 3) rc = fail(HEAT_E_STATE, "inner iteration launched out of step");
         if (rc) return rc;
         op->pcg_fi_fired = 0;
         if (outer) *outer = nout;

@@ -83,6 +83,10 @@ struct FusedState {
     double cg_gamma[2], cg_alpha[2];
     double cg_red[2][8];
     int cg_it[2];
+    // pass number of the refined solve (set by the pass's start launch) and
+    // the status its inner solve ended with: 1 inner done, 2 the solve is
+    // finished (outer test met or residual not finite), 3 launched out of step
+    int cg_pass, cg_status;
 };
 
 enum { SEU_U = 0, SEU_R = 1, SEU_P = 2, SEU_Z = 3, SEU_AP = 4 };
@@ -147,17 +151,14 @@ __device__ __forceinline__ double tree256(double (*sd)[NT], int k, double v) {
     return sd[k][0];
 }
 
-// final stage over `count` partials of K sums (stride count), all threads get the results
+// K independent 256-wide trees (row k over v[k]) with one set of barriers;
+// each row's pairs and order are tree256's.  Results valid in every thread.
 template <int K>
-__device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* __restrict__ part, int count,
-                                                int stride, double out[K]) {
+__device__ __forceinline__ void tree_rows(double (*sd)[NT], const double (&v)[K], double* out) {
+    static_assert(NT == 256, "two LDS strides, then one wave");
     const int t = threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        double a = 0.0;
-        for (int i = t; i < count; i += NT) a += part[(size_t)k * stride + i];
-        sd[k][t] = a;
-    }
+    for (int k = 0; k < K; ++k) sd[k][t] = v[k];
     __syncthreads();
     if (t < 128) {
 #pragma unroll
@@ -175,6 +176,48 @@ __device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* 
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = sd[k][0];
     __syncthreads();
+}
+
+// Final stage over `count` partials of K sums (row k at part + k * stride):
+// thread t adds partials t, t + NT, t + 2 NT, ... of each row in that order,
+// then tree_rows; every thread gets the results.  The first four partials of
+// each row are loaded together (partials_load4) before any is added, so a
+// row of up to 4 NT partials costs one memory round trip.
+template <int K>
+__device__ __forceinline__ void partials_load4(const double* __restrict__ part, int count, int stride,
+                                               double (&x)[K][4]) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = t + j * NT;
+            x[k][j] = i < count ? part[(size_t)k * stride + i] : 0.0;
+        }
+}
+template <int K, int KX>
+__device__ __forceinline__ void partials_fold(double (*sd)[NT], const double (&x)[KX][4],
+                                              const double* __restrict__ part, int count, int stride,
+                                              double* out) {
+    static_assert(K <= KX, "rows loaded");
+    const int t = threadIdx.x;
+    double a[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        a[k] = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (t + j * NT < count) a[k] += x[k][j];
+        for (int i = t + 4 * NT; i < count; i += NT) a[k] += part[(size_t)k * stride + i];
+    }
+    tree_rows<K>(sd, a, out);
+}
+template <int K>
+__device__ __forceinline__ void reduce_partials(double (*sd)[NT], const double* __restrict__ part, int count,
+                                                int stride, double* out) {
+    double x[K][4];
+    partials_load4<K>(part, count, stride, x);
+    partials_fold<K, K>(sd, x, part, count, stride, out);
 }
 
 // r = b - Ap; z = r/d; super-block partials of b.b, r.r, r.z (super-block j).
