@@ -13,6 +13,9 @@
  * (11 digits), per-step CG counts and D1/D2/D3 traces of the N=32 lognormal
  * run, fault classes/detected_by/first-firing steps of the G and R campaign,
  * the V1 quadrants norm, and the closed forms of the reference matrices.
+ * ho_proto_run restates the MNHack7 prototype run (src/heat_solver.cu: the
+ * quadrant material at each quadrature point) for V1; every printed step of
+ * its recorded N=15 run (CG count and ||T|| to 7 digits) is pinned.
  */
 #include "heat_oracle.h"
 
@@ -43,6 +46,7 @@ struct ho_op {
     ho_grid g;
     double dt;
     int kernel, colored, verify, no_source, source_mode;
+    int proto;                  /* MNHack7 prototype: material of each quadrature point (ho_proto_run) */
     double source_value;
     double* k;
     double* rhoc;
@@ -197,9 +201,22 @@ static void elem_apply_fast(const ho_op* op, const double xe[8], double ye[8],
     }
 }
 
-/* src/heat_solver_het.cu:310-338 */
+/* The MNHack7 prototype's material (src/heat_solver.cu:101-111): four
+ * regions split at 0.5, kappa 100..400 and rho*c 1, 1.2, 1.2, 1.25. */
+static void proto_material(double x, double y, double z, double* k, double* rc) {
+    double kk, rho, c;
+    if (x < 0.5 && y < 0.5 && z < 0.5)       { kk = 100.0; rho = 1.0; c = 1.0; }
+    else if (x >= 0.5 && y < 0.5 && z < 0.5) { kk = 200.0; rho = 0.8; c = 1.5; }
+    else if (x < 0.5 && y >= 0.5 && z < 0.5) { kk = 300.0; rho = 0.6; c = 2.0; }
+    else                                     { kk = 400.0; rho = 0.5; c = 2.5; }
+    *k = kk;
+    *rc = rho * c;
+}
+
+/* src/heat_solver_het.cu:310-338; with op->proto the material of each
+ * quadrature point instead of the element's (src/heat_solver.cu:193-232) */
 static void elem_apply_gauss(const ho_op* op, const double xe[8], double ye[8],
-                             double k_e, double rhoc_e, double dt) {
+                             double k_e, double rhoc_e, double dt, int ex, int ey, int ez) {
     const double h = op->g.h;
     const double det_J = h * h * h / 8.0;
     const double inv_Jx = 2.0 / h, inv_Jy = 2.0 / h, inv_Jz = 2.0 / h;
@@ -217,11 +234,15 @@ static void elem_apply_gauss(const ho_op* op, const double xe[8], double ye[8],
             dNy[i] = inv_Jy * dNref[i][1];
             dNz[i] = inv_Jz * dNref[i][2];
         }
+        double kq = k_e, rq = rhoc_e;
+        if (op->proto)
+            proto_material(ex * h + (pts[gi] + 1.0) * 0.5 * h, ey * h + (pts[gj] + 1.0) * 0.5 * h,
+                           ez * h + (pts[gk] + 1.0) * 0.5 * h, &kq, &rq);
         for (int i = 0; i < 8; ++i) {
             double Mx = 0.0, Kx = 0.0;
             for (int j = 0; j < 8; ++j) {
-                Mx += rhoc_e * N[i] * N[j] * xe[j];
-                Kx += k_e * (dNx[i] * dNx[j] + dNy[i] * dNy[j] + dNz[i] * dNz[j]) * xe[j];
+                Mx += rq * N[i] * N[j] * xe[j];
+                Kx += kq * (dNx[i] * dNx[j] + dNy[i] * dNy[j] + dNz[i] * dNz[j]) * xe[j];
             }
             ye[i] += det_J * (Mx + dt * Kx);
         }
@@ -277,7 +298,7 @@ static void operator_element(ho_op* op, const double* x, double* y, double dt_ef
     for (int i = 0; i < 8; ++i) xe[i] = x[gid[i]];
     const double k_e = op->k[e], rhoc_e = op->rhoc[e];
     if (op->kernel == HO_KERNEL_FAST) elem_apply_fast(op, xe, ye, k_e, rhoc_e, dt_eff);
-    else if (op->kernel == HO_KERNEL_GAUSS) elem_apply_gauss(op, xe, ye, k_e, rhoc_e, dt_eff);
+    else if (op->kernel == HO_KERNEL_GAUSS) elem_apply_gauss(op, xe, ye, k_e, rhoc_e, dt_eff, ix, iy, iz);
     else elem_apply_struct(op, xe, ye, k_e, rhoc_e, dt_eff);
     if (fi->armed && fi->level == 2 && fi->rank == 0 && e == fi->target)
         ye[0] = u2d(d2u(ye[0]) ^ (1ULL << fi->bit));
@@ -501,6 +522,27 @@ static void diag_element(ho_op* op, double* d, long e, int ix, int iy, int iz) {
     long gid[8];
     element_nodes(&op->g, ix, iy, iz, gid);
     const double k_e = op->k[e], rhoc_e = op->rhoc[e];
+    if (op->proto) {
+        /* the prototype's Gauss sum with the material of each point (src/heat_solver.cu:297-343) */
+        const double h = op->g.h, det_J = h * h * h / 8.0, ij = 2.0 / h;
+        const double gp = 1.0 / 1.7320508075688772;
+        const double pts[2] = {-gp, gp};
+        double de[8] = {0};
+        for (int gi = 0; gi < 2; ++gi)
+        for (int gj = 0; gj < 2; ++gj)
+        for (int gk = 0; gk < 2; ++gk) {
+            double kq, rq, N[8], dN[8][3];
+            proto_material(ix * h + (pts[gi] + 1.0) * 0.5 * h, iy * h + (pts[gj] + 1.0) * 0.5 * h,
+                           iz * h + (pts[gk] + 1.0) * 0.5 * h, &kq, &rq);
+            shape_func_grad(pts[gi], pts[gj], pts[gk], N, dN);
+            for (int i = 0; i < 8; ++i) {
+                const double gx = ij * dN[i][0], gy = ij * dN[i][1], gz = ij * dN[i][2];
+                de[i] += det_J * (rq * N[i] * N[i] + op->dt * kq * (gx * gx + gy * gy + gz * gz));
+            }
+        }
+        for (int i = 0; i < 8; ++i) d[gid[i]] += de[i];
+        return;
+    }
     for (int i = 0; i < 8; ++i) {
         double de;
         if (op->kernel == HO_KERNEL_STRUCT)
@@ -598,6 +640,52 @@ void ho_destroy(ho_op* op) {
     free(op->ones); free(op->wM); free(op->wA);
     free(op->r); free(op->z); free(op->p); free(op->Ap);
     free(op);
+}
+
+static int pcg_impl(ho_op* op, const double* b, double* x, int max_iter, double tol,
+                    double* hist, FILE* diag_log);
+
+/* The MNHack7 prototype run (src/heat_solver.cu:679-758): the unit cube at
+ * N, T = 0, n_steps = (int)(t_final / dt + 0.5) backward-Euler steps of the
+ * Gauss-quadrature operator with the quadrant material at each quadrature
+ * point, atomic scatter, the Gaussian source, each step PCG to 1e-8 in at
+ * most 500 iterations from the previous T.  iters[s] and norms[s] receive
+ * step s's iteration count and ||T||_2 (the prototype prints them at
+ * s % 10 == 0 and the last step).  Returns n_steps, or -1. */
+int ho_proto_run(int n, double t_final, double dt, int nthreads, int max_steps, int* iters, double* norms) {
+    const int n_steps = (int)(t_final / dt + 0.5);
+    if (n < 1 || n_steps > max_steps) return -1;
+    const long ne = (long)n * n * n;
+    double* k = (double*)malloc(sizeof(double) * ne);
+    double* rc = (double*)malloc(sizeof(double) * ne);
+    const double h = 1.0 / n;
+    for (int ix = 0; ix < n; ++ix)                      /* element-centre values: kmin/kmax only */
+        for (int iy = 0; iy < n; ++iy)
+            for (int iz = 0; iz < n; ++iz) {
+                const long e = ((long)ix * n + iy) * n + iz;
+                proto_material((ix + 0.5) * h, (iy + 0.5) * h, (iz + 0.5) * h, &k[e], &rc[e]);
+            }
+    ho_op* op = create_impl(n, dt, k, rc, HO_KERNEL_GAUSS, 0, 0, HO_SOURCE_GAUSSIAN, 0.0, 0, 0);
+    free(k);
+    free(rc);
+    if (!op) return -1;
+    op->nthreads = nthreads < 1 ? 1 : nthreads;
+    op->proto = 1;
+    const long nn = op->g.nn;
+    memset(op->diag, 0, sizeof(double) * nn);           /* the diagonal again, point by point */
+    element_loop(op, op->diag, 0);
+    dirichlet_value(op, op->diag, 1.0);
+    double* T = (double*)calloc(nn, sizeof(double));
+    double* b = (double*)malloc(sizeof(double) * nn);
+    for (int s = 0; s < n_steps; ++s) {
+        ho_build_rhs(op, T, b);
+        iters[s] = pcg_impl(op, b, T, 500, 1e-8, NULL, NULL);
+        norms[s] = sqrt(ho_dot(op, T, T));
+    }
+    free(T);
+    free(b);
+    ho_destroy(op);
+    return n_steps;
 }
 
 void ho_set_threads(ho_op* op, int nthreads) { op->nthreads = nthreads < 1 ? 1 : nthreads; }

@@ -94,6 +94,8 @@ int    ho_pcg_solve_mixed_cg1  (ho_op* op, const double* b, double* x, int max_o
 void   ho_initial_condition(ho_op* op, double* v, int mode, double t, double alpha);
 
 /* the whole command-line program (src/heat_solver_het.cu:1355-1629) */
+/* the MNHack7 prototype run: per-step CG iterations and ||T||_2; returns the step count or -1 */
+int    ho_proto_run(int n, double t_final, double dt, int nthreads, int max_steps, int* iters, double* norms);
 int    ho_main(int argc, char** argv);
 
 #ifdef __cplusplus

@@ -43,6 +43,7 @@ _SIG = {
     "ho_pcg_fault_fired": (_I, [_P]),
     "ho_fused_tile_sums": (None, [_P, _P, _P, _P, _P]),
     "ho_initial_condition": (None, [_P, _P, _I, _D, _D]),
+    "ho_proto_run": (_I, [_I, _D, _D, _I, _I, _P, _P]),
 }
 _lib = None
 
@@ -209,6 +210,17 @@ class OracleOp:
         o = np.empty(4)
         lib().ho_get_stats(self.h, _p(o))
         return o
+
+
+def proto_run(n, t_final, dt, threads=8):
+    """The MNHack7 prototype run (quadrant material at each quadrature point):
+    per-step CG iterations and ||T||_2."""
+    cap = int(t_final / dt + 0.5) + 1
+    its = np.zeros(cap, dtype=np.int32)
+    nrm = np.zeros(cap)
+    k = lib().ho_proto_run(n, t_final, dt, threads, cap, _p(its), _p(nrm))
+    assert k > 0
+    return its[:k], nrm[:k]
 
 
 def flip_bit(v, idx, bit):

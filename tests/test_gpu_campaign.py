@@ -129,6 +129,60 @@ def test_dump_snaps_files(tmp_path):
 
 
 @pytest.mark.gpu
+def test_dump_snaps_and_diag_cg_match_oracle_cli(tmp_path):
+    """The driver's per-snapshot dumps (--dump-snaps, src/heat_solver_het.cu:
+    1239-1245) and its one-solve CG log (--diag-cg, :1491-1504) against the
+    oracle's command line run with the same arguments in the GPU library's
+    arithmetic (its closed-form element rows, --kernel struct, which the
+    driver's fast kernel uses, and the fused summation order): the same
+    files, every snapshot of the clean and the faulted runs and the final
+    fields bit for bit, and the same CG log (the iteration count, and every
+    residual, alpha and beta as printed).  The source is a constant
+    (--source-const): the Gaussian bump's exp differs between the GPU's and
+    the host's libm in the last place, which no longer gives equal bits."""
+    from oracle_lib import ORACLE_CLI
+    N = 12
+    fp = str(tmp_path / "ln12")
+    k, r = fields.lognormal(N, sigma=1.0, L=0.125, seed=5)
+    fields.write_field(fp, k, r, {"n": N})
+    target = validations.node_index(6, 6, 6, N)
+    args = [str(N), "0.01", "5e-4", fp]
+    snap_args = ["--tag", "sm", "--snap", "5", "--kernel", "fast", "--scatter", "colored", "--dump-snaps",
+                 "--source-const", "1000", "--fi", "1", "4", str(target), "55"]
+    outs = {}
+    for who, cmd0 in (("hip", [build.DRIVER]), ("oracle", [ORACLE_CLI])):
+        d = tmp_path / who
+        d.mkdir()
+        extra = ["--pcg-order", "fused", "--kernel", "struct"] if who == "oracle" else []
+        cmd = cmd0 + args + [str(d)] + snap_args + extra
+        print(f"[case] {' '.join(cmd)}", flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr
+        log = str(d / "cg.csv")
+        cmd = (cmd0 + args + [str(d), "--kernel", "fast", "--scatter", "colored", "--source-const", "1000",
+                              "--diag-cg", log] + extra)
+        print(f"[case] {' '.join(cmd)}", flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        assert res.returncode == 0, res.stderr
+        outs[who] = d
+    files = sorted(f for f in os.listdir(outs["hip"]) if f.endswith(".bin"))
+    assert files == sorted(f for f in os.listdir(outs["oracle"]) if f.endswith(".bin"))
+    assert sum(f.startswith("snap_sm_") for f in files) == 15 and len(files) == 17
+    diff = {}
+    for f in files:
+        a = np.fromfile(str(outs["hip"] / f))
+        b = np.fromfile(str(outs["oracle"] / f))
+        assert a.size == b.size == (N + 1) ** 3, f
+        if not np.array_equal(a, b):
+            diff[f] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    assert not diff, diff
+    ha = open(outs["hip"] / "cg.csv").read().splitlines()
+    hb = open(outs["oracle"] / "cg.csv").read().splitlines()
+    print(f"  {len(files)} field files bit-identical; CG log of {len(ha) - 2} iterations", flush=True)
+    assert len(ha) > 10 and ha == hb
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("batch,detect", [(5, True), (64, True), (64, False)])
 def test_batched_campaign_equals_sequential(tmp_path, batch, detect):
     """Fault runs advanced together as one batch of handles (--campaign-batch)
