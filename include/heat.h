@@ -235,6 +235,12 @@ int heat_batch_instance(heat_batch* b, int i, heat_op** op);
 /* Per-solve mask (nonzero = active); masked solves keep their T. */
 int heat_batch_set_active(heat_batch* b, const int* active);
 int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph);
+/* Nonzero ON (new; no reference counterpart): each batched PCG solve of
+ * heat_batch_step and heat_batch_step_mixed runs as ONE persistent
+ * (cooperative) launch whose resident blocks walk every iteration's phases
+ * with grid-wide barriers between them, instead of graph-replayed chunks of
+ * launches.  Same iterates, same bits. */
+int heat_batch_set_persistent(heat_batch* b, int on);
 /* A batch of COUNT right-hand sides on one mesh (new): the members share
  * OP's coefficients and diagonal (OP stays owned by the caller and must
  * outlive the batch) and own their T, b, PCG vectors and source vector F,

@@ -1743,6 +1743,9 @@ struct heat_batch {
     int *d_tile_off = nullptr, *d_sb_off = nullptr, *d_node_off = nullptr;
     double *sp = nullptr, *tp = nullptr;
     int pcg_chunk = 8, use_graph = 1;
+    int persistent = 0;                      // one persistent launch per solve (k_batch_persistent)
+    int persist_iters = 0;                   // iterations that launch may run
+    int* dpflag = nullptr;                   // device: its all-done decisions by iteration parity
     int detect = 0;                          // members built with detectors (all or none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1776,7 +1779,7 @@ void batch_free(heat_batch* b) {
     for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
     if (b->hflag) (void)hipHostFree(b->hflag);
     if (b->hiters) (void)hipHostFree(b->hiters);
-    (void)hipFree(b->dflag); (void)hipFree(b->diters); (void)hipFree(b->dviews);
+    (void)hipFree(b->dflag); (void)hipFree(b->diters); (void)hipFree(b->dviews); (void)hipFree(b->dpflag);
     if (b->hdone) (void)hipHostFree(b->hdone);
     (void)hipFree(b->d_tile_off); (void)hipFree(b->d_sb_off); (void)hipFree(b->d_node_off);
     (void)hipFree(b->sp); (void)hipFree(b->tp);
@@ -1830,12 +1833,42 @@ int batch_upload_views(heat_batch* b) {
     return HEAT_OK;
 }
 
+// The whole batched solve as one cooperative launch of k_batch_persistent:
+// as many blocks as stay resident together (a multiple of NXCD), at most
+// `iters` iterations; the kernel stores the all-done flag in b->hdone.
+template <typename T>
+int batch_persistent_t(heat_batch* b, const std::vector<FusedView<T>>& views, const FusedView<T>* dv, int iters) {
+    int count = (int)views.size();
+    int pre_sp = 0, pre_tp = 0;
+    for (const auto& v : views) { pre_sp |= v.pre_sp; pre_tp |= v.pre_tp; }
+    const bool detect = b->detect && sizeof(T) == sizeof(double);
+    const void* fn = detect ? reinterpret_cast<const void*>(&k_batch_persistent<T, true>)
+                            : reinterpret_cast<const void*>(&k_batch_persistent<T, false>);
+    int per_cu = 0, dev = 0, ncu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, 0));
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = (per_cu * ncu / NXCD) * NXCD;
+    if (grid < NXCD) return fail(HEAT_E_HIP, "the persistent batch kernel does not fit on the device");
+    if (!b->dpflag) HIP_TRY(hipMalloc(&b->dpflag, 2 * sizeof(int)));
+    const FusedView<T>* a_views = dv;
+    const int* a_tile = b->d_tile_off;
+    const int* a_sb = b->d_sb_off;
+    int* a_flag = b->dpflag;
+    int* a_hdone = b->hdone_dev;
+    void* args[] = {&a_views, &a_tile, &a_sb, &count, &iters, &pre_sp, &pre_tp, &a_flag, &a_hdone};
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(NT), args, 0, b->stream));
+    return HEAT_OK;
+}
+
 // One chunk of pcg_chunk batched fused iterations over the views `dv` (FP64
 // solves, or the FP32 inner solves of refined ones), then the all-done flag;
-// captured once per value type as a hipGraph.
+// captured once per value type as a hipGraph.  A persistent batch runs the
+// whole solve in one launch instead.
 template <typename T>
 int batch_chunk_t(heat_batch* b, const std::vector<FusedView<T>>& views, const FusedView<T>* dv,
                   hipGraph_t& graph_, hipGraphExec_t& gexec_, int& graph_chunk_) {
+    if (b->persistent) return batch_persistent_t<T>(b, views, dv, b->persist_iters);
     const int count = (int)views.size();
     hipStream_t s = b->stream;
     const int total_tiles = b->tile_off[count], total_sb = b->sb_off[count];
@@ -1907,6 +1940,15 @@ template <typename Chunk>
 int batch_until_done(heat_batch* b, int max_iter, Chunk chunk) {
     hipStream_t s = b->stream;
     const volatile int* hdone = b->hdone;
+    if (b->persistent) {
+        // one launch: every solve stops by iteration max_iter (+1 for its last test)
+        b->persist_iters = max_iter + 2;
+        const int rc = chunk();
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        if (!*hdone) return fail(HEAT_E_STATE, "batch PCG did not report completion");
+        return HEAT_OK;
+    }
     const int max_chunks = max_iter / b->pcg_chunk + 3;
     int rc = chunk();
     if (rc) return rc;
@@ -2084,6 +2126,12 @@ int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph) {
     if (!b || pcg_chunk < 1 || pcg_chunk > 64) return fail(HEAT_E_ARG, "pcg_chunk must be in 1..64");
     b->pcg_chunk = pcg_chunk;
     b->use_graph = use_graph ? 1 : 0;
+    return HEAT_OK;
+}
+
+int heat_batch_set_persistent(heat_batch* b, int on) {
+    if (!b) return fail(HEAT_E_ARG, "null argument");
+    b->persistent = on ? 1 : 0;
     return HEAT_OK;
 }
 

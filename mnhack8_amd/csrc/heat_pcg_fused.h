@@ -34,6 +34,8 @@
 // c = w.p and xx = p.p are summed where the tile writes p (every node of the
 // mesh exactly once, boundary included), in plane order, window slot order.
 #pragma once
+#include <hip/hip_cooperative_groups.h>
+
 #include "heat_kernels.h"
 
 namespace heat {
@@ -266,13 +268,33 @@ __global__ __launch_bounds__(NT) void k_fused_init(FusedView<T> v) {
 // at iteration 0, the init partials), in the reference's order of checks
 // (:1020-1035); block 0 records the history and the done flag.  False: the
 // solve is done and the block does nothing more (uniform over the grid).
-template <typename T>
+// The done flag as one value for the whole block.  PERSIST (the persistent
+// batched kernel, whose blocks go on to other tiles after a body returns):
+// thread 0 reads it and hands it on through LDS, since block 0 of a solve
+// may set it while another block of the same phase reads it, and a block
+// whose waves read different values would leave the body at different
+// barriers.  Otherwise each wave reads it (a wave that leaves ends).
+template <bool PERSIST>
+__device__ __forceinline__ int block_done(const FusedState* fs) {
+    if constexpr (!PERSIST) {
+        return fs->done;
+    } else {
+        __shared__ int s_done;
+        if (threadIdx.x == 0) s_done = __hip_atomic_load(&fs->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const int d = s_done;
+        __syncthreads();
+        return d;
+    }
+}
+
+template <typename T, bool PERSIST = false>
 __device__ __forceinline__ bool fused_head(const FusedView<T>& v, int b, double (*sd)[NT], double& beta) {
     const double* __restrict__ sp = v.sp;
     const int nsb = v.nsb, sp_stride = v.sp_stride;
     FusedState* fs = v.fs;
     double* __restrict__ hist = v.hist;
-    if (fs->done) return false;
+    if (block_done<PERSIST>(fs)) return false;
     const int iter = fs->iter;
     // ---- beta, convergence (reference order of checks, :1020-1035)
     beta = 0.0;
@@ -592,10 +614,10 @@ __global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_apply(FusedView<T> v) {
 // detectors, over the tile's own nodes (each node of the mesh exactly once),
 // in plane order, slot order, with the D1 sums w.p and p.p of K1 (partial
 // rows 2 and 3); without them, as a plain stream.
-template <typename T, bool DETECT>
+template <typename T, bool DETECT, bool PERSIST = false>
 __device__ __forceinline__ void fused_dir_body(const FusedView<T>& v, int b, double (*sd)[NT]) {
     double beta;
-    if (!fused_head<T>(v, b, sd, beta)) return;
+    if (!fused_head<T, PERSIST>(v, b, sd, beta)) return;
     constexpr int NTH = FTY * FTZ, XW = FTZ + 2, XL = XW * (FTY + 2);
     constexpr int XR = (XL + NTH - 1) / NTH;
     const FusedState* fs = v.fs;
@@ -723,7 +745,7 @@ __global__ __launch_bounds__(FTY* FTZ, 3) void k_fused_tdma(FusedView<T> v) {
 
 // K2.  tp: K1 partials.  sp: output super-block partials (r.r, r.z).
 // (super-block j of the solve's nsb)
-template <typename T, bool DETECT>
+template <typename T, bool DETECT, bool PERSIST = false>
 __device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) {
     T* __restrict__ x = v.x;
     const T* __restrict__ P0 = v.P0;
@@ -745,7 +767,7 @@ __device__ __forceinline__ void fused_update_body(const FusedView<T>& v, int j) 
     // the flag back (a system-scope store to pinned host memory)
     if (v.hflag && j == 0 && threadIdx.x == 0)
         __hip_atomic_store(v.hflag, fs->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    if (fs->done) return;
+    if (block_done<PERSIST>(fs)) return;
     const int k1 = fs->k1_iter;
     double o[4];
     if (v.pre_tp) { for (int k = 0; k < 4; ++k) o[k] = fs->red_tp[k]; }
@@ -811,8 +833,7 @@ __global__ __launch_bounds__(NT) void k_fused_update(FusedView<T> v) {
 // One-block finalize of the super-block partials K1 will read (the rows its
 // iteration needs) and of the tile partials K2 will read.
 template <typename T>
-__device__ __forceinline__ void fused_fin_sp_body(const FusedView<T>& v) {
-    __shared__ double sd[5][NT];
+__device__ __forceinline__ void fused_fin_sp_body(const FusedView<T>& v, double (*sd)[NT]) {
     FusedState* fs = v.fs;
     if (fs->done) return;
     double o[5];
@@ -829,8 +850,7 @@ __device__ __forceinline__ void fused_fin_sp_body(const FusedView<T>& v) {
 }
 
 template <typename T, bool DETECT>
-__device__ __forceinline__ void fused_fin_tp_body(const FusedView<T>& v) {
-    __shared__ double sd[4][NT];
+__device__ __forceinline__ void fused_fin_tp_body(const FusedView<T>& v, double (*sd)[NT]) {
     FusedState* fs = v.fs;
     if (fs->done) return;
     double o[4];
@@ -841,10 +861,16 @@ __device__ __forceinline__ void fused_fin_tp_body(const FusedView<T>& v) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(NT) void k_fused_fin_sp(FusedView<T> v) { fused_fin_sp_body<T>(v); }
+__global__ __launch_bounds__(NT) void k_fused_fin_sp(FusedView<T> v) {
+    __shared__ double sd[5][NT];
+    fused_fin_sp_body<T>(v, sd);
+}
 
 template <typename T, bool DETECT>
-__global__ __launch_bounds__(NT) void k_fused_fin_tp(FusedView<T> v) { fused_fin_tp_body<T, DETECT>(v); }
+__global__ __launch_bounds__(NT) void k_fused_fin_tp(FusedView<T> v) {
+    __shared__ double sd[4][NT];
+    fused_fin_tp_body<T, DETECT>(v, sd);
+}
 
 // FP64 outer residual of the refined solve from Ax (the projected FP64 apply):
 // r64 = b - Ax, inner start e = 0, r32 = (float) r64, z32 = r32 / d32;
@@ -1040,14 +1066,101 @@ __global__ __launch_bounds__(NT) void k_batch_update(const FusedView<T>* __restr
 // one block per solve (solves with few partials skip: their blocks reduce in place)
 template <typename T>
 __global__ __launch_bounds__(NT) void k_batch_fin_sp(const FusedView<T>* __restrict__ views) {
+    __shared__ double sd[5][NT];
     const FusedView<T>& v = views[blockIdx.x];
-    if (v.pre_sp) fused_fin_sp_body<T>(v);
+    if (v.pre_sp) fused_fin_sp_body<T>(v, sd);
 }
 
 template <typename T, bool DETECT>
 __global__ __launch_bounds__(NT) void k_batch_fin_tp(const FusedView<T>* __restrict__ views) {
+    __shared__ double sd[4][NT];
     const FusedView<T>& v = views[blockIdx.x];
-    if (v.pre_tp) fused_fin_tp_body<T, DETECT>(v);
+    if (v.pre_tp) fused_fin_tp_body<T, DETECT>(v, sd);
+}
+
+// The batched iteration as ONE persistent (cooperative) launch: every block
+// stays resident and walks the phases of up to `iters` iterations -- [one-
+// block folds of the super-block partials], the direction pass, the operator
+// walk, [one-block folds of the tile partials], the update -- over all the
+// batch's tiles and super-blocks, a grid-wide barrier between phases where
+// the launches end in the graph-replayed form; after each iteration block 0
+// decides whether every solve is done (flag[it & 1]) and all blocks read that
+// decision after one more barrier, so all leave together.  Same bodies, same
+// order of operations, same bits as the launches of batch_chunk_t.
+template <typename F>
+__device__ __forceinline__ void persist_blocks(int total, F&& f) {
+    // tile t of `total` for this block: each XCD (blocks are dealt round-robin
+    // to the NXCD XCDs; the grid is a multiple of NXCD) works through one
+    // contiguous slab, as xcd_slab_block arranges for a launch of `total` blocks
+    const int per = (total + NXCD - 1) / NXCD;
+    const int xcd = (int)(blockIdx.x % NXCD), lane = (int)(blockIdx.x / NXCD), lanes = (int)(gridDim.x / NXCD);
+    for (int t = lane; t < per; t += lanes) {
+        const int vb = xcd * per + t;
+        if (vb < total) f(vb);
+    }
+}
+
+template <typename T, bool DETECT>
+__global__ __launch_bounds__(NT) void k_batch_persistent(const FusedView<T>* __restrict__ views,
+                                                         const int* __restrict__ tile_off,
+                                                         const int* __restrict__ sb_off, int count, int iters,
+                                                         int pre_sp, int pre_tp, int* __restrict__ flag,
+                                                         int* hflag) {
+    static_assert(FTY * FTZ == NT, "one block shape for every phase");
+    __shared__ TdmaHalf<T, false, FTY, FTZ> H0, H1;
+    __shared__ double sd[5][NT];
+    __shared__ int open;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int total_t = tile_off[count], total_sb = sb_off[count];
+    int all = 0;
+    for (int it = 0; it < iters; ++it) {
+        if (pre_sp) {
+            for (int i = blockIdx.x; i < count; i += gridDim.x) {
+                if (views[i].pre_sp) fused_fin_sp_body<T>(views[i], sd);
+                __syncthreads();
+            }
+            grid.sync();
+        }
+        persist_blocks(total_t, [&](int gb) {
+            const int i = find_solve(tile_off, count, gb);
+            fused_dir_body<T, DETECT, true>(views[i], gb - tile_off[i], sd);
+            __syncthreads();
+        });
+        grid.sync();
+        persist_blocks(total_t, [&](int gb) {
+            const int i = find_solve(tile_off, count, gb);
+            fused_tdma_body<T, DETECT>(views[i], gb - tile_off[i], H0, H1, sd);
+            __builtin_amdgcn_s_waitcnt(0);              // no copy into H0/H1 still in flight
+            __syncthreads();
+        });
+        grid.sync();
+        if (pre_tp) {
+            for (int i = blockIdx.x; i < count; i += gridDim.x) {
+                if (views[i].pre_tp) fused_fin_tp_body<T, DETECT>(views[i], sd);
+                __syncthreads();
+            }
+            grid.sync();
+        }
+        persist_blocks(total_sb, [&](int j) {
+            const int i = find_solve(sb_off, count, j);
+            fused_update_body<T, DETECT, true>(views[i], j - sb_off[i]);
+            __syncthreads();
+        });
+        grid.sync();
+        if (blockIdx.x == 0) {
+            if (threadIdx.x == 0) open = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < count; i += NT)
+                if (!__hip_atomic_load(&views[i].fs->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    atomicOr(&open, 1);
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(&flag[it & 1], open ? 0 : 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        grid.sync();
+        all = __hip_atomic_load(&flag[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (all) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && hflag) __hip_atomic_store(hflag, all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- FP32-refined solves of a batch (each solve as heat_pcg_solve_mixed) ----

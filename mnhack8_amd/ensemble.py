@@ -41,11 +41,13 @@ def members(count=96, seed=1, dt=1e-3, sizes=SIZES, sigmas=SIGMAS):
 
 
 def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log=None, precision="refined",
-        tol_inner=1e-4, max_outer=50):
+        tol_inner=1e-4, max_outer=50, persistent=True):
     """Time the batch (and, for comparison, the same members one after the
     other through the single-solve API) and return a summary.  precision
     "refined": every solve FP32-refined (iteration counts are inner ones);
-    "fp64": plain FP64 PCG."""
+    "fp64": plain FP64 PCG.  persistent: also time the same steps with each
+    batched solve as one persistent (cooperative) launch instead of
+    graph-replayed chunks, and check that T is the same bits."""
     refined = precision == "refined"
     if precision not in ("refined", "fp64"):
         raise ValueError("precision must be 'refined' or 'fp64'")
@@ -94,6 +96,26 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
         "iterations_max_per_step": [int(x) for x in its.max(axis=1)],
         "setup_s": round(setup_s, 3),
     }
+    if persistent:
+        finals = [op.download("T") for op in batch.ops]
+        batch.set_persistent(True)
+        for op, z in zip(batch.ops, zero):
+            op.upload("T", z)
+            op.synchronize()
+        batch_steps(batch, 1)                    # warm-up
+        for op, z in zip(batch.ops, zero):
+            op.upload("T", z)
+            op.synchronize()
+        t0 = time.perf_counter()
+        pits = batch_steps(batch, steps)
+        pers_s = time.perf_counter() - t0
+        same = all(np.array_equal(op.download("T"), f) for op, f in zip(batch.ops, finals))
+        batch.set_persistent(False)
+        out.update({"persistent_ms": round(pers_s * 1e3, 3),
+                    "persistent_matches_batch_bitwise": bool(same and np.array_equal(pits, its)),
+                    "launch_form_kept": "persistent" if pers_s < batch_s else "graph-replayed chunks"})
+        if log:
+            log(f"persistent launch: {pers_s * 1e3:.1f} ms (graph-replayed chunks {batch_s * 1e3:.1f} ms)")
     if sequential:
         finals = [op.download("T") for op in batch.ops]
         t0 = time.perf_counter()
@@ -126,10 +148,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--precision", choices=("refined", "fp64"), default="refined")
+    ap.add_argument("--no-persistent", action="store_true", help="skip the persistent-launch timing")
     a = ap.parse_args(argv)
     log = lambda m: print(f"[ensemble] {m}", file=sys.stderr, flush=True)
     print(json.dumps(run(a.members, a.steps, a.tol, seed=a.seed, sequential=not a.no_sequential, log=log,
-                         precision=a.precision)))
+                         precision=a.precision, persistent=not a.no_persistent)))
 
 
 if __name__ == "__main__":

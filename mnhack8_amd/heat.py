@@ -92,6 +92,7 @@ SIGNATURES = {
     "heat_batch_instance": (_I, [_P, _I, C.POINTER(_P)]),
     "heat_batch_set_active": (_I, [_P, _P]),
     "heat_batch_set_tuning": (_I, [_P, _I, _I]),
+    "heat_batch_set_persistent": (_I, [_P, _I]),
     "heat_batch_step": (_I, [_P, _I, _I, _D, _P]),
     "heat_batch_detect": (_I, [_P, _P, _P, _P, _P]),
     "heat_batch_step_mixed": (_I, [_P, _I, _I, _I, _D, _D, _P]),
@@ -432,6 +433,10 @@ class HeatBatch:
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_batch_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))
+
+    def set_persistent(self, on=True):
+        """Each batched solve as one persistent (cooperative) launch."""
+        _check(self._lib.heat_batch_set_persistent(self.h, int(bool(on))))
 
     def detect(self):
         """Per-member detectors after a step of a batch with detectors:

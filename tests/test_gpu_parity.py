@@ -669,14 +669,17 @@ def _oracle_steps(orc, T, nsteps, max_iter, tol):
     return its, T
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_bitwise_with_oracle(graph):
-    """An ensemble of differently sized and driven solves advanced together:
-    per-member iteration counts and T after each step bit-identical to the
-    oracle's fused solve of that member alone."""
-    print(f"[case] batch graph={graph}", flush=True)
+@pytest.mark.parametrize("mode", ["graph", "eager", "persistent"])
+def test_batch_bitwise_with_oracle(mode):
+    """An ensemble of differently sized and driven solves advanced together
+    (graph-replayed chunks of launches, the same launches one by one, or one
+    persistent cooperative launch per solve): per-member iteration counts
+    and T after each step bit-identical to the oracle's fused solve of that
+    member alone."""
+    print(f"[case] batch {mode}", flush=True)
     batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS)
-    batch.set_tuning(4, graph)
+    batch.set_tuning(4, mode == "graph")
+    batch.set_persistent(mode == "persistent")
     nsteps, max_iter, tol = 3, 300, 1e-10
     its = batch.step(nsteps, max_iter, tol)
     for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
@@ -696,14 +699,15 @@ def _oracle_steps_mixed(orc, T, nsteps, max_outer, max_inner, tol, tol_inner):
     return its, T
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batch_mixed_bitwise_with_oracle(graph):
+@pytest.mark.parametrize("mode", ["graph", "eager", "persistent"])
+def test_batch_mixed_bitwise_with_oracle(mode):
     """FP32-refined steps of the whole ensemble at once: per-member inner
     iteration totals and T after each step bit-identical to the oracle's
     refined solve (fused order) of that member alone."""
-    print(f"[case] refined batch graph={graph}", flush=True)
+    print(f"[case] refined batch {mode}", flush=True)
     batch, orcs, T0s = _batch_and_oracles(BATCH_MEMBERS)
-    batch.set_tuning(4, graph)
+    batch.set_tuning(4, mode == "graph")
+    batch.set_persistent(mode == "persistent")
     nsteps, max_outer, max_inner, tol, tol_inner = 3, 8, 200, 1e-10, 1e-4
     its = batch.step_mixed(nsteps, max_outer, max_inner, tol, tol_inner)
     for i, (op, orc, T0) in enumerate(zip(batch.ops, orcs, T0s)):
@@ -1044,14 +1048,15 @@ def test_stream_ceiling_kernel(n):
     op.cleanup()
 
 
-def test_batch_with_detectors_matches_single_solves():
+@pytest.mark.parametrize("persistent", [False, True])
+def test_batch_with_detectors_matches_single_solves(persistent):
     """A batch of handles built with detectors: each member's step (its first
     residual with the D1 record and any armed level-R fault, then the batched
     iterations with D1 on every apply) gives the same iterations, T bits and
     D1 record (largest ratio, number of checks) as that solve run alone, also
     with a level-R fault armed in one member and a level-G flip in another;
     and heat_batch_detect gives the driver's D1, D2 and D3 values."""
-    print("[case] batch with detectors vs single solves", flush=True)
+    print(f"[case] batch with detectors vs single solves, persistent={persistent}", flush=True)
     h = heat()
     sizes, seeds = [8, 11, 14, 8], [3, 4, 5, 6]
     dt, max_iter, tol = 1e-3, 300, 1e-10
@@ -1064,6 +1069,7 @@ def test_batch_with_detectors_matches_single_solves():
                                       source_value=1.0, detect=True))
         T0s.append(np.random.default_rng(sd).standard_normal((n + 1) ** 3) * 0.1)
     batch = h.HeatBatch(members)
+    batch.set_persistent(persistent)
     for op, s_op, T0 in zip(batch.ops, singles, T0s):
         op.upload("T", T0)
         s_op.upload("T", T0)
