@@ -236,18 +236,27 @@ def main():
     op.set_tile_shape(shape_id)
     opa = HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="atomic", source="const", source_value=1.0)
     opa.upload("p", x)
-    opa.time_matvec("p", "Ap", 3)
-    ms_atomic, _ = opa.time_matvec("p", "Ap", 50)
-    y_at = opa.download("Ap")
+    atomic = {}
+    for impl in ("tiled", "element"):
+        opa.set_atomic_impl(impl)
+        opa.time_matvec("p", "Ap", 3)
+        ms_a, _ = opa.time_matvec("p", "Ap", 100)
+        y_at = opa.download("Ap")
+        atomic[impl] = {"us_per_apply": round(ms_a * 1e3, 2), "gdof_s": round(nn / (ms_a * 1e-3) / 1e9, 2),
+                        "frac": round(alg_bytes / (ms_a * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "max_rel_diff_vs_colour": float(np.max(np.abs(y_at - y_col)) / np.max(np.abs(y_col)))}
     opa.cleanup()
     result["scatter_compare"] = {
         "node_centric_colour_order": {"us_per_apply": round(avg_ms * 1e3, 2), "gdof_s": round(gdofs, 2)},
         "colour_sweep_8_launches": {"us_per_apply": round(ms_sweep * 1e3, 2),
                                     "gdof_s": round(nn / (ms_sweep * 1e-3) / 1e9, 2)},
-        "atomic_scatter": {"us_per_apply": round(ms_atomic * 1e3, 2), "gdof_s": round(nn / (ms_atomic * 1e-3) / 1e9, 2),
-                           "max_rel_diff_vs_colour": float(np.max(np.abs(y_at - y_col)) / np.max(np.abs(y_col)))},
+        "atomic_scatter": dict(atomic["tiled"], kernel="k_apply_scatter (zero fill + element tiles summed in LDS, "
+                                                       "global atomics where tiles meet)"),
+        "atomic_scatter_element_kernel": dict(atomic["element"], kernel="k_apply_elem (one thread per element, "
+                                                                        "8 global atomic adds each)"),
     }
-    log(f"scatter forms: colour sweep {ms_sweep * 1e3:.1f} us, atomic {ms_atomic * 1e3:.1f} us")
+    log(f"scatter forms: colour sweep {ms_sweep * 1e3:.1f} us, atomic tiled {atomic['tiled']['us_per_apply']} us, "
+        f"atomic element kernel {atomic['element']['us_per_apply']} us")
 
     if not a.no_hbm:
         # the same apply where its data cannot stay in the Infinity Cache
@@ -286,6 +295,15 @@ def main():
         op.pcg_solve("b", "T", 5000, 1e-13)
         xref = op.download("T")
         res = {}
+        # each solve on its own autotuned tile geometry (chunking of the PCG tiles)
+        geo = {}
+        for which in ("fused", "one_launch"):
+            planes, wave, ms_it = op.autotune_pcg(which, 20)
+            geo[which] = {"planes": planes, "wave": wave, "chunks_per_column": op.pcg_geometry(which)[2],
+                          "us_per_iteration": round(ms_it * 1e3, 2)}
+            log(f"PCG geometry ({which}): {planes} planes per chunk, wave fill {wave} "
+                f"({ms_it * 1e3:.1f} us per iteration)")
+        res["tile_geometry"] = geo
         runs = (("fp64", "fused"), ("fp32_refined", "fused"),
                 ("fp64_reference_order", "reference"), ("fp32_refined_reference_order", "reference"))
         for label, mode in runs:

@@ -49,6 +49,8 @@ extern "C" {
 #define HEAT_COLORED_MARCH  2   /* threads walk x-columns of nodes with a register window */
 #define HEAT_COLORED_TILE   3   /* blocks walk x with LDS-staged plane tiles */
 #define HEAT_COLORED_TDMA   4   /* as TILE, tiles copied to LDS asynchronously two planes ahead (default) */
+#define HEAT_ATOMIC_ELEMENT 0   /* --scatter atomic: one thread per element, 8 global atomic adds each */
+#define HEAT_ATOMIC_TILED   1   /* element tiles summed in LDS, global atomics only where tiles meet (default) */
 
 /* PCG execution (both run the reference's algorithm; they differ in the fixed
  * order in which inner products are summed) */
@@ -105,6 +107,12 @@ int heat_upload(heat_op* op, double* dev_dst, const double* host_src, int64_t co
 int heat_download(heat_op* op, double* host_dst, const double* dev_src, int64_t count);
 /* Select how --scatter colored is executed (HEAT_COLORED_*). */
 int heat_set_colored_impl(heat_op* op, int impl);
+/* Select how --scatter atomic is executed (new): HEAT_ATOMIC_ELEMENT is the
+ * reference's operator_kernel with colour -1 (src/heat_solver_het.cu:351-378,
+ * 892-893), HEAT_ATOMIC_TILED (default) the same element rows with the
+ * tile-interior sums in LDS.  Both add in an unfixed order, so they agree
+ * with each other and with the colour order to rounding. */
+int heat_set_atomic_impl(heat_op* op, int impl);
 
 /* HeatOperator::apply_operator (src/heat_solver_het.cu:885-921):
  * y = P (rhoc M + dt_eff k K) x, P zeroing Dirichlet rows when project != 0,
@@ -309,6 +317,39 @@ int heat_time_stream(heat_op* op, const double* x, double* y, int reps, double* 
  * mesh with HIP events, keep the fastest for all later applies, and report it
  * (*impl = HEAT_COLORED_*, *planes, *shape, *best_ms per apply). */
 int heat_autotune(heat_op* op, int reps, int* impl, int* planes, int* shape, double* best_ms);
+/* Tile geometry of the PCG kernels (new; the reference's solve has no tiles):
+ * WHICH = 0 the fused iteration (FP64 solves, the standard FP32 inner solve
+ * and batches), 1 the one-launch FP32 inner iteration.  Each (y, z) column of
+ * tiles is split into ceil(planes_in_column / PLANES) chunks of whole plane
+ * pairs, raised until the tiles fill whole waves of WAVE tiles (WAVE = 0: no
+ * fill) but not to chunks shorter than 8 planes.  PLANES = 0 restores the
+ * default (16 planes, waves of 768 tiles).  The chunking fixes the order of
+ * the per-tile inner-product partials, so a solve's bits depend on it (the
+ * oracle takes the same two numbers).  HEAT_E_STATE for a handle whose
+ * geometry a batch holds (heat_batch_set_geometry). */
+int heat_set_pcg_geometry(heat_op* op, int which, int planes, int wave);
+/* The geometry in force: PLANES, WAVE and the resulting chunks per column and
+ * tile count (any pointer may be NULL). */
+int heat_pcg_geometry(const heat_op* op, int which, int* planes, int* wave, int* chunks, int* tiles);
+/* Autotuner of the PCG tile geometry (new): time ITERS iterations of the
+ * solve of geometry WHICH (0: heat_pcg_solve in fused order; 1: one outer
+ * pass of heat_pcg_solve_mixed with the one-launch inner iteration) for
+ * PLANES in {4, 8, 12, 16, 24, 32, 64} and WAVE in {0, 768} on scratch
+ * vectors, keep the fastest and report it (*best_ms: per iteration).  The
+ * work vectors r, z, p and Ap are overwritten; the D1 record is kept. */
+int heat_autotune_pcg(heat_op* op, int which, int iters, int* planes, int* wave, double* best_ms);
+/* The fused geometry of every solve of a batch (new): as
+ * heat_set_pcg_geometry(member, 0, PLANES, WAVE) for each member (for a
+ * shared-mesh batch, the mesh handle), with the batch's tile layout remade. */
+int heat_batch_set_geometry(heat_batch* b, int planes, int wave);
+/* The fused geometry of member I (see heat_pcg_geometry). */
+int heat_batch_geometry(heat_batch* b, int i, int* planes, int* wave, int* chunks, int* tiles);
+/* Autotuner of a batch's geometry (new): time one step of ITERS iterations
+ * (MODE 0: heat_batch_step, 1: heat_batch_step_mixed with one outer pass),
+ * every member starting from its T, for the candidates of
+ * heat_autotune_pcg; keep the fastest (*best_ms: per iteration).  Every T is
+ * restored.  HEAT_E_STATE for a batch with detectors. */
+int heat_batch_autotune(heat_batch* b, int mode, int iters, int* planes, int* wave, double* best_ms);
 /* The D1 threshold (chk_tol, src/heat_solver_het.cu:637) used by the driver. */
 int heat_abft_threshold(heat_op* op, double* chk_tol);
 

@@ -44,9 +44,9 @@ constexpr int C1ER = (C1EN + C1NT - 1) / C1NT;
 constexpr int C1ROWS = 8;                                    // partial rows (see cg1_head)
 
 // chunks of interior planes per (y, z) column: the fused solve's rule on the
-// n - 1 interior planes (a function of the mesh alone)
-__host__ __device__ inline int cg1_chunks(int n, int ncol) {
-    return fused_chunks(n - 1 > 0 ? n - 1 : 1, ncol);
+// n - 1 interior planes (a function of the mesh and the geometry alone)
+__host__ __device__ inline int cg1_chunks(int n, int ncol, int planes = FCX, int wave = FWAVE) {
+    return fused_chunks(n - 1 > 0 ? n - 1 : 1, ncol, planes, wave);
 }
 
 // The pinned status word every step's head stores: the pass (mod 1024), the

@@ -1044,6 +1044,171 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// Atomic scatter by element tiles (the form --scatter atomic runs; the
+// reference's operator_kernel with colour -1, src/heat_solver_het.cu:351-378,
+// is k_apply_elem<.., true> above and stays as the comparison form).
+//
+// A block owns a tile of STY x STZ elements in (y, z) and walks a chunk of
+// element planes in x.  The x values of the tile's node footprint (9 x 33
+// nodes per plane) are staged in LDS, three planes in a ring.  Per element
+// plane, each thread forms its element's eight rows (the closed-form rows of
+// k_apply_elem) and parks them in LDS, one array per local node; then each
+// thread gathers, for its footprint nodes, the four contributions of the
+// tile's elements below and above the node (a fixed order) into register
+// sums.  The node plane below the element plane is then complete within the
+// tile: nodes that no other tile or chunk touches (inside the footprint, not
+// on the chunk's first or last plane) are stored, the others are added to y
+// with one global FP64 atomic add each.  y must be zero on entry; with
+// PROJECT the mesh-boundary nodes are never written (P zeroes them).  The
+// order of the adds at nodes where tiles or chunks meet is not fixed: the
+// result differs from the colour order by rounding only, as the reference's
+// atomic scatter does.
+// ---------------------------------------------------------------------------
+constexpr int STY = 8, STZ = 32, SNT = STY * STZ;             // element tile (y, z)
+constexpr int SFY = STY + 1, SFZ = STZ + 1, SFN = SFY * SFZ;  // node footprint of a plane: 9 x 33
+constexpr int SFR = (SFN + SNT - 1) / SNT;                    // footprint nodes per thread
+constexpr int SCX = 8;                                        // target element planes per chunk
+
+template <bool MASS, bool PROJECT>
+__global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict__ x, double* __restrict__ y,
+                                                      const double* __restrict__ cr,
+                                                      const double* __restrict__ ck, Grid g, int nby,
+                                                      int nbz, int nch, int nblocks, const DevState* st,
+                                                      int check_done) {
+    __shared__ double xs[3][SFN];
+    __shared__ double cb[8][SNT];          // element rows by local node
+    if (check_done && st->done) return;
+    const int b = xcd_slab_block(nblocks);
+    if (b >= nblocks) return;
+    const int bz = b % nbz, by = (b / nbz) % nby, chunk = b / (nbz * nby);
+    const int n = g.n, nn1 = g.nn1;
+    const size_t S = (size_t)g.sx, nsq = (size_t)n * n;
+    const int ex0 = (int)(((long long)chunk * n) / nch);
+    const int ex1 = (int)(((long long)(chunk + 1) * n) / nch);
+    const int tid = threadIdx.x, ty = tid / STZ, tz = tid - (tid / STZ) * STZ;
+    const int y0 = by * STY, z0 = bz * STZ;
+    const int ey = y0 + ty, ez = z0 + tz;
+    const bool elem_ok = ey < n && ez < n;
+    // this thread's footprint nodes (ry, rz): offset in a plane (-1: outside
+    // the mesh or the footprint), whether no other tile touches the node, and
+    // whether it lies on the mesh boundary in y or z
+    int fo[SFR], fr[SFR];
+    bool finner[SFR], fbnd[SFR];
+#pragma unroll
+    for (int j = 0; j < SFR; ++j) {
+        const int i = tid + j * SNT;
+        const int ry = i / SFZ, rz = i - (i / SFZ) * SFZ;
+        const int yy = y0 + ry, zz = z0 + rz;
+        fo[j] = (i < SFN && yy <= n && zz <= n) ? yy * nn1 + zz : -1;
+        fr[j] = i;
+        finner[j] = ry >= 1 && ry < STY && rz >= 1 && rz < STZ;
+        fbnd[j] = yy == 0 || yy == n || zz == 0 || zz == n;
+    }
+    double xr[SFR];
+    auto load_x = [&](int p) {
+#pragma unroll
+        for (int j = 0; j < SFR; ++j) xr[j] = (fo[j] >= 0 && p <= n) ? x[(size_t)p * S + fo[j]] : 0.0;
+    };
+    auto park_x = [&](int slot) {
+#pragma unroll
+        for (int j = 0; j < SFR; ++j)
+            if (fr[j] < SFN) xs[slot][fr[j]] = xr[j];
+    };
+    auto load_e = [&](int e, double& c0, double& c1) {
+        c0 = 0.0;
+        c1 = 0.0;
+        if (elem_ok && e < ex1) {
+            const size_t q = (size_t)e * nsq + (size_t)ey * n + ez;
+            c0 = cr[q];
+            if (!MASS) c1 = ck[q];
+        }
+    };
+    // local node of offsets (a, b, c) in (x, y, z) (inverse of c_off)
+    auto L = [](int a, int bb, int c) { return (c << 2) | (bb ? (a ? 2 : 3) : (a ? 1 : 0)); };
+    // sum of the rows of local node (ox, 1 - a, 1 - c) of the tile elements
+    // (ry - 1 + a, rz - 1 + c) around footprint node i, a and c in order
+    auto gather = [&](int i, int ox) {
+        const int ry = i / SFZ, rz = i - (i / SFZ) * SFZ;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int ety = ry - 1 + a, etz = rz - 1 + c;
+                if (ety >= 0 && ety < STY && etz >= 0 && etz < STZ) v += cb[L(ox, 1 - a, 1 - c)][ety * STZ + etz];
+            }
+        return v;
+    };
+    // node plane p is complete in v[]: store the nodes only this block
+    // touches, add the others
+    auto flush = [&](int p, const double (&v)[SFR]) {
+        const bool plane_inner = p > ex0 && p < ex1;
+        const bool plane_bnd = p == 0 || p == n;
+        double* yp = y + (size_t)p * S;
+#pragma unroll
+        for (int j = 0; j < SFR; ++j) {
+            if (fo[j] < 0 || (PROJECT && (plane_bnd || fbnd[j]))) continue;
+            if (plane_inner && finner[j]) yp[fo[j]] = v[j];
+            else unsafeAtomicAdd(&yp[fo[j]], v[j]);
+        }
+    };
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        load_x(ex0 + p);
+        park_x((ex0 + p) % 3);
+    }
+    load_x(ex0 + 3);
+    double c_r, c_k;
+    load_e(ex0, c_r, c_k);
+    double up[SFR];                              // node plane ex: the rows of element plane ex - 1
+#pragma unroll
+    for (int j = 0; j < SFR; ++j) up[j] = 0.0;
+    __syncthreads();
+    const int base = ty * SFZ + tz;
+    for (int ex = ex0; ex < ex1; ++ex) {
+        double n_r, n_k;
+        load_e(ex + 1, n_r, n_k);                  // in flight during this plane
+        {
+            const double* xl = xs[ex % 3];
+            const double* xh = xs[(ex + 1) % 3];
+            double xe[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const double* xp = c_off[i][0] ? xh : xl;
+                xe[i] = xp[base + c_off[i][1] * SFZ + c_off[i][2]];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int ox = c_off[i][0], oy = c_off[i][1], oz = c_off[i][2];
+                const double S0 = xe[i];
+                const double S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
+                const double S2 = (xe[L(1 - ox, 1 - oy, oz)] + xe[L(1 - ox, oy, 1 - oz)]) + xe[L(ox, 1 - oy, 1 - oz)];
+                const double S3 = xe[L(1 - ox, 1 - oy, 1 - oz)];
+                // elements outside the mesh have zero coefficients: zero rows
+                cb[i][tid] = struct_row(S0, S1, S2, S3, c_r, MASS ? 0.0 : c_k);
+            }
+        }
+        __syncthreads();
+        double v[SFR];
+#pragma unroll
+        for (int j = 0; j < SFR; ++j) {
+            v[j] = 0.0;
+            if (fr[j] < SFN) {
+                v[j] = up[j] + gather(fr[j], 0);   // node plane ex is complete within the tile
+                up[j] = gather(fr[j], 1);          // node plane ex + 1: element plane ex so far
+            }
+        }
+        flush(ex, v);
+        park_x(ex % 3);                            // plane ex + 3 replaces plane ex
+        load_x(ex + 4);
+        c_r = n_r;
+        c_k = n_k;
+        __syncthreads();
+    }
+    flush(ex1, up);                                // the chunk's last node plane (shared with the next chunk)
+}
+
 // Projection and the inner products of an element-centric apply
 // (the tail of apply_operator, src/heat_solver_het.cu:904-919).
 template <typename T, int EPI>
@@ -1236,6 +1401,13 @@ __global__ __launch_bounds__(NT) void k_mixed_correct(double* __restrict__ x,
 // ---------------------------------------------------------------------------
 // Setup and utility kernels
 // ---------------------------------------------------------------------------
+// y = 0 before a scatter inside a captured PCG (returns once the solve is done)
+__global__ __launch_bounds__(NT) void k_zero_unless_done(double* __restrict__ v, int n, const DevState* st) {
+    if (st->done) return;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i < n) v[i] = 0.0;
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ v, int n, T val) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;

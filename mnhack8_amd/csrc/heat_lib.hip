@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstring>
 #include <mutex>
@@ -70,6 +71,7 @@ struct heat_op {
     Grid g;
     hipStream_t stream = nullptr;
     int colored_impl = HEAT_COLORED_TDMA;
+    int atomic_impl = HEAT_ATOMIC_TILED;   // --scatter atomic: tiled scatter, or the element kernel
     int march_cx = 8;
     int tdma_res[2][2][3] = {};      // resident blocks of k_apply_tdma [float][mass][tile shape]
     int tile_shape = 0;
@@ -113,6 +115,11 @@ struct heat_op {
     double* fsp = nullptr;        // super-block partials, 5 x nsb
     double* ftp = nullptr;        // fused-apply tile partials, 4 x ntiles
     int nsb = 0, fnby = 0, fnbz = 0, fnch = 0, fntiles = 0;
+    // tile geometry of the PCG kernels: target planes per chunk and the
+    // whole-wave fill (fused_chunks) of [0] the fused iteration and [1] the
+    // one-launch inner iteration; the partial arrays hold every chunk count
+    int geo_planes[2] = {FCX, FCX}, geo_wave[2] = {FWAVE, FWAVE};
+    int batch_refs = 0;            // batches whose views hold this handle's geometry
     // the one-launch inner iteration of the refined solve (heat_cg1.h)
     int inner_mode = HEAT_INNER_ONE_LAUNCH;
     float *c1_r = nullptr, *c1_s[2] = {nullptr, nullptr}, *c1_w[2] = {nullptr, nullptr};
@@ -187,6 +194,15 @@ bool gather_path(const heat_op* op) {
     return op->cfg.kernel == HEAT_KERNEL_FAST && op->cfg.scatter == HEAT_SCATTER_COLORED &&
            op->colored_impl != HEAT_COLORED_SWEEP;
 }
+
+// the tiled atomic scatter of the fast rows (k_apply_scatter)
+bool tiled_atomic_path(const heat_op* op) {
+    return op->cfg.kernel == HEAT_KERNEL_FAST && op->cfg.scatter == HEAT_SCATTER_ATOMIC &&
+           op->atomic_impl == HEAT_ATOMIC_TILED;
+}
+
+// PCG iterations of these paths run in captured chunks with no host check
+bool chunked_path(const heat_op* op) { return gather_path(op) || tiled_atomic_path(op); }
 
 // Resident blocks of one kernel on the whole device (CUs x blocks per CU).
 int resident_blocks(const void* kernel, int threads) {
@@ -321,14 +337,43 @@ int launch_colored(heat_op* op, const T* x, T* y, const T* cr, const T* ck, bool
     return HEAT_OK;
 }
 
-// y = A x through the element-centric kernels (colour sweep or atomic),
-// without projection.
-int element_apply(heat_op* op, const double* x, double* y, double dt_eff, FiArm f) {
+// Chunks of element planes per tile column of the tiled atomic scatter:
+// about SCX planes each, never more chunks than planes.
+inline int scatter_chunks(int n) { return std::max(1, std::min(n, (n + SCX - 1) / SCX)); }
+
+// y = A x through the element-centric kernels (colour sweep or atomic).
+// *projected (optional) is set when the kernel has applied P itself (the
+// tiled atomic scatter, asked with project); otherwise y is unprojected.
+int element_apply(heat_op* op, const double* x, double* y, double dt_eff, FiArm f, bool project = false,
+                  bool* projected = nullptr, int check_done = 0) {
     const Grid& g = op->g;
     hipStream_t s = op->stream;
-    k_fill<double><<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, 0.0);
+    if (projected) *projected = false;
+    if (check_done) k_zero_unless_done<<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, op->st);
+    else k_fill<double><<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, 0.0);
     CHECK_LAUNCH();
     const bool gauss = op->cfg.kernel == HEAT_KERNEL_GAUSS;
+    if (tiled_atomic_path(op) && !f.armed) {
+        // the tiled scatter (an armed level-R fault goes through the
+        // element kernel, which carries the flip)
+        const int nby = (g.n + STY - 1) / STY, nbz = (g.n + STZ - 1) / STZ, nch = scatter_chunks(g.n);
+        const int nblocks = nby * nbz * nch;
+        const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
+        const bool mass = dt_eff == 0.0;
+        const DevState* st = op->st;
+        if (mass && project)
+            k_apply_scatter<true, true><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        else if (mass)
+            k_apply_scatter<true, false><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        else if (project)
+            k_apply_scatter<false, true><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        else
+            k_apply_scatter<false, false><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        CHECK_LAUNCH();
+        if (projected) *projected = project;
+        return HEAT_OK;
+    }
+    if (check_done) return fail(HEAT_E_STATE, "internal: element path inside a captured solve");
     if (op->cfg.scatter == HEAT_SCATTER_ATOMIC) {
         if (gauss)
             k_apply_elem<double, 1, true><<<blocks(g.ne, ET), ET, 0, s>>>(x, y, op->kcell, op->rhoc, g, dt_eff, op->m3, op->k2, f, -1);
@@ -396,20 +441,21 @@ int apply_impl(heat_op* op, const double* x, double* y, double dt_eff, bool allo
             if (rc0) return rc0;
         }
     } else {
-        if (check_done) return fail(HEAT_E_STATE, "internal: element path inside a captured solve");
-        int rc = element_apply(op, x, y, dt_eff, f);
+        bool projected = false;
+        int rc = element_apply(op, x, y, dt_eff, f, project, &projected, check_done);
         if (rc) return rc;
-        const int pj = project ? 1 : 0;
+        const int pj = (project && !projected) ? 1 : 0;
+        const int cd = check_done;
         switch (epi) {
             case EPI_NONE:
-                if (project) k_apply_post<double, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0);
+                if (pj) k_apply_post<double, EPI_NONE><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, cd);
                 break;
-            case EPI_D1: k_apply_post<double, EPI_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
-            case EPI_PCG: k_apply_post<double, EPI_PCG><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
-            default: k_apply_post<double, EPI_PCG_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, 0); break;
+            case EPI_D1: k_apply_post<double, EPI_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, cd); break;
+            case EPI_PCG: k_apply_post<double, EPI_PCG><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, cd); break;
+            default: k_apply_post<double, EPI_PCG_D1><<<op->nbn, NT, 0, s>>>(x, y, g, pj, w, nw, op->partials, op->st, cd); break;
         }
         CHECK_LAUNCH();
-        rc0 = launch_fin_apply(op, epi, nw, 0);
+        rc0 = launch_fin_apply(op, epi, nw, cd);
         if (rc0) return rc0;
     }
     if (allow_fi && op->fi.armed && op->fi_level == 2) op->fi.armed = 0;   // one-shot (:902)
@@ -576,9 +622,10 @@ int launch_chunk_generic(heat_op* op, const void* const key[6], Body body) {
 int launch_chunk(heat_op* op, double* x, bool detect) {
     const void* key6[6] = {x, (const void*)(intptr_t)((detect ? 1 : 0) + 2 * op->pcg_fi.armed),
                            (const void*)(intptr_t)op->pcg_chunk,
-                           op->cur_hist, (const void*)(intptr_t)(op->pcg_mode + 10 * op->colored_impl),
+                           op->cur_hist,
+                           (const void*)(intptr_t)(op->pcg_mode + 10 * op->colored_impl + 100 * op->atomic_impl),
                            (const void*)(intptr_t)(op->march_cx + 100 * op->tile_shape)};
-    if (op->pcg_mode == HEAT_PCG_FUSED) {
+    if (op->pcg_mode == HEAT_PCG_FUSED && gather_path(op)) {
         return launch_chunk_generic(op, key6, [&]() {
             const FusedView<double> v = make_view<double>(op, x, op->vec[HEAT_VEC_P], op->p2, op->vec[HEAT_VEC_R],
                                                           op->vec[HEAT_VEC_Z], op->vec[HEAT_VEC_AP], nullptr,
@@ -694,11 +741,12 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
     op->nsb = (int)((g.nn + SB - 1) / SB);
     op->fnby = (n + FTY - 1) / FTY;
     op->fnbz = (n + FTZ - 1) / FTZ;
-    op->fnch = fused_chunks(g.nn1, op->fnby * op->fnbz);
+    op->fnch = fused_chunks(g.nn1, op->fnby * op->fnbz, op->geo_planes[0], op->geo_wave[0]);
     op->fntiles = op->fnby * op->fnbz * op->fnch;
     HIP_TRY(hipMalloc(&op->fsp, sizeof(double) * 5 * (size_t)op->nsb));
     HIP_TRY(hipMalloc(&op->p2, sizeof(double) * (size_t)g.nn));
-    HIP_TRY(hipMalloc(&op->ftp, sizeof(double) * 4 * (size_t)op->fntiles));
+    // room for the partials of every geometry (at most one chunk per plane pair)
+    HIP_TRY(hipMalloc(&op->ftp, sizeof(double) * 4 * (size_t)op->fnby * op->fnbz * fused_chunks_max(g.nn1)));
     HIP_TRY(hipMalloc(&op->fst, sizeof(FusedState)));
     HIP_TRY(hipHostMalloc(&op->hfst, sizeof(FusedState), hipHostMallocDefault));
     memset(op->hfst, 0, sizeof(FusedState));
@@ -837,14 +885,15 @@ int ensure_cg1(heat_op* op) {
     const int m = g.n - 1 > 0 ? g.n - 1 : 1;
     op->c1_nby = (m + C1TY - 1) / C1TY;
     op->c1_nbz = (m + C1TZ - 1) / C1TZ;
-    op->c1_nch = cg1_chunks(g.n, op->c1_nby * op->c1_nbz);
+    op->c1_nch = cg1_chunks(g.n, op->c1_nby * op->c1_nbz, op->geo_planes[1], op->geo_wave[1]);
     op->c1_ntiles = op->c1_nby * op->c1_nbz * op->c1_nch;
     HIP_TRY(hipMalloc(&op->c1_r, nb4));
     for (int q = 0; q < 2; ++q) {
         HIP_TRY(hipMalloc(&op->c1_s[q], nb4));
         HIP_TRY(hipMalloc(&op->c1_w[q], nb4));
     }
-    HIP_TRY(hipMalloc(&op->c1_tp, sizeof(double) * 2 * C1ROWS * (size_t)op->c1_ntiles));
+    // room for the partials of every geometry (at most one chunk per plane pair)
+    HIP_TRY(hipMalloc(&op->c1_tp, sizeof(double) * 2 * C1ROWS * (size_t)op->c1_nby * op->c1_nbz * fused_chunks_max(m)));
     HIP_TRY(hipMalloc(&op->c1_dinv, nb4));
     k_recip<<<blocks(g.nn, NT), NT, 0, s>>>(op->c1_dinv, op->d32, g.nn);
     CHECK_LAUNCH();
@@ -1028,6 +1077,12 @@ int heat_set_colored_impl(heat_op* op, int impl) {
     return HEAT_OK;
 }
 
+int heat_set_atomic_impl(heat_op* op, int impl) {
+    if (!op || (impl != HEAT_ATOMIC_ELEMENT && impl != HEAT_ATOMIC_TILED)) return fail(HEAT_E_ARG, "bad atomic impl");
+    op->atomic_impl = impl;
+    return HEAT_OK;
+}
+
 int heat_apply_operator(heat_op* op, const double* x, double* y, double dt_eff, int allow_fi, int project) {
     if (!op || !x || !y) return fail(HEAT_E_ARG, "null argument");
     const int epi = (op->cfg.detect && project) ? EPI_D1 : EPI_NONE;
@@ -1201,7 +1256,7 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
     CHECK_LAUNCH();
     k_fin_init<<<1, FT, 0, s>>>(op->partials, (int)op->nbn, op->st, op->cur_hist);
     CHECK_LAUNCH();
-    if (gather_path(op)) {
+    if (chunked_path(op)) {
         rc = run_until_done(op, max_iter, [&]() { return launch_chunk(op, x, detect); });
     } else {
         // element-centric paths: iterate with a host check per iteration
@@ -1698,6 +1753,135 @@ int heat_set_tile_shape(heat_op* op, int shape) {
     return HEAT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Set geometry `which` (0 the fused iteration, 1 the one-launch inner
+// iteration) of a handle; a captured chunk holds the old views and is dropped.
+int set_geometry(heat_op* op, int which, int planes, int wave) {
+    if (planes == 0) { planes = FCX; wave = FWAVE; }
+    if (planes < 2 || planes > (1 << 20) || wave < 0 || wave > (1 << 20))
+        return fail(HEAT_E_ARG, "geometry: planes must be in 2..2^20 (0: the default) and wave in 0..2^20");
+    op->geo_planes[which] = planes;
+    op->geo_wave[which] = wave;
+    if (which == 0) {
+        op->fnch = fused_chunks(op->g.nn1, op->fnby * op->fnbz, planes, wave);
+        op->fntiles = op->fnby * op->fnbz * op->fnch;
+    } else if (op->c1_r) {
+        op->c1_nch = cg1_chunks(op->g.n, op->c1_nby * op->c1_nbz, planes, wave);
+        op->c1_ntiles = op->c1_nby * op->c1_nbz * op->c1_nch;
+    }
+    if (op->gexec) { (void)hipGraphExecDestroy(op->gexec); op->gexec = nullptr; }
+    if (op->graph) { (void)hipGraphDestroy(op->graph); op->graph = nullptr; }
+    return HEAT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int heat_set_pcg_geometry(heat_op* op, int which, int planes, int wave) {
+    if (!op || which < 0 || which > 1) return fail(HEAT_E_ARG, "geometry 0 (fused) or 1 (one-launch inner)");
+    if (op->batch_refs > 0)
+        return fail(HEAT_E_STATE, "the handle's geometry is held by a batch (heat_batch_set_geometry sets it)");
+    return set_geometry(op, which, planes, wave);
+}
+
+int heat_pcg_geometry(const heat_op* op, int which, int* planes, int* wave, int* chunks, int* tiles) {
+    if (!op || which < 0 || which > 1) return fail(HEAT_E_ARG, "geometry 0 (fused) or 1 (one-launch inner)");
+    if (planes) *planes = op->geo_planes[which];
+    if (wave) *wave = op->geo_wave[which];
+    int nch, nt;
+    if (which == 0) {
+        nch = op->fnch;
+        nt = op->fntiles;
+    } else {
+        const int m = op->g.n - 1 > 0 ? op->g.n - 1 : 1;
+        const int ncol = ((m + C1TY - 1) / C1TY) * ((m + C1TZ - 1) / C1TZ);
+        nch = cg1_chunks(op->g.n, ncol, op->geo_planes[1], op->geo_wave[1]);
+        nt = ncol * nch;
+    }
+    if (chunks) *chunks = nch;
+    if (tiles) *tiles = nt;
+    return HEAT_OK;
+}
+
+// Time `iters` PCG iterations for every candidate geometry and keep the
+// fastest: which = 0 the fused FP64 iteration (heat_pcg_solve), 1 the
+// one-launch FP32 inner iteration (one pass of heat_pcg_solve_mixed).  The
+// solves run on scratch vectors (right-hand side 1 at interior nodes, x from
+// 0, tolerance 0); the handle's work vectors r, z, p, Ap are overwritten, its
+// D1 record is restored.  Candidates giving the same chunk count are timed once.
+int heat_autotune_pcg(heat_op* op, int which, int iters, int* planes, int* wave, double* best_ms) {
+    if (!op || which < 0 || which > 1 || iters < 1) return fail(HEAT_E_ARG, "bad PCG autotune request");
+    if (op->batch_refs > 0)
+        return fail(HEAT_E_STATE, "the handle's geometry is held by a batch (heat_batch_autotune tunes it)");
+    if (!gather_path(op)) return fail(HEAT_E_STATE, "PCG geometry applies to the colour-ordered fast path only");
+    if (op->pcg_fi.armed || op->fi.armed) return fail(HEAT_E_STATE, "a fault is armed: tune before arming it");
+    if (which == 1) {
+        int rc = ensure_cg1(op);
+        if (rc) return rc;
+    }
+    const Grid& g = op->g;
+    hipStream_t s = op->stream;
+    double *bv = nullptr, *xv = nullptr;
+    HIP_TRY(hipMalloc(&bv, sizeof(double) * (size_t)g.nn));
+    if (hipMalloc(&xv, sizeof(double) * (size_t)g.nn) != hipSuccess) {
+        (void)hipFree(bv);
+        return fail(HEAT_E_HIP, "autotune: out of device memory");
+    }
+    k_ones_interior<<<op->nbn, NT, 0, s>>>(bv, g);
+    int rc = read_state(op);                       // the D1 record, restored at the end
+    DevState saved = *op->hst;
+    const int keep_mode = op->pcg_mode, keep_inner = op->inner_mode;
+    const int keep_planes = op->geo_planes[which], keep_wave = op->geo_wave[which];
+    op->pcg_mode = HEAT_PCG_FUSED;
+    op->inner_mode = HEAT_INNER_ONE_LAUNCH;
+    const int cand_planes[] = {4, 8, 12, 16, 24, 32, 64};
+    const int cand_wave[] = {0, FWAVE};
+    int pick_p = keep_planes, pick_w = keep_wave;
+    double best = 1e300;
+    std::vector<int> seen;
+    for (int cp : cand_planes) {
+        for (int cw : cand_wave) {
+            if (rc) break;
+            const int nch = which == 0 ? fused_chunks(g.nn1, op->fnby * op->fnbz, cp, cw)
+                                       : cg1_chunks(g.n, op->c1_nby * op->c1_nbz, cp, cw);
+            if (std::find(seen.begin(), seen.end(), nch) != seen.end()) continue;
+            seen.push_back(nch);
+            rc = set_geometry(op, which, cp, cw);
+            double t_best = 1e300;
+            for (int rep = 0; rep < 3 && rc == HEAT_OK; ++rep) {
+                k_fill<double><<<op->nbn, NT, 0, s>>>(xv, g.nn, 0.0);
+                if (hipStreamSynchronize(s) != hipSuccess) { rc = fail(HEAT_E_HIP, "autotune: stream error"); break; }
+                const auto t0 = std::chrono::steady_clock::now();
+                int it = 0, outer = 0;
+                rc = which == 0 ? heat_pcg_solve(op, bv, xv, iters, 0.0, &it, nullptr)
+                                : heat_pcg_solve_mixed(op, bv, xv, 1, iters, 0.0, 0.0, &outer, &it);
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (ms < t_best) t_best = ms;
+            }
+            if (rc == HEAT_OK && t_best < best) { best = t_best; pick_p = cp; pick_w = cw; }
+        }
+    }
+    op->pcg_mode = keep_mode;
+    op->inner_mode = keep_inner;
+    const int rc2 = set_geometry(op, which, rc ? keep_planes : pick_p, rc ? keep_wave : pick_w);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(bv);
+    (void)hipFree(xv);
+    *op->hst = saved;
+    if (rc == HEAT_OK) rc = rc2;
+    if (rc == HEAT_OK && hipMemcpy(op->st, op->hst, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HEAT_E_HIP, "autotune: could not restore the D1 record");
+    if (rc) return rc;
+    if (planes) *planes = pick_p;
+    if (wave) *wave = pick_w;
+    if (best_ms) *best_ms = best / iters;
+    return HEAT_OK;
+}
+
 int heat_set_tuning(heat_op* op, int pcg_chunk, int use_graph) {
     if (!op || pcg_chunk < 1 || pcg_chunk > 256) return fail(HEAT_E_ARG, "pcg_chunk must be in [1, 256]");
     op->pcg_chunk = pcg_chunk;
@@ -1786,6 +1970,7 @@ void batch_free(heat_batch* b) {
     if (b->stream) (void)hipStreamDestroy(b->stream);
     for (heat_op* op : b->ops) heat_cleanup(op);
     for (void* p : b->owned) (void)hipFree(p);
+    if (b->mesh) --b->mesh->batch_refs;
 }
 
 // device tables and partial arrays of a batch whose views are filled in
@@ -2012,6 +2197,7 @@ int heat_batch_setup(int count, const heat_config* cfgs, const double* const* k,
             snprintf(msg, sizeof msg, "%s", heat_last_error());
             return bail(fail(rc, "batch solve %d: %s", i, msg));
         }
+        op->batch_refs = 1;                      // its geometry is the batch's (heat_batch_set_geometry)
         b->ops.push_back(op);
         HIP_TRY_OR(hipStreamSynchronize(op->stream), return bail(HEAT_E_HIP));
     }
@@ -2048,6 +2234,7 @@ int heat_batch_setup_rhs(heat_op* op, int count, const double* F, heat_batch** o
     // check, heat_batch_step_mrhs applies it with the handle's w_A)
     heat_batch* b = new heat_batch();
     b->mesh = op;
+    ++op->batch_refs;                            // its fused geometry is held by the batch's views
     auto bail = [&](int rc) { batch_free(b); delete b; return rc; };
     const size_t nn = (size_t)op->g.nn;
     if ((long long)nn * count > (1LL << 30)) return bail(fail(HEAT_E_ARG, "batch too large"));
@@ -2132,6 +2319,140 @@ int heat_batch_set_tuning(heat_batch* b, int pcg_chunk, int use_graph) {
 int heat_batch_set_persistent(heat_batch* b, int on) {
     if (!b) return fail(HEAT_E_ARG, "null argument");
     b->persistent = on ? 1 : 0;
+    return HEAT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Lay the batch's tiles out again after its members' fused geometry changed:
+// tile offsets, the shared tile-partial rows, every view (FP64 and FP32) and
+// the captured chunks.
+int batch_relayout(heat_batch* b) {
+    const int count = (int)b->views.size();
+    for (int i = 0; i < count; ++i) {
+        const heat_op* op = b->mesh ? b->mesh : b->ops[i];
+        b->tile_off[i + 1] = b->tile_off[i] + op->fntiles;
+    }
+    const int total_tiles = b->tile_off[count];
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipFree(b->tp));
+    b->tp = nullptr;
+    HIP_TRY(hipMalloc(&b->tp, 4 * sizeof(double) * (size_t)total_tiles));
+    for (int i = 0; i < count; ++i) {
+        const heat_op* op = b->mesh ? b->mesh : b->ops[i];
+        auto relay = [&](auto& v) {
+            v.tp = b->tp + b->tile_off[i];
+            v.tp_stride = total_tiles;
+            v.ntiles = op->fntiles;
+            v.nch = op->fnch;
+            v.pre_tp = v.ntiles > FUSED_INBLOCK_MAX;
+        };
+        relay(b->views[i]);
+        if (!b->views32.empty()) relay(b->views32[i]);
+    }
+    HIP_TRY(hipMemcpy(b->d_tile_off, b->tile_off.data(), (count + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    if (b->gexec32) { (void)hipGraphExecDestroy(b->gexec32); b->gexec32 = nullptr; }
+    if (b->graph32) { (void)hipGraphDestroy(b->graph32); b->graph32 = nullptr; }
+    return batch_upload_views(b);
+}
+
+int batch_set_geometry(heat_batch* b, int planes, int wave) {
+    if (b->mesh) {
+        int rc = set_geometry(b->mesh, 0, planes, wave);
+        if (rc) return rc;
+    } else {
+        for (heat_op* op : b->ops) {
+            int rc = set_geometry(op, 0, planes, wave);
+            if (rc) return rc;
+        }
+    }
+    return batch_relayout(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int heat_batch_set_geometry(heat_batch* b, int planes, int wave) {
+    if (!b) return fail(HEAT_E_ARG, "null argument");
+    return batch_set_geometry(b, planes, wave);
+}
+
+int heat_batch_geometry(heat_batch* b, int i, int* planes, int* wave, int* chunks, int* tiles) {
+    if (!b || i < 0 || i >= (int)b->views.size()) return fail(HEAT_E_ARG, "bad batch member");
+    const heat_op* op = b->mesh ? b->mesh : b->ops[i];
+    return heat_pcg_geometry(op, 0, planes, wave, chunks, tiles);
+}
+
+// Time one batched step of `iters` iterations (mode 0: FP64 steps, 1: refined
+// steps with one outer pass) for every candidate geometry, every member
+// starting from the T it holds, and keep the fastest.  Every member's T is
+// restored afterwards.  Batches with detectors are not tuned (their D1
+// records would change).
+int heat_batch_autotune(heat_batch* b, int mode, int iters, int* planes, int* wave, double* best_ms) {
+    if (!b || mode < 0 || mode > 1 || iters < 1) return fail(HEAT_E_ARG, "bad batch autotune request");
+    if (b->detect) return fail(HEAT_E_STATE, "a batch with detectors is not tuned (tune one without)");
+    if (mode == 1 && b->mesh) return fail(HEAT_E_STATE, "refined batch steps need a batch of handles");
+    const int count = (int)b->views.size();
+    const size_t total = (size_t)b->node_off[count];
+    double* saved = nullptr;
+    HIP_TRY(hipMalloc(&saved, sizeof(double) * total));
+    hipStream_t s = b->stream;
+    auto copy_T = [&](bool save) -> int {
+        for (int i = 0; i < count; ++i) {
+            double* T = b->views[i].x;
+            double* keep = saved + b->node_off[i];
+            const size_t nb = sizeof(double) * (size_t)(b->node_off[i + 1] - b->node_off[i]);
+            HIP_TRY(hipMemcpyAsync(save ? keep : T, save ? T : keep, nb, hipMemcpyDeviceToDevice, s));
+        }
+        HIP_TRY(hipStreamSynchronize(s));
+        return HEAT_OK;
+    };
+    const heat_op* op0 = b->mesh ? b->mesh : b->ops[0];
+    const int keep_p = op0->geo_planes[0], keep_w = op0->geo_wave[0];
+    int rc = copy_T(true);
+    const int cand_planes[] = {4, 8, 12, 16, 24, 32, 64};
+    const int cand_wave[] = {0, FWAVE};
+    std::vector<int> its(count);
+    std::vector<std::vector<int>> seen;
+    int pick_p = keep_p, pick_w = keep_w;
+    double best = 1e300;
+    for (int cp : cand_planes) {
+        for (int cw : cand_wave) {
+            if (rc) break;
+            std::vector<int> key;
+            for (int i = 0; i < count; ++i) {
+                const heat_op* op = b->mesh ? b->mesh : b->ops[i];
+                key.push_back(fused_chunks(op->g.nn1, op->fnby * op->fnbz, cp, cw));
+            }
+            if (std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+            seen.push_back(key);
+            rc = batch_set_geometry(b, cp, cw);
+            double t_best = 1e300;
+            for (int rep = 0; rep < 2 && rc == HEAT_OK; ++rep) {
+                rc = copy_T(false);
+                if (rc) break;
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = mode == 0 ? heat_batch_step(b, 1, iters, 0.0, its.data())
+                               : heat_batch_step_mixed(b, 1, 1, iters, 0.0, 0.0, its.data());
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (ms < t_best) t_best = ms;
+            }
+            if (rc == HEAT_OK && t_best < best) { best = t_best; pick_p = cp; pick_w = cw; }
+        }
+    }
+    int rc2 = batch_set_geometry(b, rc ? keep_p : pick_p, rc ? keep_w : pick_w);
+    if (rc2 == HEAT_OK) rc2 = copy_T(false);
+    (void)hipFree(saved);
+    if (rc == HEAT_OK) rc = rc2;
+    if (rc) return rc;
+    if (planes) *planes = pick_p;
+    if (wave) *wave = pick_w;
+    if (best_ms) *best_ms = best / iters;
     return HEAT_OK;
 }
 

@@ -186,15 +186,28 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
                     c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, zero, 0, 0, 0);
                     c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[kzi], 0, 0, 0);
                 }
+                // the run's accumulator words are distinct (its elements share
+                // no node): read them all, add, then write them all, so the
+                // reads wait once instead of once per element
+                float la[NE], lb[NE];
+#pragma unroll
+                for (int kzi = 0; kzi < NE; ++kzi) {
+                    la[kzi] = ylo[o_row + 2 * kzi * MR];
+                    lb[kzi] = yhi[o_row + 2 * kzi * MR];
+                }
 #pragma unroll
                 for (int kzi = 0; kzi < NE; ++kzi) {
                     const float ce_r = er_row[2 * kzi];
                     const float ce_k = ek_row[2 * kzi];
                     const float ya = MASS ? ce_r * c[kzi][0] : fmaf(ce_r, c[kzi][0], ce_k * c[kzi][2]);
                     const float yb = MASS ? ce_r * c[kzi][1] : fmaf(ce_r, c[kzi][1], ce_k * c[kzi][3]);
-                    const int o = o_row + 2 * kzi * MR;
-                    ylo[o] += ya;
-                    yhi[o] += yb;
+                    la[kzi] += ya;
+                    lb[kzi] += yb;
+                }
+#pragma unroll
+                for (int kzi = 0; kzi < NE; ++kzi) {
+                    ylo[o_row + 2 * kzi * MR] = la[kzi];
+                    yhi[o_row + 2 * kzi * MR] = lb[kzi];
                 }
             };
 #pragma unroll 1

@@ -45,20 +45,26 @@ constexpr int FCX = 16;                       // target node planes per tile
 constexpr int FCX_MIN = 8;                    // shortest chunk the wave fill may make
 constexpr int FWAVE = 768;                    // tiles per wave: 256 CUs x 3 resident blocks
 
-// Chunks per (y,z) column of the fused apply: ceil(nn1 / FCX), raised until
-// the tiles fill whole waves of FWAVE, but never to chunks shorter than
-// FCX_MIN planes (small meshes, whose tiles a batch launches together, keep
-// long chunks).  The chunks split the column's plane pairs as evenly as
-// possible (at most one pair apart; the result never exceeds the pair
-// count).  A function of the mesh alone, so the partial sums' order (and
-// every bit of the solve) does not depend on the device.
-__host__ __device__ inline int fused_chunks(int nn1, int ncol) {
-    const int nch = (nn1 + FCX - 1) / FCX;
-    const long long waves = ((long long)ncol * nch + FWAVE - 1) / FWAVE;
-    long long fill = waves * FWAVE / ncol;
+// Chunks per (y,z) column of the fused apply: ceil(nn1 / planes), raised
+// until the tiles fill whole waves of `wave` tiles (wave = 0: no fill), but
+// never to chunks shorter than FCX_MIN planes (small meshes, whose tiles a
+// batch launches together, keep long chunks).  The chunks split the
+// column's plane pairs as evenly as possible (at most one pair apart; the
+// result never exceeds the pair count for planes >= 2).  A function of the
+// mesh and of (planes, wave) alone, so the partial sums' order (and every
+// bit of the solve) does not depend on the device.  The default geometry is
+// (FCX, FWAVE); an autotuner may pick another pair, and the oracle takes
+// the same pair.
+__host__ __device__ inline int fused_chunks(int nn1, int ncol, int planes = FCX, int wave = FWAVE) {
+    const int nch = (nn1 + planes - 1) / planes;
+    if (wave <= 0) return nch;
+    const long long waves = ((long long)ncol * nch + wave - 1) / wave;
+    long long fill = waves * wave / ncol;
     if (fill > nn1 / FCX_MIN) fill = nn1 / FCX_MIN;
     return fill > nch ? (int)fill : nch;
 }
+// the largest chunk count fused_chunks can return for nn1 planes (planes >= 2)
+__host__ __device__ inline int fused_chunks_max(int nn1) { return (nn1 + 1) / 2; }
 constexpr int SB = 2048;                      // nodes per update super-block
 constexpr int SBS = SB / NT;                  // strided nodes per thread
 

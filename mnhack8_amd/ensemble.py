@@ -41,13 +41,15 @@ def members(count=96, seed=1, dt=1e-3, sizes=SIZES, sigmas=SIGMAS):
 
 
 def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log=None, precision="refined",
-        tol_inner=1e-4, max_outer=50, persistent=True):
+        tol_inner=1e-4, max_outer=50, persistent=True, autotune=True):
     """Time the batch (and, for comparison, the same members one after the
     other through the single-solve API) and return a summary.  precision
     "refined": every solve FP32-refined (iteration counts are inner ones);
     "fp64": plain FP64 PCG.  persistent: also time the same steps with each
     batched solve as one persistent (cooperative) launch instead of
-    graph-replayed chunks, and check that T is the same bits."""
+    graph-replayed chunks, and check that T is the same bits.  autotune:
+    first pick the batch's tile geometry (HeatBatch.autotune); the member
+    handles, and so the one-after-another solves, run the same geometry."""
     refined = precision == "refined"
     if precision not in ("refined", "fp64"):
         raise ValueError("precision must be 'refined' or 'fp64'")
@@ -74,6 +76,16 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
     for op, z in zip(batch.ops, zero):
         op.upload("T", z)
         op.synchronize()
+    schedule = {"planes": 16, "wave": 768, "tuned": False}
+    if autotune:
+        t0 = time.perf_counter()
+        planes, wave, ms_it = batch.autotune("refined" if refined else "fp64", 20)
+        schedule = {"planes": planes, "wave": wave, "tuned": True, "ms_per_iteration": round(ms_it, 4),
+                    "tuning_s": round(time.perf_counter() - t0, 3),
+                    "chunks_per_column": sorted({batch.geometry(i)[2] for i in range(count)})}
+        if log:
+            log(f"autotuned tile geometry: {planes} planes per chunk, wave fill {wave} "
+                f"({ms_it * 1e3:.1f} us per batched iteration)")
     batch_steps(batch, 1)                        # warm-up (graph capture, clocks)
     for op, z in zip(batch.ops, zero):
         op.upload("T", z)
@@ -95,6 +107,7 @@ def run(count=96, steps=20, tol=1e-8, max_iter=500, seed=1, sequential=True, log
         "iterations_total": int(its.sum()),
         "iterations_max_per_step": [int(x) for x in its.max(axis=1)],
         "setup_s": round(setup_s, 3),
+        "tile_schedule": schedule,
     }
     if persistent:
         finals = [op.download("T") for op in batch.ops]
@@ -149,10 +162,11 @@ def main(argv=None):
     ap.add_argument("--no-sequential", action="store_true")
     ap.add_argument("--precision", choices=("refined", "fp64"), default="refined")
     ap.add_argument("--no-persistent", action="store_true", help="skip the persistent-launch timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile geometry")
     a = ap.parse_args(argv)
     log = lambda m: print(f"[ensemble] {m}", file=sys.stderr, flush=True)
     print(json.dumps(run(a.members, a.steps, a.tol, seed=a.seed, sequential=not a.no_sequential, log=log,
-                         precision=a.precision, persistent=not a.no_persistent)))
+                         precision=a.precision, persistent=not a.no_persistent, autotune=not a.no_autotune)))
 
 
 if __name__ == "__main__":

@@ -26,6 +26,7 @@ KERNEL = {"gauss": 0, "fast": 1}
 SCATTER = {"atomic": 0, "colored": 1}
 SOURCE = {"gaussian": 0, "const": 1}
 COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3, "tdma": 4}
+ATOMIC_IMPL = {"element": 0, "tiled": 1}
 TILE_SHAPES = {0: (8, 32), 1: (4, 64), 2: (16, 16)}
 SEU_VEC = {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4}
 VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
@@ -60,6 +61,7 @@ SIGNATURES = {
     "heat_upload": (_I, [_P, _P, _P, _I64]),
     "heat_download": (_I, [_P, _P, _P, _I64]),
     "heat_set_colored_impl": (_I, [_P, _I]),
+    "heat_set_atomic_impl": (_I, [_P, _I]),
     "heat_apply_operator": (_I, [_P, _P, _P, _D, _I, _I]),
     "heat_apply_operator_f32": (_I, [_P, _P, _P, _D, _I]),
     "heat_apply_matvec": (_I, [_P, _P, _P]),
@@ -104,7 +106,14 @@ SIGNATURES = {
     "heat_time_stream": (_I, [_P, _P, _P, _I, C.POINTER(_D)]),
     "heat_dot_static": (_I, [_P, _P, _I64, _P, _P]),
     "heat_set_inner_solver": (_I, [_P, _I]),
+    "heat_set_pcg_geometry": (_I, [_P, _I, _I, _I]),
+    "heat_pcg_geometry": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "heat_autotune_pcg": (_I, [_P, _I, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
+    "heat_batch_set_geometry": (_I, [_P, _I, _I]),
+    "heat_batch_geometry": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "heat_batch_autotune": (_I, [_P, _I, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
 }
+PCG_GEOMETRY = {"fused": 0, "one_launch": 1}
 
 _lib = None
 
@@ -213,6 +222,10 @@ class HeatOperator:
     def set_colored_impl(self, impl):
         _check(self._lib.heat_set_colored_impl(self.h, COLORED_IMPL[impl]))
 
+    def set_atomic_impl(self, impl):
+        """--scatter atomic as "tiled" (default) or "element" (the reference's kernel shape)."""
+        _check(self._lib.heat_set_atomic_impl(self.h, ATOMIC_IMPL[impl]))
+
     def set_march_chunk(self, planes):
         _check(self._lib.heat_set_march_chunk(self.h, int(planes)))
 
@@ -230,6 +243,25 @@ class HeatOperator:
     def set_inner_solver(self, mode):
         """Inner solve of the refined solve: "one_launch" (default) or "standard"."""
         _check(self._lib.heat_set_inner_solver(self.h, {"one_launch": 0, "standard": 1}[mode]))
+
+    def set_pcg_geometry(self, which, planes=0, wave=768):
+        """Tile geometry of the PCG kernels ("fused" or "one_launch"): target
+        planes per chunk (0: the default) and whole-wave fill (0: none)."""
+        _check(self._lib.heat_set_pcg_geometry(self.h, PCG_GEOMETRY[which], int(planes), int(wave)))
+
+    def pcg_geometry(self, which="fused"):
+        """(planes, wave, chunks per column, tiles) of geometry `which`."""
+        v = [C.c_int() for _ in range(4)]
+        _check(self._lib.heat_pcg_geometry(self.h, PCG_GEOMETRY[which], *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune_pcg(self, which="fused", iters=30):
+        """Time `iters` iterations per candidate geometry and keep the fastest:
+        returns (planes, wave, ms per iteration)."""
+        p, w, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.heat_autotune_pcg(self.h, PCG_GEOMETRY[which], int(iters), C.byref(p), C.byref(w),
+                                           C.byref(ms)))
+        return p.value, w.value, ms.value
 
     def set_pcg_mode(self, mode):
         """"fused" (three launches per iteration) or "reference" (the reference's dot order)."""
@@ -433,6 +465,25 @@ class HeatBatch:
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_batch_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))
+
+    def set_geometry(self, planes=0, wave=768):
+        """Fused tile geometry of every member (see HeatOperator.set_pcg_geometry)."""
+        _check(self._lib.heat_batch_set_geometry(self.h, int(planes), int(wave)))
+
+    def geometry(self, i):
+        """(planes, wave, chunks per column, tiles) of member i."""
+        v = [C.c_int() for _ in range(4)]
+        _check(self._lib.heat_batch_geometry(self.h, int(i), *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, mode="fp64", iters=20):
+        """Time one step of `iters` iterations per candidate geometry ("fp64"
+        or "refined" steps), keep the fastest; every T is restored.  Returns
+        (planes, wave, ms per iteration)."""
+        p, w, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.heat_batch_autotune(self.h, {"fp64": 0, "refined": 1}[mode], int(iters), C.byref(p),
+                                             C.byref(w), C.byref(ms)))
+        return p.value, w.value, ms.value
 
     def set_persistent(self, on=True):
         """Each batched solve as one persistent (cooperative) launch."""

@@ -38,6 +38,10 @@ typedef struct {
     int nn1;          /* nodes per axis */
     long nn, ne;
     long sx, sy;      /* node strides (x slowest, z fastest) */
+    /* tile geometry of the GPU library's fused order (fused_chunks): target
+     * planes per chunk and whole-wave fill, [0] the fused iteration, [1] the
+     * one-launch inner iteration */
+    int geo_planes[2], geo_wave[2];
 } ho_grid;
 
 typedef struct { int level, step; long target; int bit, rank, armed; } ho_fi;
@@ -590,6 +594,8 @@ static ho_op* create_impl(int n, double dt, const double* k, const double* rhoc,
     op->g.nn = (long)(n + 1) * (n + 1) * (n + 1);
     op->g.ne = (long)n * n * n;
     op->g.sx = (long)(n + 1) * (n + 1); op->g.sy = n + 1;
+    op->g.geo_planes[0] = op->g.geo_planes[1] = 16;
+    op->g.geo_wave[0] = op->g.geo_wave[1] = 768;
     op->dt = dt; op->kernel = kernel; op->colored = colored; op->verify = verify;
     op->source_mode = source_mode; op->source_value = source_value;
     op->chk_tol = 1e-10; op->nthreads = 1;
@@ -787,21 +793,36 @@ int ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double tol
 enum { F_TY = 8, F_TZ = 32, F_CX = 16, F_CX_MIN = 8, F_WAVE = 768, F_NT = 256, F_SB = 2048 };
 enum { F_XW = F_TZ + 2, F_XN = (F_TZ + 2) * (F_TY + 2), F_XR = (F_XN + F_NT - 1) / F_NT };
 
-/* chunks per (y,z) column: ceil(nn1 / 16), raised until the tiles fill whole */
-/* waves of 768 but not to chunks shorter than 8 planes                       */
+/* chunks per (y,z) column: ceil(nn1 / planes), raised until the tiles fill  */
+/* whole waves of `wave` tiles (wave 0: no fill) but not to chunks shorter    */
+/* than 8 planes; the default geometry is (16, 768)                           */
 /* (mnhack8_amd/csrc/heat_pcg_fused.h fused_chunks)                           */
-static int fused_chunks(int nn1, int ncol) {
-    const int nch = (nn1 + F_CX - 1) / F_CX;
-    const long long waves = ((long long)ncol * nch + F_WAVE - 1) / F_WAVE;
-    long long fill = waves * F_WAVE / ncol;
+static int fused_chunks(int nn1, int ncol, int planes, int wave) {
+    const int nch = (nn1 + planes - 1) / planes;
+    if (wave <= 0) return nch;
+    const long long waves = ((long long)ncol * nch + wave - 1) / wave;
+    long long fill = waves * wave / ncol;
     if (fill > nn1 / F_CX_MIN) fill = nn1 / F_CX_MIN;
     return fill > nch ? (int)fill : nch;
+}
+
+static int fused_nch(const ho_grid* g, int ncol) {
+    return fused_chunks(g->nn1, ncol, g->geo_planes[0], g->geo_wave[0]);
 }
 
 static long fused_ntiles(const ho_grid* g) {
     const int n = g->n;
     const int ncol = ((n + F_TY - 1) / F_TY) * ((n + F_TZ - 1) / F_TZ);
-    return (long)ncol * fused_chunks(g->nn1, ncol);
+    return (long)ncol * fused_nch(g, ncol);
+}
+
+int ho_set_pcg_geometry(ho_op* op, int which, int planes, int wave) {
+    if (which < 0 || which > 1) return -1;
+    if (planes == 0) { planes = F_CX; wave = F_WAVE; }
+    if (planes < 2 || wave < 0) return -1;
+    op->g.geo_planes[which] = planes;
+    op->g.geo_wave[which] = wave;
+    return 0;
 }
 
 static double tree_256(double* s) {
@@ -846,7 +867,7 @@ static void tile_sums(const ho_grid* g, const double* p, const double* Ap, const
                       double* part, double out[4]) {
     const int n = g->n;
     const int nby = (n + F_TY - 1) / F_TY, nbz = (n + F_TZ - 1) / F_TZ;
-    const int nch = fused_chunks(g->nn1, nby * nbz);
+    const int nch = fused_nch(g, nby * nbz);
     const long ntiles = (long)nby * nbz * nch;
 #pragma omp parallel for schedule(static) num_threads(g_fused_threads) if (g_fused_threads > 1)
     for (long b = 0; b < ntiles; ++b) {
@@ -1091,7 +1112,7 @@ static c1_geom cg1_geom(const ho_grid* g) {
     q.m = g->n - 1 > 0 ? g->n - 1 : 1;
     q.nby = (q.m + C1_TY - 1) / C1_TY;
     q.nbz = (q.m + C1_TZ - 1) / C1_TZ;
-    q.nch = fused_chunks(q.m, q.nby * q.nbz);
+    q.nch = fused_chunks(q.m, q.nby * q.nbz, g->geo_planes[1], g->geo_wave[1]);
     q.ntiles = (long)q.nby * q.nbz * q.nch;
     return q;
 }

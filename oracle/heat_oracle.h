@@ -69,6 +69,10 @@ int    ho_pcg_solve(ho_op* op, const double* b, double* x, int max_iter, double 
  * two-launch order (tile / super-block partials, 256-wide trees). */
 int    ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol,
                           double* hist);
+/* The GPU library's tile geometry (heat_set_pcg_geometry): WHICH 0 the fused
+ * iteration, 1 the one-launch inner iteration; PLANES 0 restores the default
+ * (16, 768).  Returns 0, or -1 for a bad argument. */
+int    ho_set_pcg_geometry(ho_op* op, int which, int planes, int wave);
 /* One-shot upset of the PCG state for the next fused solve: at iteration
  * ITER the stored copy of VEC (HO_SEU_*) at IDX gets bit BIT flipped after
  * the sums of the kernel that produced it (the refined solve: its FP32 inner

@@ -44,6 +44,7 @@ _SIG = {
     "ho_fused_tile_sums": (None, [_P, _P, _P, _P, _P]),
     "ho_initial_condition": (None, [_P, _P, _I, _D, _D]),
     "ho_proto_run": (_I, [_I, _D, _D, _I, _I, _P, _P]),
+    "ho_set_pcg_geometry": (_I, [_P, _I, _I, _I]),
 }
 _lib = None
 
@@ -84,6 +85,11 @@ class OracleOp:
 
     def set_threads(self, threads):
         lib().ho_set_threads(self.h, int(threads))
+
+    def set_pcg_geometry(self, which, planes, wave):
+        """The GPU library's tile geometry (0 fused, 1 one-launch inner)."""
+        if lib().ho_set_pcg_geometry(self.h, int(which), int(planes), int(wave)):
+            raise ValueError("bad geometry")
 
     def __del__(self):
         if getattr(self, "h", None):

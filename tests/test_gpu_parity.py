@@ -109,6 +109,74 @@ def test_apply_unprojected_and_atomic(n):
         assert relerr(host(opa, yd), orca.apply(x, dt, project=proj)) < 1e-14
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 8, 9, 16, 31, 33, 40, 70])
+def test_atomic_tiled_scatter(n):
+    """The tiled atomic scatter (element tiles summed in LDS, global atomic
+    adds only where tiles and chunks meet) and the one-thread-per-element
+    atomic kernel: within rounding of the oracle's sums of the same element
+    rows (the order of atomic adds is not fixed), projected and not, for the
+    stiffness and the mass operator; projected Dirichlet rows exactly 0.
+    Sizes cover partial tiles in y and z and chunk boundaries in x."""
+    print(f"[case] tiled atomic scatter n={n}", flush=True)
+    k, rhoc = make_field(n, seed=17 + n)
+    dt = 1e-3
+    x = np.random.default_rng(n).standard_normal((n + 1) ** 3) + 0.5
+    orca = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="atomic")
+    opa = heat().HeatOperator(n, dt, k, rhoc, kernel="fast", scatter="atomic")
+    xd, yd = dev(x), dev(np.zeros_like(x))
+    t = np.arange(n + 1)
+    X, Y, Z = np.meshgrid(t, t, t, indexing="ij")
+    bnd = ((X == 0) | (X == n) | (Y == 0) | (Y == n) | (Z == 0) | (Z == n)).ravel()
+    for impl in ("tiled", "element"):
+        opa.set_atomic_impl(impl)
+        for dt_eff in (dt, 0.0):
+            for proj in (True, False):
+                yd.zero_()
+                opa.apply_operator(xd, yd, dt_eff, project=proj)
+                y = host(opa, yd)
+                yo = orca.apply(x, dt_eff, project=proj)
+                assert relerr(y, yo) < 1e-14, (impl, dt_eff, proj)
+                if proj:
+                    assert (y[bnd] == 0).all()
+    opa.cleanup()
+
+
+@pytest.mark.parametrize("detect", [False, True])
+def test_atomic_pcg_in_captured_chunks(detect):
+    """PCG with the tiled atomic scatter runs in captured chunks (no host
+    check per iteration): the iteration count within one of the oracle's
+    atomic solve, the iterate within 1e-10, the D1 record below its
+    threshold; graph replays and eager chunks agree to rounding."""
+    n, dt = 24, 1e-3
+    print(f"[case] atomic pcg detect={detect}", flush=True)
+    k, rhoc = make_field(n, seed=23, sigma=1.5)
+    op = heat().HeatOperator(n, dt, k, rhoc, scatter="atomic", detect=detect, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", scatter="atomic", detect=detect, source="const",
+                   source_value=1.0)
+    T0 = orc.initial_condition("sine", 0.0, 1.0)
+    b = orc.build_rhs(T0)
+    ito, xo, _ = orc.pcg(b, T0, 500, 1e-10)
+    sols = []
+    for graph in (True, False):
+        op.set_tuning(4, graph)
+        op.upload("b", b)
+        op.upload("T", T0)
+        op.abft_take()
+        it = op.pcg_solve("b", "T", 500, 1e-10)
+        x = op.download("T")
+        print(f"  graph={graph}: iterations {it} (oracle {ito})", flush=True)
+        assert abs(it - ito) <= 1
+        assert relerr(x, xo) < 1e-10
+        r = b - orc.apply(x, dt)
+        assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(b) * 1.01
+        if detect:
+            d1, calls = op.abft_take()
+            assert calls == it + 1 and d1 < 1e-12
+        sols.append(x)
+    assert relerr(sols[0], sols[1]) < 1e-12
+    op.cleanup()
+
+
 @pytest.mark.parametrize("scatter", ["colored", "atomic"])
 def test_apply_gauss(scatter):
     n = 6

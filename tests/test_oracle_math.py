@@ -292,6 +292,51 @@ def test_fused_order_sums_cover_every_node(n):
     assert abs(op.fused_sb_dot(x, y) - math.fsum((x * y).tolist())) <= 1e-14 * np.sum(np.abs(x * y))
 
 
+@pytest.mark.parametrize("n", [7, 33, 40])
+@pytest.mark.parametrize("planes,wave", [(2, 0), (4, 0), (6, 768), (64, 768), (1 << 20, 0)])
+def test_fused_order_sums_any_geometry(n, planes, wave):
+    """The same coverage check under every tile geometry the GPU library may
+    be given (one plane pair per chunk up to one chunk per column): a chunk
+    boundary that drops or repeats a plane breaks one of the four sums."""
+    op = OracleOp(n, 1e-3, np.ones(n ** 3), np.ones(n ** 3), kernel="struct")
+    op.set_pcg_geometry(0, planes, wave)
+    rng = np.random.default_rng(200 + n)
+    p = rng.standard_normal(op.nn) + 3.0
+    Ap = rng.standard_normal(op.nn) + 2.0
+    w = rng.standard_normal(op.nn) + 1.0
+    t = op.fused_tile_sums(p, Ap, w)
+    inner = interior_mask(n)
+    exact = [math.fsum((p * Ap)[inner]), math.fsum(Ap[inner]), math.fsum(w * p), math.fsum(p * p)]
+    bound = [np.sum(np.abs(p * Ap)), np.sum(np.abs(Ap)), np.sum(np.abs(w * p)), np.sum(p * p)]
+    for q in range(4):
+        assert abs(t[q] - exact[q]) <= 1e-13 * bound[q] + 1e-300, q
+
+
+@pytest.mark.parametrize("n", [9, 20])
+def test_geometry_changes_rounding_only(n):
+    """Another tile geometry only reorders the partial sums: the fused and the
+    one-launch refined solves keep their iteration counts (to one) and their
+    iterates (to rounding); naming the default geometry gives the default bits."""
+    op, b = lognormal_problem(n)
+    it0, x0, _ = op.pcg_fused(b, np.zeros(op.nn), 500, 1e-10)
+    o0, i0, y0 = op.pcg_mixed_cg1(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    for planes, wave in ((2, 0), (4, 0), (1 << 20, 0)):
+        op.set_pcg_geometry(0, planes, wave)
+        op.set_pcg_geometry(1, planes, wave)
+        it, x, _ = op.pcg_fused(b, np.zeros(op.nn), 500, 1e-10)
+        o, i, y = op.pcg_mixed_cg1(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+        assert abs(it - it0) <= 1 and o == o0 and abs(i - i0) <= 1
+        assert np.max(np.abs(x - x0)) <= 1e-9 * np.max(np.abs(x0))
+        assert np.max(np.abs(y - y0)) <= 1e-9 * np.max(np.abs(y0))
+    op.set_pcg_geometry(0, 16, 768)
+    op.set_pcg_geometry(1, 16, 768)
+    it, x, _ = op.pcg_fused(b, np.zeros(op.nn), 500, 1e-10)
+    o, i, y = op.pcg_mixed_cg1(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert (it, o, i) == (it0, o0, i0) and np.array_equal(x, x0) and np.array_equal(y, y0)
+    with pytest.raises(ValueError):
+        op.set_pcg_geometry(0, 1, 0)
+
+
 def fused_sb_dot_restated(x, y):
     """Super-block order: 2048-node blocks, thread t over nodes t + 256 s, a
     256-wide tree per block, then a 256-wide strided fold and tree."""
