@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-mrhs", action="store_true", help="skip the 16-RHS N=320 apply measurement")
     ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-resident N=256 apply measurement")
+    ap.add_argument("--seu-sites", type=int, default=2000,
+                    help="sites per precision of the N=128 PCG-state upset campaign (0: skip it)")
     ap.add_argument("--sweep", action="store_true", help="time every apply schedule and print the table")
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="run this many axpy passes (2 node vectors read, 1 written) first: the known-bytes "
@@ -331,6 +333,34 @@ def main():
         res["speedup_fp32_refined"] = round(res["fp64"]["ms"] / res["fp32_refined"]["ms"], 3)
         res["speedup_fused_vs_reference_order"] = round(res["fp64_reference_order"]["ms"] / res["fp64"]["ms"], 3)
         result["time_to_1e-8"] = res
+
+    if a.seu_sites > 0 and not a.no_solve:
+        # the upset campaign of the mixed-precision configuration: the same
+        # sites for the FP64 and the refined solve, plus the refined solve's
+        # own FP64 state, coverage with Wilson intervals
+        from mnhack8_amd import seu_campaign
+        log(f"SEU campaign: {a.seu_sites} sites per precision at N={n}")
+        t0 = time.perf_counter()
+        camp = seu_campaign.Campaign(n=n, seed=a.seed)
+        clean = camp.clean()
+        rows = camp.run(a.seu_sites, seed=a.seed, log=log)
+        summ = seu_campaign.summarize(rows, clean)
+        keep = ("sites", "benign", "detected", "silent", "coverage", "coverage_ci95", "coverage_D1_only",
+                "by_detector")
+        result["seu_campaign"] = {
+            "sites_per_precision": a.seu_sites, "wall_s": round(time.perf_counter() - t0, 1),
+            "thresholds": clean["thresholds"],
+            "fp64": {k: summ["fp64"][k] for k in keep},
+            "fp32_refined": {k: summ["fp32_refined"][k] for k in keep},
+            "fp32_refined_whole_state": {k: summ["fp32_refined_whole_state"][k] for k in keep},
+            "coverage_difference_pp": summ["coverage_difference_pp"],
+            "coverage_difference_ci95_pp": summ["coverage_difference_ci95_pp"],
+            "d1_coverage_difference_pp": summ["d1_coverage_difference_pp"],
+            "d1_coverage_difference_ci95_pp": summ["d1_coverage_difference_ci95_pp"]}
+        log(f"SEU campaign: coverage FP64 {summ['fp64']['coverage']:.3f}, refined (whole state) "
+            f"{summ['fp32_refined_whole_state']['coverage']:.3f}, difference "
+            f"{summ['coverage_difference_pp']:+.1f} pp")
+        camp.op.cleanup()
 
     if not a.no_ensemble:
         from mnhack8_amd import ensemble

@@ -207,9 +207,19 @@ int heat_set_pcg_mode(heat_op* op, int mode);
 #define HEAT_SEU_P  2
 #define HEAT_SEU_Z  3
 #define HEAT_SEU_AP 4
+/* The refined solve's FP64 state (heat_pcg_solve_mixed only; bits 0..63;
+ * ITERATION counts its outer passes from 0): the iterate x as the
+ * correction of that pass stores it, and A x as that pass's FP64 apply
+ * stores it for the outer residual b - A x. */
+#define HEAT_SEU_X64  5
+#define HEAT_SEU_AX64 6
 int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteration);
 /* 1 if the fault armed for the last solve fired (its iteration was reached). */
 int heat_pcg_fault_fired(heat_op* op, int* fired);
+/* How the last heat_pcg_solve_mixed ended (new): 1 the outer tolerance was
+ * met, 2 the outer residual was not finite (the solve stops: no correction
+ * can recover it), 0 the pass limit was reached. */
+int heat_pcg_mixed_status(heat_op* op, int* status);
 
 /* Sixteen right-hand sides at once (new; the reference applies its operator
  * to one vector, src/heat_solver_het.cu:885-921).  X and Y are FP32 device

@@ -110,6 +110,7 @@ struct heat_op {
     int pcg_mode = HEAT_PCG_FUSED;
     struct { int armed = 0, vec = 0, bit = 0, iter = 0; long long idx = 0; } pcg_fi;   // one-shot PCG-state SEU
     int pcg_fi_fired = 0;
+    int mixed_status = 0;          // how the last refined solve ended (heat_pcg_mixed_status)
     FusedState* fst = nullptr;
     FusedState* hfst = nullptr;   // pinned mirror
     double* fsp = nullptr;        // super-block partials, 5 x nsb
@@ -1217,6 +1218,8 @@ int heat_pcg_solve(heat_op* op, const double* b, double* x, int max_iter, double
     const bool detect = op->cfg.detect != 0;
     double* Ap = op->vec[HEAT_VEC_AP];
     int rc = HEAT_OK;
+    if (op->pcg_fi.armed && op->pcg_fi.vec > HEAT_SEU_AP)
+        return fail(HEAT_E_STATE, "upsets of x64 / Ax64 are sites of the refined solve (heat_pcg_solve_mixed)");
     if (gather_path(op) && op->pcg_mode == HEAT_PCG_FUSED) {
         k_fused_params<<<1, 1, 0, s>>>(op->fst, tol, max_iter, 0, 0.0);
         CHECK_LAUNCH();
@@ -1303,8 +1306,15 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
     if (rc) return rc;
     hipStream_t s = op->stream;
     const Grid& g = op->g;
-    if (op->pcg_fi.armed && (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER || op->pcg_fi.bit > 31))
-        return fail(HEAT_E_STATE, "PCG-state faults of the refined solve need the fused order and BIT in 0..31");
+    if (op->pcg_fi.armed && (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER ||
+                             (op->pcg_fi.vec <= HEAT_SEU_AP && op->pcg_fi.bit > 31)))
+        return fail(HEAT_E_STATE, "PCG-state faults of the refined solve need the fused order and BIT in 0..31 "
+                                  "(0..63 for x64 and Ax64)");
+    // an upset of the FP64 outer state is injected by the host loop below;
+    // the FP32 inner kernels see none
+    const bool fi64 = op->pcg_fi.armed && op->pcg_fi.vec > HEAT_SEU_AP;
+    const int fi_armed_inner = (op->pcg_fi.armed && !fi64) ? 1 : 0;
+    int fired64 = 0;
     if (op->pcg_mode == HEAT_PCG_REFERENCE_ORDER) {
         k_mixed_params<<<1, 1, 0, s>>>(op->st, tol, tol_inner, max_inner);
         CHECK_LAUNCH();
@@ -1344,13 +1354,14 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
             CHECK_LAUNCH();
             ++nout;
         }
+        op->mixed_status = op->hst->outer_done == 1 || op->hst->outer_done == 2 ? op->hst->outer_done : 0;
         if (outer) *outer = nout;
         if (inner_iters) *inner_iters = op->hst->inner_total;
         return HEAT_OK;
     }
     k_fused_params<<<1, 1, 0, s>>>(op->fst, tol_inner, max_inner, 1, tol);
     CHECK_LAUNCH();
-    k_fused_fault<<<1, 1, 0, s>>>(op->fst, op->pcg_fi.armed, op->pcg_fi.vec, op->pcg_fi.idx, op->pcg_fi.bit,
+    k_fused_fault<<<1, 1, 0, s>>>(op->fst, fi_armed_inner, op->pcg_fi.vec, op->pcg_fi.idx, op->pcg_fi.bit,
                                   op->pcg_fi.iter);
     CHECK_LAUNCH();
     const bool detect = op->cfg.detect != 0;
@@ -1428,6 +1439,7 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         if (rc == HEAT_OK) rc = read_fused(op);
         if (rc == HEAT_OK && status == 3) rc = fail(HEAT_E_STATE, "inner iteration launched out of step");
         if (rc) return rc;
+        op->mixed_status = op->hfst->outer_done == 1 || op->hfst->outer_done == 2 ? op->hfst->outer_done : 0;
         op->pcg_fi_fired = 0;
         if (outer) *outer = nout;
         if (inner_iters) *inner_iters = op->hfst->inner_total;
@@ -1451,6 +1463,10 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         double* Ax = op->vec[HEAT_VEC_AP];
         rc = apply_impl(op, x, Ax, op->cfg.dt, false, true, EPI_NONE, 0);
         if (rc) break;
+        if (fi64 && op->pcg_fi.vec == HEAT_SEU_AX64 && op->pcg_fi.iter == nout) {
+            k_flip<<<1, 1, 0, s>>>(Ax, op->pcg_fi.idx, op->pcg_fi.bit);   // the pass's stored A x
+            fired64 = 1;
+        }
         k_fused_mixed_init<<<op->nsb, NT, 0, s>>>(b, Ax, op->d32, op->e32, op->r32, op->z32, g.nn, op->fsp,
                                                    op->nsb, op->fst);
         if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
@@ -1473,11 +1489,16 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
         if (op->hfst->outer_done || nout >= max_outer) break;
         k_mixed_correct<<<op->nbn, NT, 0, s>>>(x, op->e32, g.nn);
         if (hipGetLastError() != hipSuccess) { rc = fail(HEAT_E_HIP, "kernel launch failed"); break; }
+        if (fi64 && op->pcg_fi.vec == HEAT_SEU_X64 && op->pcg_fi.iter == nout) {
+            k_flip<<<1, 1, 0, s>>>(x, op->pcg_fi.idx, op->pcg_fi.bit);    // x as this correction stored it
+            fired64 = 1;
+        }
         ++nout;
     }
     op->pcg_chunk = saved_chunk;
     if (rc) return rc;
-    op->pcg_fi_fired = op->pcg_fi.armed ? op->hfst->fi_fired : 0;
+    op->mixed_status = op->hfst->outer_done == 1 || op->hfst->outer_done == 2 ? op->hfst->outer_done : 0;
+    op->pcg_fi_fired = fi64 ? fired64 : (op->pcg_fi.armed ? op->hfst->fi_fired : 0);
     op->pcg_fi.armed = 0;
     if (outer) *outer = nout;
     if (inner_iters) *inner_iters = op->hfst->inner_total;
@@ -1486,7 +1507,7 @@ int heat_pcg_solve_mixed(heat_op* op, const double* b, double* x, int max_outer,
 
 int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteration) {
     if (!op) return fail(HEAT_E_ARG, "null handle");
-    if (vec < HEAT_SEU_U || vec > HEAT_SEU_AP) return fail(HEAT_E_ARG, "VEC must be one of HEAT_SEU_*");
+    if (vec < HEAT_SEU_U || vec > HEAT_SEU_AX64) return fail(HEAT_E_ARG, "VEC must be one of HEAT_SEU_*");
     if (node < 0 || node >= op->g.nn) return fail(HEAT_E_ARG, "node %lld out of range", (long long)node);
     if (bit < 0 || bit > 63) return fail(HEAT_E_ARG, "BIT must be in 0..63");
     if (iteration < 0) return fail(HEAT_E_ARG, "iteration must be >= 0");
@@ -1501,6 +1522,12 @@ int heat_arm_pcg_fault(heat_op* op, int vec, int64_t node, int bit, int iteratio
 int heat_pcg_fault_fired(heat_op* op, int* fired) {
     if (!op || !fired) return fail(HEAT_E_ARG, "null argument");
     *fired = op->pcg_fi_fired;
+    return HEAT_OK;
+}
+
+int heat_pcg_mixed_status(heat_op* op, int* status) {
+    if (!op || !status) return fail(HEAT_E_ARG, "null argument");
+    *status = op->mixed_status;
     return HEAT_OK;
 }
 

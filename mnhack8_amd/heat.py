@@ -28,7 +28,7 @@ SOURCE = {"gaussian": 0, "const": 1}
 COLORED_IMPL = {"gather": 0, "sweep": 1, "march": 2, "tile": 3, "tdma": 4}
 ATOMIC_IMPL = {"element": 0, "tiled": 1}
 TILE_SHAPES = {0: (8, 32), 1: (4, 64), 2: (16, 16)}
-SEU_VEC = {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4}
+SEU_VEC = {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4, "x64": 5, "Ax64": 6}
 VEC = {"T": 0, "b": 1, "r": 2, "z": 3, "p": 4, "Ap": 5, "diag": 6, "F": 7, "wM": 8, "wA": 9, "ones": 10}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "bad field", 4: "self-check failed", 5: "bad state"}
 
@@ -100,6 +100,7 @@ SIGNATURES = {
     "heat_batch_step_mixed": (_I, [_P, _I, _I, _I, _D, _D, _P]),
     "heat_batch_step_mrhs": (_I, [_P, _I, _I, _I, _D, _D, _P, _P]),
     "heat_pcg_fault_fired": (_I, [_P, C.POINTER(_I)]),
+    "heat_pcg_mixed_status": (_I, [_P, C.POINTER(_I)]),
     "heat_autotune": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_D)]),
     "heat_set_tile_shape": (_I, [_P, _I]),
     "heat_time_matvec": (_I, [_P, _P, _P, _I, _I64, C.POINTER(_D), C.POINTER(_D)]),
@@ -279,13 +280,22 @@ class HeatOperator:
 
     def arm_pcg_fault(self, vec, node, bit, iteration):
         """One-shot flip of bit `bit` of node `node` of PCG vector `vec`
-        ("u", "r", "p", "z", "Ap") where iteration `iteration` produces it."""
+        ("u", "r", "p", "z", "Ap") where iteration `iteration` produces it;
+        for the refined solve also its FP64 state "x64" and "Ax64", where
+        outer pass `iteration` stores it."""
         _check(self._lib.heat_arm_pcg_fault(self.h, SEU_VEC[vec], int(node), int(bit), int(iteration)))
 
     def pcg_fault_fired(self):
         f = C.c_int()
         _check(self._lib.heat_pcg_fault_fired(self.h, C.byref(f)))
         return bool(f.value)
+
+    def mixed_status(self):
+        """How the last refined solve ended: 1 tolerance met, 2 outer residual
+        not finite, 0 pass limit."""
+        f = C.c_int()
+        _check(self._lib.heat_pcg_mixed_status(self.h, C.byref(f)))
+        return f.value
 
     def set_tuning(self, pcg_chunk=8, use_graph=True):
         _check(self._lib.heat_set_tuning(self.h, int(pcg_chunk), int(bool(use_graph))))

@@ -17,10 +17,23 @@ Detectors, as in the reference's run_case (src/heat_solver_het.cu:1202-1234):
       (src/heat_solver_het.cu:1202-1210).
   NF  a non-finite iterate.
   S   the solver stops without reaching its tolerance (iteration limit).
+  ST  the refined solve reports that its outer residual was not finite (it
+      stops there: no correction can recover it, heat_pcg_mixed_status 2).
 Classes: benign if the returned iterate meets the solve's own tolerance
 (true relative residual <= 10 tol); otherwise detected (any detector) or
 silent (none).  Coverage = detected / (detected + silent), with a Wilson 95 %
-interval.  Both with and without S are reported.
+interval.  Both with and without S are reported, and D1 (the ABFT check)
+alone.
+
+Populations.  The drawn sites (vector of {u, r, p, z, Ap}, node, bit
+fraction, iteration fraction) are the same for both solves: the same vector
+and node, the bit at the same fraction of the word and the iteration at the
+same fraction of the clean solve.  The refined solve also keeps FP64 state
+of its own, the iterate x and the outer apply's A x (once per outer pass);
+a further set of sites, in proportion 2 of 7 arrays, draws those (bits
+0..63, outer pass at a fraction of the clean solve's passes), and the
+refined solve's coverage is reported over its whole state as well.  The
+difference of two coverages carries Newcombe's hybrid-score 95 % interval.
 """
 from __future__ import annotations
 
@@ -35,6 +48,7 @@ import time
 import numpy as np
 
 VECS = ("u", "r", "p", "z", "Ap")
+VECS64 = ("x64", "Ax64")           # the refined solve's FP64 state
 
 
 def wilson(k, n, z=1.96):
@@ -45,6 +59,18 @@ def wilson(k, n, z=1.96):
     c = (p + z * z / (2 * n)) / d
     h = z * math.sqrt(p * (1 - p) / n + z * z / (4 * n * n)) / d
     return (c - h, c + h)
+
+
+def newcombe(k1, n1, k2, n2):
+    """95 % interval of p1 - p2 from the two Wilson intervals (Newcombe's
+    hybrid score method, Stat. Med. 17 (1998) 873, method 10)."""
+    if n1 == 0 or n2 == 0:
+        return (float("nan"), float("nan"))
+    p1, p2 = k1 / n1, k2 / n2
+    l1, u1 = wilson(k1, n1)
+    l2, u2 = wilson(k2, n2)
+    d = p1 - p2
+    return (d - math.sqrt((p1 - l1) ** 2 + (u2 - p2) ** 2), d + math.sqrt((u1 - p1) ** 2 + (p2 - l2) ** 2))
 
 
 class Campaign:
@@ -74,19 +100,23 @@ class Campaign:
 
     # -- one solve -------------------------------------------------------
     def solve(self, prec):
+        """(work, stuck, d1, calls, passes, status): iterations (FP64) or
+        inner iterations (refined), the limit reached, the D1 record, the
+        outer passes and the refined solve's end status (1 for FP64)."""
         op = self.op
         op.copy(self.zero, "T")
         op.abft_take()
         if prec == "fp64":
             it = op.pcg_solve("b", "T", self.max_iter, self.tol)
             stuck = it >= self.max_iter
-            work = it
+            work, passes, status = it, 0, 1
         else:
             outer, inner = op.pcg_solve_mixed("b", "T", self.max_outer, self.max_inner, self.tol, self.tol_inner)
-            stuck = outer >= self.max_outer
-            work = inner
+            status = op.mixed_status()
+            stuck = status == 0
+            work, passes = inner, outer
         d1, calls = op.abft_take()
-        return work, stuck, d1, calls
+        return work, stuck, d1, calls, passes, status
 
     def true_residual(self):
         op = self.op
@@ -111,11 +141,11 @@ class Campaign:
         out = {}
         for prec, xref in (("fp64", self.x64), ("fp32_refined", self.x32)):
             t0 = time.perf_counter()
-            work, stuck, d1, calls = self.solve(prec)
+            work, stuck, d1, calls, passes, status = self.solve(prec)
             self.op.synchronize()
             ms = (time.perf_counter() - t0) * 1e3
             self.op.copy("T", xref)
-            out[prec] = {"iterations": work, "d1_floor": d1, "d1_calls": calls, "ms": ms,
+            out[prec] = {"iterations": work, "outer_passes": passes, "d1_floor": d1, "d1_calls": calls, "ms": ms,
                          "true_residual": self.true_residual()}
         # FP32 D1 noise floor over further right-hand sides (later time steps)
         floors = [out["fp32_refined"]["d1_floor"]]
@@ -135,10 +165,33 @@ class Campaign:
         return out
 
     # -- the campaign ----------------------------------------------------
+    def site(self, prec, xref, vec, node, bit, it_site, index, population):
+        self.op.arm_pcg_fault(vec, node, bit, it_site)
+        work, stuck, d1, calls, passes, status = self.solve(prec)
+        fired = self.op.pcg_fault_fired()
+        fin = self.finite()
+        res = self.true_residual() if fin else float("inf")
+        err = self.error_vs(xref) if fin else float("inf")
+        d1f = (not math.isfinite(d1)) or d1 > self.thr[prec]
+        d2f = not (res <= 1e-6)
+        stf = status == 2
+        benign = fin and res <= 10 * self.tol
+        det = (not fin) or d1f or d2f or stf
+        cls = "benign" if benign else ("detected" if det or stuck else "silent")
+        return {"precision": prec, "population": population, "site": index, "vec": vec, "node": node, "bit": bit,
+                "iteration": it_site, "fired": int(fired), "work": work, "passes": passes, "stuck": int(stuck),
+                "d1": d1, "d1_fired": int(d1f), "true_residual": res, "d2_fired": int(d2f),
+                "nonfinite": int(not fin), "status_fired": int(stf), "E": err, "class": cls,
+                "detected_without_S": int(det)}
+
     def run(self, sites=2000, seed=1, log=None):
         rng = np.random.default_rng(seed)
         draws = [(int(rng.integers(len(VECS))), int(rng.integers(self.nn)), float(rng.random()),
                   float(rng.random())) for _ in range(sites)]
+        # the refined solve's own FP64 state: 2 arrays of 7
+        n64 = int(round(sites * len(VECS64) / (len(VECS) + len(VECS64))))
+        draws64 = [(int(rng.integers(len(VECS64))), int(rng.integers(self.nn)), float(rng.random()),
+                    float(rng.random())) for _ in range(n64)]
         rows = []
         for prec, xref, wbits in (("fp64", self.x64, 64), ("fp32_refined", self.x32, 32)):
             iters = self.clean_info[prec]["iterations"]
@@ -146,53 +199,65 @@ class Campaign:
             for s, (v, node, ub, ui) in enumerate(draws):
                 bit = min(wbits - 1, int(ub * wbits))
                 it_site = min(iters - 1, int(ui * iters))
-                self.op.arm_pcg_fault(VECS[v], node, bit, it_site)
-                work, stuck, d1, calls = self.solve(prec)
-                fired = self.op.pcg_fault_fired()
-                fin = self.finite()
-                res = self.true_residual() if fin else float("inf")
-                err = self.error_vs(xref) if fin else float("inf")
-                d1f = (not math.isfinite(d1)) or d1 > self.thr[prec]
-                d2f = not (res <= 1e-6)
-                benign = fin and res <= 10 * self.tol
-                det = (not fin) or d1f or d2f
-                cls = "benign" if benign else ("detected" if det or stuck else "silent")
-                rows.append({"precision": prec, "site": s, "vec": VECS[v], "node": node, "bit": bit,
-                             "iteration": it_site, "fired": int(fired), "work": work, "stuck": int(stuck),
-                             "d1": d1, "d1_fired": int(d1f), "true_residual": res, "d2_fired": int(d2f),
-                             "nonfinite": int(not fin), "E": err, "class": cls,
-                             "detected_without_S": int(det)})
-                if log and (s + 1) % 250 == 0:
+                rows.append(self.site(prec, xref, VECS[v], node, bit, it_site, s, "common"))
+                if log and (s + 1) % 500 == 0:
                     log(f"{prec}: {s + 1}/{sites} sites, {time.perf_counter() - t0:.1f} s")
+        passes = max(1, self.clean_info["fp32_refined"]["outer_passes"])
+        t0 = time.perf_counter()
+        for s, (v, node, ub, ui) in enumerate(draws64):
+            bit = min(63, int(ub * 64))
+            it_site = min(passes - 1, int(ui * passes))
+            rows.append(self.site("fp32_refined", self.x32, VECS64[v], node, bit, it_site, s, "fp64_state"))
+        if log:
+            log(f"fp32_refined FP64 state: {n64} sites, {time.perf_counter() - t0:.1f} s")
         return rows
+
+
+def _cov(harm, key=None):
+    n = len(harm)
+    k = sum(1 for r in harm if (r[key] if key else r["class"] == "detected"))
+    return {"harmful": n, "detected": k, "coverage": k / n if n else float("nan"), "ci95": wilson(k, n)}
+
+
+def _summary(rs):
+    harm = [r for r in rs if r["class"] != "benign"]
+    det = sum(1 for r in harm if r["class"] == "detected")
+    n_h = len(harm)
+    by_vec = {}
+    for v in VECS + VECS64:
+        sv = [r for r in rs if r["vec"] == v]
+        if not sv:
+            continue
+        hv = [r for r in harm if r["vec"] == v]
+        by_vec[v] = {"sites": len(sv), **_cov(hv), "silent": sum(1 for r in hv if r["class"] == "silent"),
+                     "D1_only": _cov(hv, "d1_fired")}
+    return {
+        "sites": len(rs), "fired": sum(r["fired"] for r in rs),
+        "benign": len(rs) - n_h, "detected": det, "silent": n_h - det,
+        "coverage": det / n_h if n_h else float("nan"), "coverage_ci95": wilson(det, n_h),
+        "coverage_without_S": (sum(1 for r in harm if r["detected_without_S"]) / n_h) if n_h else float("nan"),
+        "coverage_D1_only": _cov(harm, "d1_fired"),
+        "by_detector": {"D1": sum(r["d1_fired"] for r in harm), "D2": sum(r["d2_fired"] for r in harm),
+                        "NF": sum(r["nonfinite"] for r in harm), "ST": sum(r["status_fired"] for r in harm),
+                        "S": sum(r["stuck"] for r in harm)},
+        "by_vector": by_vec,
+    }
 
 
 def summarize(rows, clean):
     out = {"clean": clean}
-    for prec in ("fp64", "fp32_refined"):
-        rs = [r for r in rows if r["precision"] == prec]
-        harm = [r for r in rs if r["class"] != "benign"]
-        det = sum(1 for r in harm if r["class"] == "detected")
-        det_noS = sum(1 for r in harm if r["detected_without_S"])
-        n_h = len(harm)
-        by_vec = {}
-        for v in VECS:
-            hv = [r for r in harm if r["vec"] == v]
-            by_vec[v] = {"sites": sum(1 for r in rs if r["vec"] == v), "harmful": len(hv),
-                         "detected": sum(1 for r in hv if r["class"] == "detected"),
-                         "silent": sum(1 for r in hv if r["class"] == "silent")}
-        out[prec] = {
-            "sites": len(rs), "fired": sum(r["fired"] for r in rs),
-            "benign": len(rs) - n_h, "detected": det, "silent": n_h - det,
-            "coverage": det / n_h if n_h else float("nan"), "coverage_ci95": wilson(det, n_h),
-            "coverage_without_S": det_noS / n_h if n_h else float("nan"),
-            "coverage_D1_only": sum(r["d1_fired"] for r in harm) / n_h if n_h else float("nan"),
-            "by_detector": {"D1": sum(r["d1_fired"] for r in harm), "D2": sum(r["d2_fired"] for r in harm),
-                            "NF": sum(r["nonfinite"] for r in harm), "S": sum(r["stuck"] for r in harm)},
-            "by_vector": by_vec,
-        }
-    a, b = out["fp64"]["coverage"], out["fp32_refined"]["coverage"]
-    out["coverage_difference_pp"] = 100 * (b - a)
+    common = [r for r in rows if r["population"] == "common"]
+    out["fp64"] = _summary([r for r in common if r["precision"] == "fp64"])
+    out["fp32_refined"] = _summary([r for r in common if r["precision"] == "fp32_refined"])
+    out["fp32_refined_whole_state"] = _summary([r for r in rows if r["precision"] == "fp32_refined"])
+    a, b = out["fp64"], out["fp32_refined_whole_state"]
+    out["coverage_difference_pp"] = 100 * (b["coverage"] - a["coverage"])
+    lo, hi = newcombe(b["detected"], b["detected"] + b["silent"], a["detected"], a["detected"] + a["silent"])
+    out["coverage_difference_ci95_pp"] = (100 * lo, 100 * hi)
+    da, db = a["coverage_D1_only"], b["coverage_D1_only"]
+    out["d1_coverage_difference_pp"] = 100 * (db["coverage"] - da["coverage"])
+    lo, hi = newcombe(db["detected"], db["harmful"], da["detected"], da["harmful"])
+    out["d1_coverage_difference_ci95_pp"] = (100 * lo, 100 * hi)
     return out
 
 
@@ -216,7 +281,10 @@ def main(argv=None):
     summ = summarize(rows, clean)
     with open(os.path.join(a.out, "summary.json"), "w") as f:
         json.dump(summ, f, indent=1)
-    print(json.dumps({k: summ[k] for k in ("fp64", "fp32_refined", "coverage_difference_pp")}), flush=True)
+    print(json.dumps({k: summ[k] for k in ("fp64", "fp32_refined", "fp32_refined_whole_state",
+                                           "coverage_difference_pp", "coverage_difference_ci95_pp",
+                                           "d1_coverage_difference_pp", "d1_coverage_difference_ci95_pp")}),
+          flush=True)
 
 
 if __name__ == "__main__":

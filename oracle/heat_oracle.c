@@ -74,6 +74,7 @@ struct ho_op {
     /* one-shot PCG-state upset of the fused solves: vector, node, bit, iteration */
     int sfi_armed, sfi_vec, sfi_bit, sfi_iter, sfi_fired;
     long sfi_idx;
+    int mixed_status;           /* how the last refined solve ended (ho_mixed_status) */
 };
 
 static char g_err[512];
@@ -947,6 +948,7 @@ void ho_arm_pcg_fault(ho_op* op, int vec, long idx, int bit, int iter) {
     op->sfi_armed = 1; op->sfi_vec = vec; op->sfi_idx = idx; op->sfi_bit = bit; op->sfi_iter = iter;
 }
 int ho_pcg_fault_fired(const ho_op* op) { return op->sfi_fired; }
+int ho_mixed_status(const ho_op* op) { return op->mixed_status; }
 
 int ho_pcg_solve_fused(ho_op* op, const double* b, double* x, int max_iter, double tol, double* hist) {
     const ho_grid* g = &op->g;
@@ -1029,6 +1031,10 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
     op->sfi_fired = 0;
     for (;;) {
         apply_projected(op, x, r64);
+        if (fi && op->sfi_vec == HO_SEU_AX64 && nout == op->sfi_iter) {   /* the pass's stored A x */
+            ho_flip_bit(r64, op->sfi_idx, op->sfi_bit);
+            op->sfi_fired = 1;
+        }
         for (long i = 0; i < n; ++i) {
             r64[i] = fma(-1.0, r64[i], b[i]);
             r[i] = (float)r64[i];
@@ -1084,8 +1090,13 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
                 for (long i = 0; i < n; ++i) p[i] = (float)fma(beta, (double)p[i], (double)z[i]);
             }
         }
+        op->mixed_status = outer_done;
         if (outer_done || nout >= max_outer) break;
         for (long i = 0; i < n; ++i) x[i] += (double)e[i];
+        if (fi && op->sfi_vec == HO_SEU_X64 && nout == op->sfi_iter) {    /* x as this correction stores it */
+            ho_flip_bit(x, op->sfi_idx, op->sfi_bit);
+            op->sfi_fired = 1;
+        }
         ++nout;
     }
     if (outer) *outer = nout;
@@ -1264,6 +1275,7 @@ int ho_pcg_solve_mixed_cg1(ho_op* op, const double* b, double* x, int max_outer,
             cg1_sums(g, &q, step_rows, op->detect ? 6 : 3, part, t);
             ++total;
         }
+        op->mixed_status = outer_done;
         if (outer_done || nout >= max_outer) break;
         for (long i = 0; i < n; ++i) x[i] += (double)e[i];
         ++nout;

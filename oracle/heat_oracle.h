@@ -82,8 +82,14 @@ int    ho_set_pcg_geometry(ho_op* op, int which, int planes, int wave);
 #define HO_SEU_P  2
 #define HO_SEU_Z  3
 #define HO_SEU_AP 4
+/* the refined solve's FP64 state, ITER = outer pass (0-based): x as the
+ * pass's correction stores it, A x as the pass's apply stores it */
+#define HO_SEU_X64  5
+#define HO_SEU_AX64 6
 void   ho_arm_pcg_fault(ho_op* op, int vec, long idx, int bit, int iter);
 int    ho_pcg_fault_fired(const ho_op* op);
+/* how the last refined solve ended: 1 tolerance met, 2 residual not finite, 0 pass limit */
+int    ho_mixed_status(const ho_op* op);
 /* The fused order's sums alone: super-block x.y, and the tile sums
  * [p.Ap (interior), 1_int.Ap, w.p, p.p]. */
 double ho_fused_sb_dot(ho_op* op, const double* x, const double* y);

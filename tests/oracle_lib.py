@@ -45,6 +45,7 @@ _SIG = {
     "ho_initial_condition": (None, [_P, _P, _I, _D, _D]),
     "ho_proto_run": (_I, [_I, _D, _D, _I, _I, _P, _P]),
     "ho_set_pcg_geometry": (_I, [_P, _I, _I, _I]),
+    "ho_mixed_status": (_I, [_P]),
 }
 _lib = None
 
@@ -184,11 +185,15 @@ class OracleOp:
         return o.value, i.value, x
 
     def arm_pcg_fault(self, vec, node, bit, iteration):
-        lib().ho_arm_pcg_fault(self.h, {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4}[vec], int(node), int(bit),
-                               int(iteration))
+        lib().ho_arm_pcg_fault(self.h, {"u": 0, "r": 1, "p": 2, "z": 3, "Ap": 4, "x64": 5, "Ax64": 6}[vec],
+                               int(node), int(bit), int(iteration))
 
     def pcg_fault_fired(self):
         return bool(lib().ho_pcg_fault_fired(self.h))
+
+    def mixed_status(self):
+        """1 tolerance met, 2 outer residual not finite, 0 pass limit (last refined solve)."""
+        return lib().ho_mixed_status(self.h)
 
     def fused_sb_dot(self, x, y):
         x = np.ascontiguousarray(x, dtype=np.float64)

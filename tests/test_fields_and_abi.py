@@ -106,17 +106,38 @@ def test_gen_field_cli(tmp_path):
 
 
 def test_seu_campaign_summary_logic():
-    """Classification bookkeeping of the upset campaign (host logic only)."""
+    """Classification bookkeeping of the upset campaign (host logic only):
+    the common population per precision, the refined solve's whole state
+    (its FP64 sites added), D1-only coverage and the interval of a
+    difference of coverages."""
     from mnhack8_amd import seu_campaign as sc
     rows = []
+    def row(prec, pop, vec, cls, d1):
+        return {"precision": prec, "population": pop, "vec": vec, "class": cls, "fired": 1, "d1_fired": d1,
+                "d2_fired": 0, "nonfinite": 0, "status_fired": 0, "stuck": 0,
+                "detected_without_S": int(cls == "detected")}
     for prec in ("fp64", "fp32_refined"):
         for i, cls in enumerate(["benign"] * 6 + ["detected"] * 3 + ["silent"]):
-            rows.append({"precision": prec, "vec": sc.VECS[i % 5], "class": cls, "fired": 1,
-                         "d1_fired": int(cls == "detected"), "d2_fired": 0, "nonfinite": 0,
-                         "stuck": 0, "detected_without_S": int(cls == "detected")})
+            rows.append(row(prec, "common", sc.VECS[i % 5], cls, int(cls == "detected" and i % 2 == 0)))
+    rows += [row("fp32_refined", "fp64_state", "x64", "benign", 0), row("fp32_refined", "fp64_state", "Ax64",
+                                                                         "detected", 0)]
     s = sc.summarize(rows, {})
     assert s["fp64"]["benign"] == 6 and s["fp64"]["detected"] == 3 and s["fp64"]["silent"] == 1
-    assert s["fp64"]["coverage"] == 0.75 and s["coverage_difference_pp"] == 0.0
+    assert s["fp64"]["coverage"] == 0.75
+    assert s["fp32_refined"]["sites"] == 10 and s["fp32_refined_whole_state"]["sites"] == 12
+    assert s["fp32_refined_whole_state"]["coverage"] == 0.8
+    assert abs(s["coverage_difference_pp"] - 5.0) < 1e-12
+    lo, hi = s["coverage_difference_ci95_pp"]
+    assert lo < 5.0 < hi and -100 <= lo and hi <= 100
+    assert s["fp64"]["coverage_D1_only"]["detected"] == 2
+    assert s["fp32_refined_whole_state"]["by_vector"]["Ax64"]["detected"] == 1
     lo, hi = s["fp64"]["coverage_ci95"]
     assert 0.2 < lo < 0.75 < hi <= 1.0
     assert sc.wilson(0, 0)[0] != sc.wilson(0, 0)[0]       # NaN for no harmful sites
+    # Newcombe's interval of a difference: symmetric under swapping the two
+    # proportions (with the sign), and it holds 0 for equal proportions
+    l1, h1 = sc.newcombe(7, 20, 12, 25)
+    l2, h2 = sc.newcombe(12, 25, 7, 20)
+    assert abs(l1 + h2) < 1e-12 and abs(h1 + l2) < 1e-12
+    l3, h3 = sc.newcombe(10, 40, 5, 20)
+    assert l3 < 0 < h3

@@ -660,6 +660,43 @@ def test_pcg_state_upset_refined_bitwise(vec):
         assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
 
 
+@pytest.mark.parametrize("vec", ["x64", "Ax64"])
+def test_refined_fp64_state_upset_bitwise(vec):
+    """Upsets of the refined solve's FP64 state (the iterate x as a
+    correction stores it, A x as an outer pass's apply stores it): outer and
+    inner counts, fired flag, end status and every bit of the iterate (NaN
+    and Inf patterns included) identical to the oracle."""
+    n, dt = 10, 1e-4
+    k, rhoc = make_field(n, seed=52)
+    op = heat().HeatOperator(n, dt, k, rhoc, detect=True, source="const", source_value=1.0)
+    orc = OracleOp(n, dt, k, rhoc, kernel="struct", detect=True, source="const", source_value=1.0)
+    b = orc.build_rhs(np.zeros(op.nn))
+    rng = np.random.default_rng(9)
+    for bit, it_site in [(3, 0), (40, 0), (52, 1), (61, 0), (62, 1), (63, 0), (20, 5)]:
+        node = int(rng.integers(op.nn))
+        print(f"[case] seu fp64 state vec={vec} node={node} bit={bit} pass={it_site}", flush=True)
+        op.upload("b", b)
+        op.upload("T", np.zeros(op.nn))
+        op.abft_take()
+        orc.abft_take()
+        op.arm_pcg_fault(vec, node, bit, it_site)
+        outer, inner = op.pcg_solve_mixed("b", "T", 8, 60, 1e-10, 1e-4)
+        got = (outer, inner, op.pcg_fault_fired(), op.mixed_status())
+        orc.arm_pcg_fault(vec, node, bit, it_site)
+        oo, io, xo = orc.pcg_mixed_fused(b, np.zeros(op.nn), 8, 60, 1e-10, 1e-4)
+        print(f"  outer {outer} inner {inner} status {got[3]}", flush=True)
+        assert got == (oo, io, orc.pcg_fault_fired(), orc.mixed_status())
+        assert np.array_equal(op.download("T").view(np.uint64), xo.view(np.uint64))
+        d1, calls = op.abft_take()
+        d1o, callso = orc.abft_take()
+        assert calls == callso and (d1 == d1o or (np.isnan(d1) and np.isnan(d1o)))
+    # FP64 sites belong to the refined solve
+    op.arm_pcg_fault("x64", 0, 3, 0)
+    with pytest.raises(heat().HeatError):
+        op.pcg_solve("b", "T", 50, 1e-8)
+    op.cleanup()
+
+
 def test_pcg_state_upset_argument_checks():
     n = 4
     k, rhoc = make_field(n)
@@ -697,6 +734,9 @@ def test_seu_campaign_small():
     for prec in ("fp64", "fp32_refined"):
         assert s[prec]["sites"] == 40
         assert s[prec]["benign"] + s[prec]["detected"] + s[prec]["silent"] == 40
+    whole = s["fp32_refined_whole_state"]
+    assert whole["sites"] == 40 + round(40 * 2 / 7)
+    assert set(whole["by_vector"]) == {"u", "r", "p", "z", "Ap", "x64", "Ax64"}
 
 
 BATCH_MEMBERS = [

@@ -565,3 +565,33 @@ def test_pcg_state_upset_refined_solve_self_corrects():
         r = b - op.apply(x, op.dt)
         assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10, vec
         assert o >= 1 and i > 0
+
+
+def test_refined_fp64_state_upsets():
+    """Upsets of the refined solve's FP64 state follow from its structure:
+    a finite upset of x after a correction, or of A x in a pass's residual,
+    is seen by the next outer residual and corrected (the tolerance is met,
+    status 1, with an extra pass when needed); an upset that makes the outer
+    residual infinite ends the solve with status 2 (reported, never a
+    silent return).  A pass index past the solve's passes never fires."""
+    op, b = lognormal_problem(11)
+    o0, i0, x0 = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert op.mixed_status() == 1
+    node = int(np.flatnonzero(interior_mask(11))[200])
+    for vec, bit, it in (("x64", 51, 0), ("x64", 61, 0), ("Ax64", 45, 0), ("Ax64", 60, 1), ("x64", 20, 1)):
+        op.arm_pcg_fault(vec, node, bit, it)
+        o, i, x = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+        assert op.pcg_fault_fired(), (vec, bit)
+        assert op.mixed_status() == 1, (vec, bit)
+        r = b - op.apply(x, op.dt)
+        assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-10, (vec, bit)
+        assert o >= o0
+    # the exponent's top bit of a small A x entry of pass 1 (pass 0 applies
+    # to x = 0): 2^1024 times larger, its square in the residual norm
+    # overflows to infinity
+    op.arm_pcg_fault("Ax64", node, 62, 1)
+    o, i, x = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert op.pcg_fault_fired() and op.mixed_status() == 2 and o == 1
+    op.arm_pcg_fault("x64", node, 40, o0 + 5)
+    o, i, x = op.pcg_mixed_fused(b, np.zeros(op.nn), 50, 500, 1e-10, 1e-4)
+    assert not op.pcg_fault_fired() and np.array_equal(x, x0)
