@@ -872,7 +872,13 @@ __device__ __forceinline__ bool resolve_element(const Grid& g, int tid, int colo
     return true;
 }
 
-__constant__ int c_off[8][3] = {{0,0,0},{1,0,0},{1,1,0},{0,1,0},{0,0,1},{1,0,1},{1,1,1},{0,1,1}};
+// Offsets (x, y, z) of local node i = 0..7: (0,0,0), (1,0,0), (1,1,0),
+// (0,1,0), (0,0,1), (1,0,1), (1,1,1), (0,1,1) (the reference's N0..N7,
+// src/heat_solver_het.cu:261-263), as compile-time functions so that a loop
+// over i unrolls into constant register and LDS indices
+__host__ __device__ constexpr int lox(int i) { return ((i & 3) == 1 || (i & 3) == 2) ? 1 : 0; }
+__host__ __device__ constexpr int loy(int i) { return (i & 3) >= 2 ? 1 : 0; }
+__host__ __device__ constexpr int loz(int i) { return i >> 2; }
 
 // The two-point Gauss rule on the unit interval: abscissae t0 = (1 - 1/sqrt(3))/2
 // and t1 = 1 - t0, weights 1/2.  At t_p the two linear hats phi_0 = 1 - t and
@@ -894,7 +900,7 @@ __device__ __forceinline__ void elem_gauss(const double xe[8], double ye[8], dou
                                            double rhoc_e, double dt, double h) {
     double u[2][2][2];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) u[c_off[i][0]][c_off[i][1]][c_off[i][2]] = xe[i];
+    for (int i = 0; i < 8; ++i) u[lox(i)][loy(i)][loz(i)] = xe[i];
     // along z: values at the two z points, and the z difference
     double cz[2][2][2], dz[2][2];
 #pragma unroll
@@ -983,7 +989,7 @@ __device__ __forceinline__ void elem_gauss(const double xe[8], double ye[8], dou
     const double ks = 2.0 * dt * k_e / (h * h);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int a = c_off[i][0], b = c_off[i][1], c = c_off[i][2];
+        const int a = lox(i), b = loy(i), c = loz(i);
         const double stiff = (a ? kx[b][c] : -kx[b][c]) + (b ? ky[a][c] : -ky[a][c]) + (c ? kz[a][b] : -kz[a][b]);
         ye[i] = wq * (rhoc_e * mv[a][b][c] + ks * stiff);
     }
@@ -1005,7 +1011,7 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
     T xe[8], ye[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        gid[i] = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
+        gid[i] = (ix + lox(i)) * g.sx + (iy + loy(i)) * g.sy + (iz + loz(i));
         xe[i] = x[gid[i]];
     }
     if (KIND == 1) {
@@ -1026,8 +1032,8 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int ox = c_off[i][0], oy = c_off[i][1], oz = c_off[i][2];
-            // local index of the node with the given offsets (inverse of c_off)
+            const int ox = lox(i), oy = loy(i), oz = loz(i);
+            // local index of the node with the given offsets (inverse of lox, loy, loz)
             auto L = [](int a, int b, int c) { return (c << 2) | (b ? (a ? 2 : 3) : (a ? 1 : 0)); };
             const T S0 = xe[i];
             const T S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
@@ -1076,8 +1082,11 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
                                                       const double* __restrict__ ck, Grid g, int nby,
                                                       int nbz, int nch, int nblocks, const DevState* st,
                                                       int check_done) {
+    // element rows by local node, on the tile's elements with a zero border
+    // of one element all round, so the gather below needs no bounds check
+    constexpr int CBW = STZ + 2, CBN = (STY + 2) * CBW;
     __shared__ double xs[3][SFN];
-    __shared__ double cb[8][SNT];          // element rows by local node
+    __shared__ double cb[8][CBN];
     if (check_done && st->done) return;
     const int b = xcd_slab_block(nblocks);
     if (b >= nblocks) return;
@@ -1093,7 +1102,7 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
     // this thread's footprint nodes (ry, rz): offset in a plane (-1: outside
     // the mesh or the footprint), whether no other tile touches the node, and
     // whether it lies on the mesh boundary in y or z
-    int fo[SFR], fr[SFR];
+    int fo[SFR], fr[SFR], gb[SFR];
     bool finner[SFR], fbnd[SFR];
 #pragma unroll
     for (int j = 0; j < SFR; ++j) {
@@ -1102,13 +1111,32 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
         const int yy = y0 + ry, zz = z0 + rz;
         fo[j] = (i < SFN && yy <= n && zz <= n) ? yy * nn1 + zz : -1;
         fr[j] = i;
+        gb[j] = i < SFN ? ry * CBW + rz : 0;      // padded index of element (ry - 1, rz - 1)
         finner[j] = ry >= 1 && ry < STY && rz >= 1 && rz < STZ;
         fbnd[j] = yy == 0 || yy == n || zz == 0 || zz == n;
     }
+    for (int i = tid; i < 8 * CBN; i += SNT) (&cb[0][0])[i] = 0.0;
+    // Loads are raw buffer loads with one resource per plane; an offset past
+    // the resource reads 0, so no load sits behind a branch and the waits
+    // for them stay exact (the next plane's loads stay in flight)
+    constexpr unsigned OOB = 0x7ffffff8u;
+    auto ldd = [](__amdgpu_buffer_rsrc_t r, unsigned off) -> double {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+#else
+        (void)r; (void)off;
+        return 0.0;
+#endif
+    };
+    auto rsrc = [](const double* base, size_t count) {
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, (int)(count * sizeof(double)),
+                                                 0x00020000);
+    };
     double xr[SFR];
     auto load_x = [&](int p) {
+        const __amdgpu_buffer_rsrc_t R = rsrc(x + (size_t)(p <= n ? p : 0) * S, S);
 #pragma unroll
-        for (int j = 0; j < SFR; ++j) xr[j] = (fo[j] >= 0 && p <= n) ? x[(size_t)p * S + fo[j]] : 0.0;
+        for (int j = 0; j < SFR; ++j) xr[j] = ldd(R, (fo[j] >= 0 && p <= n) ? (unsigned)fo[j] * 8u : OOB);
     };
     auto park_x = [&](int slot) {
 #pragma unroll
@@ -1116,28 +1144,23 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
             if (fr[j] < SFN) xs[slot][fr[j]] = xr[j];
     };
     auto load_e = [&](int e, double& c0, double& c1) {
-        c0 = 0.0;
-        c1 = 0.0;
-        if (elem_ok && e < ex1) {
-            const size_t q = (size_t)e * nsq + (size_t)ey * n + ez;
-            c0 = cr[q];
-            if (!MASS) c1 = ck[q];
-        }
+        const bool ok = elem_ok && e < ex1;
+        const size_t eb = (size_t)(ok ? e : 0) * nsq;
+        const unsigned off = ok ? (unsigned)(ey * n + ez) * 8u : OOB;
+        c0 = ldd(rsrc(cr + eb, nsq), off);
+        c1 = MASS ? 0.0 : ldd(rsrc(ck + eb, nsq), off);
     };
-    // local node of offsets (a, b, c) in (x, y, z) (inverse of c_off)
+    // local node of offsets (a, b, c) in (x, y, z) (inverse of lox, loy, loz)
     auto L = [](int a, int bb, int c) { return (c << 2) | (bb ? (a ? 2 : 3) : (a ? 1 : 0)); };
-    // sum of the rows of local node (ox, 1 - a, 1 - c) of the tile elements
-    // (ry - 1 + a, rz - 1 + c) around footprint node i, a and c in order
-    auto gather = [&](int i, int ox) {
-        const int ry = i / SFZ, rz = i - (i / SFZ) * SFZ;
+    // sum of the rows of local node (ox, 1 - a, 1 - c) of the elements
+    // (ry - 1 + a, rz - 1 + c) around footprint node (ry, rz), a and c in
+    // order (elements off the tile read the zero border)
+    auto gather = [&](int g0, int ox) {
         double v = 0.0;
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const int ety = ry - 1 + a, etz = rz - 1 + c;
-                if (ety >= 0 && ety < STY && etz >= 0 && etz < STZ) v += cb[L(ox, 1 - a, 1 - c)][ety * STZ + etz];
-            }
+            for (int c = 0; c < 2; ++c) v += cb[L(ox, 1 - a, 1 - c)][g0 + a * CBW + c];
         return v;
     };
     // node plane p is complete in v[]: store the nodes only this block
@@ -1175,18 +1198,18 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
             double xe[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const double* xp = c_off[i][0] ? xh : xl;
-                xe[i] = xp[base + c_off[i][1] * SFZ + c_off[i][2]];
+                const double* xp = lox(i) ? xh : xl;
+                xe[i] = xp[base + loy(i) * SFZ + loz(i)];
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const int ox = c_off[i][0], oy = c_off[i][1], oz = c_off[i][2];
+                const int ox = lox(i), oy = loy(i), oz = loz(i);
                 const double S0 = xe[i];
                 const double S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
                 const double S2 = (xe[L(1 - ox, 1 - oy, oz)] + xe[L(1 - ox, oy, 1 - oz)]) + xe[L(ox, 1 - oy, 1 - oz)];
                 const double S3 = xe[L(1 - ox, 1 - oy, 1 - oz)];
                 // elements outside the mesh have zero coefficients: zero rows
-                cb[i][tid] = struct_row(S0, S1, S2, S3, c_r, MASS ? 0.0 : c_k);
+                cb[i][(ty + 1) * CBW + tz + 1] = struct_row(S0, S1, S2, S3, c_r, MASS ? 0.0 : c_k);
             }
         }
         __syncthreads();
@@ -1195,8 +1218,8 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
         for (int j = 0; j < SFR; ++j) {
             v[j] = 0.0;
             if (fr[j] < SFN) {
-                v[j] = up[j] + gather(fr[j], 0);   // node plane ex is complete within the tile
-                up[j] = gather(fr[j], 1);          // node plane ex + 1: element plane ex so far
+                v[j] = up[j] + gather(gb[j], 0);   // node plane ex is complete within the tile
+                up[j] = gather(gb[j], 1);          // node plane ex + 1: element plane ex so far
             }
         }
         flush(ex, v);
@@ -1468,7 +1491,7 @@ __global__ __launch_bounds__(ET) void k_diag(double* __restrict__ d,
     const double de = rhoc[e] * mdiag + (dt * kcell[e]) * kdiag;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int nid = (ix + c_off[i][0]) * g.sx + (iy + c_off[i][1]) * g.sy + (iz + c_off[i][2]);
+        const int nid = (ix + lox(i)) * g.sx + (iy + loy(i)) * g.sy + (iz + loz(i));
         if (ATOMIC) unsafeAtomicAdd(&d[nid], de); else d[nid] += de;
     }
 }
@@ -1505,7 +1528,7 @@ __global__ __launch_bounds__(ET) void k_source(double* __restrict__ F, Grid g, i
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const int a = c_off[i][0], b = c_off[i][1], c = c_off[i][2];
+        const int a = lox(i), b = loy(i), c = loz(i);
         const double fe = ((pref * X[a]) * Y[b]) * Z[c];
         const int nid = (ix + a) * g.sx + (iy + b) * g.sy + (iz + c);
         if (ATOMIC) unsafeAtomicAdd(&F[nid], fe); else F[nid] += fe;
