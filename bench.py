@@ -213,8 +213,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_apply_tdma ({sched_key})", "us_per_launch": round(avg_ms * 1e3, 2),
-                     "timing": "mean of the timed back-to-back launches, HIP events on the library's stream; "
-                               "a kernel trace of the same launches gives the same per-dispatch duration",
+                     "timing": "mean of the timed back-to-back launches, HIP events on the library's stream: "
+                               "the per-dispatch duration of a kernel trace plus the 0.1-0.2 us between "
+                               "dispatches (profiles/colored_matvec_n128_dispatch.txt)",
                      "cache_resident": "the N=128 working set (68 MB per apply) stays in the 256 MB Infinity "
                                        "Cache; cold_cache and hbm_resident give the HBM-bound figures"},
         "cold_cache": {"ms_per_step": cold_ms, "value": round(nn / (cold_ms * 1e-3) / 1e9, 3),
