@@ -15,12 +15,13 @@
 // lane exchange is needed.
 //
 // Blocks own a TY x TZ tile of node columns and walk CX node planes in x.
-// Element plane ex is contracted in four classes of (ey, ez) parity; elements
-// of one class share no node, so the class's contributions are added to the
-// LDS accumulators without conflicts, and every node receives its eight
-// contributions in a fixed order: element plane below, then above, each in
-// class order.  Node plane ex is complete after element plane ex and is
-// written out (boundary rows 0: the Dirichlet projection).
+// Element plane ex is contracted in two phases of element-row parity: rows
+// of one parity share no node row, and one wave takes a row, adds each pair
+// of z-neighbouring elements' contributions in registers (a cross-lane swap)
+// and updates each LDS accumulator word once, so every node receives its
+// eight contributions in a fixed order.  Node plane ex is complete after
+// element plane ex and is written out (boundary rows 0: the Dirichlet
+// projection).
 //
 // Layout: node-major, right-hand-side-minor: X[node * 16 + r].
 #pragma once
@@ -30,7 +31,7 @@ namespace heat {
 
 constexpr int MR = 16;                                   // right-hand sides
 constexpr int MTY = 15, MTZ = 16, MCX = 16;              // node tile and planes per block
-constexpr int MNT = 512;                                 // threads: 8 waves, one element row per wave and class
+constexpr int MNT = 512;                                 // threads: 8 waves, one element row per wave and phase
 
 typedef float mr_v4f __attribute__((ext_vector_type(4)));
 
@@ -85,7 +86,6 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, q = lane >> 4;                        // RHS column; k / row group
     const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
-    const int gy = q & 1, gz = q >> 1;                               // output node pair (y, z) of group q
     const float a0 = mrhs_coef(col, q), a1 = mrhs_coef(col, 4 + q);  // A: row = lane % 16, k = lane / 16
     constexpr int NK = SUMS == 2 ? 4 : 1;                            // sums per column
     double acc_s[NK][4];                                             // this thread's 4 columns: part = tid & 3
@@ -160,67 +160,66 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
             const float* xlane = (qx ? xhi : xlo) + qy * WZ * MR + col;      // B operand base of this lane
             float* ylo = ys + (ex & 1) * YSTRIDE + YBASE;
             float* yhi = ys + ((ex + 1) & 1) * YSTRIDE + YBASE;
-            // element rows of one class per wave (rows ky = ky0 + 2 kyi, kyi = wave
-            // mod 8), each row a compile-time run along z: every LDS offset in
-            // the run is an immediate, no per-element index arithmetic and no
-            // branch or mask is left.  Elements outside the mesh have zero
-            // coefficients (staging) and add zeros; halo outputs land in the
-            // accumulator pads.  Both planes' sums are always formed: a plane
-            // this block does not own is never written out and its accumulator
-            // is cleared after the plane.
-            auto run_row = [&](auto kz0c, int ky) __attribute__((always_inline)) {
-                constexpr int KZ0 = decltype(kz0c)::value;
-                const float* __restrict__ er_row = &ec[ex & 1][0][ky * EZ + KZ0];
-                const float* __restrict__ ek_row = &ec[ex & 1][1][ky * EZ + KZ0];
-                const float* __restrict__ x_row = xlane + (ky * WZ + KZ0) * MR;
-                const int o_row = (ky - 1 + gy) * YROW + (KZ0 - 1 + gz) * MR + col;   // output of element KZ0
-                constexpr int NE = (EZ - KZ0 + 1) / 2;                            // elements in the run
-                // all contractions of the run first (a back-to-back stream for
-                // the matrix pipe), then the accumulator updates
-                mr_v4f c[NE];
+            // Element rows of one parity share no node row: per phase each
+            // wave takes one row and walks its EZ elements in z order.  The
+            // MFMA outputs of element kz hold, in lanes 0-31 (gz = 0), its
+            // contributions to node column kz - 1 and, in lanes 32-63 (gz = 1),
+            // to node column kz, each for the lower (ya) and the upper (yb)
+            // node plane.  One v_permlane32_swap of (ya, yb) gives column
+            // kz - 1's lower- and upper-plane parts side by side (lanes 0-31
+            // and 32-63) and, in a second register, column kz's, which waits
+            // for the next element.  Each accumulator word then takes one
+            // read-modify-write per phase: lane group q = 0, 1 the lower node
+            // plane and rows gy = q, q = 2, 3 the upper plane and rows q - 2.
+            // Node (qx, qy, 1) of element kz is node (qx, qy, 0) of element
+            // kz + 1, so each B operand is read once.  Elements outside the
+            // mesh have zero coefficients (staging) and add zeros; columns -1
+            // and TZ are the tile's pads and are not written.  Both planes'
+            // sums are always formed: a plane this block does not own is never
+            // written out and its accumulator is cleared after the plane.
+            float* const yq = ((q >> 1) ? yhi : ylo) + (q & 1) * YROW + col;
+            auto run_row = [&](int ky) __attribute__((always_inline)) {
+                const float* __restrict__ er_row = &ec[ex & 1][0][ky * EZ];
+                const float* __restrict__ ek_row = &ec[ex & 1][1][ky * EZ];
+                const float* __restrict__ x_row = xlane + ky * WZ * MR;
+                float* __restrict__ y_row = yq + (ky - 1) * YROW;          // column 0 of its rows
+                float bprev = x_row[0];
+                float carry = 0.f;
+                auto batch = [&](auto k0c, auto nbc) __attribute__((always_inline)) {
+                    constexpr int K0 = decltype(k0c)::value, NB = decltype(nbc)::value;
+                    mr_v4f c[NB];
 #pragma unroll
-                for (int kzi = 0; kzi < NE; ++kzi) {
-                    const float b0 = x_row[(2 * kzi) * MR];                       // node (qx, qy, 0)
-                    const float b1 = x_row[(2 * kzi + 1) * MR];                   // node (qx, qy, 1)
-                    const mr_v4f zero = {0.f, 0.f, 0.f, 0.f};
-                    c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, zero, 0, 0, 0);
-                    c[kzi] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[kzi], 0, 0, 0);
-                }
-                // the run's accumulator words are distinct (its elements share
-                // no node): read them all, add, then write them all, so the
-                // reads wait once instead of once per element
-                float la[NE], lb[NE];
+                    for (int i = 0; i < NB; ++i) {
+                        const float b1 = x_row[(K0 + i + 1) * MR];
+                        const mr_v4f zero = {0.f, 0.f, 0.f, 0.f};
+                        c[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bprev, zero, 0, 0, 0);
+                        c[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[i], 0, 0, 0);
+                        bprev = b1;
+                    }
 #pragma unroll
-                for (int kzi = 0; kzi < NE; ++kzi) {
-                    la[kzi] = ylo[o_row + 2 * kzi * MR];
-                    lb[kzi] = yhi[o_row + 2 * kzi * MR];
-                }
-#pragma unroll
-                for (int kzi = 0; kzi < NE; ++kzi) {
-                    const float ce_r = er_row[2 * kzi];
-                    const float ce_k = ek_row[2 * kzi];
-                    const float ya = MASS ? ce_r * c[kzi][0] : fmaf(ce_r, c[kzi][0], ce_k * c[kzi][2]);
-                    const float yb = MASS ? ce_r * c[kzi][1] : fmaf(ce_r, c[kzi][1], ce_k * c[kzi][3]);
-                    la[kzi] += ya;
-                    lb[kzi] += yb;
-                }
-#pragma unroll
-                for (int kzi = 0; kzi < NE; ++kzi) {
-                    ylo[o_row + 2 * kzi * MR] = la[kzi];
-                    yhi[o_row + 2 * kzi * MR] = lb[kzi];
-                }
+                    for (int i = 0; i < NB; ++i) {
+                        const int kz = K0 + i;
+                        const float ce_r = er_row[kz];
+                        const float ce_k = ek_row[kz];
+                        const float ya = MASS ? ce_r * c[i][0] : fmaf(ce_r, c[i][0], ce_k * c[i][2]);
+                        const float yb = MASS ? ce_r * c[i][1] : fmaf(ce_r, c[i][1], ce_k * c[i][3]);
+                        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(ya), __float_as_uint(yb),
+                                                                         false, false);
+                        const float cur = __uint_as_float(sw[0]) + carry;   // column kz - 1, complete
+                        carry = __uint_as_float(sw[1]);                     // column kz, this element's part
+                        if (kz >= 1) y_row[(kz - 1) * MR] += cur;
+                    }
+                };
+                static_assert(EZ == 17, "two batches of the row's elements");
+                batch(std::integral_constant<int, 0>{}, std::integral_constant<int, 9>{});
+                batch(std::integral_constant<int, 9>{}, std::integral_constant<int, 8>{});
             };
 #pragma unroll 1
-            for (int cls = 0; cls < 4; ++cls) {
-                const int cy = cls >> 1, cz = cls & 1;
-                const int ky0 = (cy - (y0 - 1)) & 1, kz0 = (cz - (z0 - 1)) & 1;
+            for (int cy = 0; cy < 2; ++cy) {
+                const int ky0 = (cy - (y0 - 1)) & 1;
                 const int nyc = (EY - ky0 + 1) >> 1;
 #pragma unroll 1
-                for (int kyi = wave; kyi < nyc; kyi += MNT / 64) {
-                    const int ky = ky0 + 2 * kyi;
-                    if (kz0) run_row(std::integral_constant<int, 1>{}, ky);
-                    else run_row(std::integral_constant<int, 0>{}, ky);
-                }
+                for (int kyi = wave; kyi < nyc; kyi += MNT / 64) run_row(ky0 + 2 * kyi);
                 __syncthreads();
             }
         } else {

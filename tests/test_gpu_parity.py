@@ -880,7 +880,11 @@ def test_mrhs_apply(n, dt_eff):
     """Sixteen right-hand sides through the matrix-core contraction: every
     column within FP32 rounding of the oracle's FP32 apply of that column
     (and of the FP64 apply), columns independent of each other bit for bit,
-    and exact scaling by 2."""
+    and exact scaling by 2.  The rounding bar is relative to the column's
+    magnitude: its largest value plus the largest mass apply of |x| (the sum
+    of absolute mass contributions at a node), since the summation orders of
+    the matrix-core kernel and of the oracle differ and a node's sum may
+    cancel."""
     print(f"[case] mrhs n={n} dt_eff={dt_eff}", flush=True)
     dt = 1e-3
     k, rhoc = make_field(n, seed=70 + n)
@@ -896,7 +900,7 @@ def test_mrhs_apply(n, dt_eff):
     for r in range(16):
         y32 = orc.apply_f32(X[:, r].copy(), dt_eff)
         y64 = orc.apply(X[:, r].astype(np.float64), dt_eff)
-        scale = max(np.max(np.abs(y64)), 1e-30)
+        scale = max(np.max(np.abs(y64)) + np.max(orc.apply(np.abs(X[:, r]).astype(np.float64), 0.0)), 1e-30)
         assert np.max(np.abs(Y[:, r] - y32)) <= 4e-6 * scale, r
         assert np.max(np.abs(Y[:, r] - y64)) <= 4e-6 * scale, r
     # a column's result does not depend on the other columns
