@@ -121,6 +121,7 @@ struct heat_op {
     // one-launch inner iteration; the partial arrays hold every chunk count
     int geo_planes[2] = {FCX, FCX}, geo_wave[2] = {FWAVE, FWAVE};
     int batch_refs = 0;            // batches whose views hold this handle's geometry
+    int mrhs_cx = 0;               // node planes per block of the 16-RHS apply (0: not chosen yet)
     // the one-launch inner iteration of the refined solve (heat_cg1.h)
     int inner_mode = HEAT_INNER_ONE_LAUNCH;
     float *c1_r = nullptr, *c1_s[2] = {nullptr, nullptr}, *c1_w[2] = {nullptr, nullptr};
@@ -827,12 +828,39 @@ int setup_impl(const heat_config* cfg, const double* k, const double* rhoc, heat
 
 int ensure_mixed(heat_op* op);
 
-// tiles of the matrix-core 16-RHS apply of a mesh
-long long mrhs_tiles(const Grid& g, int* nby, int* nbz) {
+// Node planes per block of the 16-RHS apply of a handle's mesh, chosen once:
+// the length in [MCX_MIN, MCX_MAX] that minimises (waves of resident blocks)
+// x (planes + 1), the extra plane being the halo plane each chunk loads
+// again; ties go to the shorter length.  The tile partition (and with it the
+// summation order of the tile sums) is a function of the mesh size and the
+// device's resident block count.
+int mrhs_chunk(heat_op* op) {
+    if (op->mrhs_cx) return op->mrhs_cx;
+    const Grid& g = op->g;
+    const int slots = resident_blocks((const void*)k_mrhs_apply<false>, MNT);
+    const long long cols = (long long)((g.n + MTY) / MTY) * ((g.n + MTZ) / MTZ);
+    int best = 16;
+    if (slots > 0) {
+        long long bcost = -1;
+        for (int cx = MCX_MIN; cx <= MCX_MAX; ++cx) {
+            const long long nblk = cols * ((g.nn1 + cx - 1) / cx);
+            const long long cost = ((nblk + slots - 1) / slots) * (cx + 1);
+            if (bcost < 0 || cost < bcost) {
+                bcost = cost;
+                best = cx;
+            }
+        }
+    }
+    op->mrhs_cx = best;
+    return best;
+}
+
+// tiles of the matrix-core 16-RHS apply of a mesh with cx node planes per block
+long long mrhs_tiles(const Grid& g, int cx, int* nby, int* nbz) {
     const int n = g.n;
     *nby = (n + 1 + MTY - 1) / MTY;
     *nbz = (n + 1 + MTZ - 1) / MTZ;
-    const int nch = (n + 1 + MCX - 1) / MCX;
+    const int nch = (n + 1 + cx - 1) / cx;
     return (long long)*nby * *nbz * nch;
 }
 
@@ -841,15 +869,16 @@ long long mrhs_tiles(const Grid& g, int* nby, int* nbz) {
 int launch_mrhs(heat_op* op, const float* X, float* Y, bool mass, hipStream_t s, int sums = 0,
                 double* part = nullptr, const double* w = nullptr) {
     int nby, nbz;
-    const long long nt = mrhs_tiles(op->g, &nby, &nbz);
+    const int cx = mrhs_chunk(op);
+    const long long nt = mrhs_tiles(op->g, cx, &nby, &nbz);
     if (nt > (1LL << 30)) return fail(HEAT_E_ARG, "mesh too large for the multi-RHS tiling");
     const int ntiles = (int)nt;
     const unsigned grid = (unsigned)(((ntiles + NXCD - 1) / NXCD) * NXCD);
     const Grid& g = op->g;
-    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles);
-    else if (sums == 1) k_mrhs_apply<false, 1><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, part, w);
-    else if (sums == 2) k_mrhs_apply<false, 2><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, part, w);
-    else k_mrhs_apply<false><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles);
+    if (mass) k_mrhs_apply<true><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, cx);
+    else if (sums == 1) k_mrhs_apply<false, 1><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, cx, part, w);
+    else if (sums == 2) k_mrhs_apply<false, 2><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, cx, part, w);
+    else k_mrhs_apply<false><<<grid, MNT, 0, s>>>(X, Y, op->cr32, op->ck32, g, nby, nbz, ntiles, cx);
     CHECK_LAUNCH();
     return HEAT_OK;
 }
@@ -2627,14 +2656,14 @@ int heat_batch_step_mrhs(heat_batch* b, int nsteps, int max_outer, int max_inner
         HIP_TRY(hipMalloc(&b->mrAP, bytes));
         HIP_TRY(hipMalloc(&b->mrpart, sizeof(double) * MRK * MRC * MRB));
         int nby_, nbz_;
-        HIP_TRY(hipMalloc(&b->mrtile, sizeof(double) * MRK * MRC * (size_t)mrhs_tiles(g, &nby_, &nbz_)));
+        HIP_TRY(hipMalloc(&b->mrtile, sizeof(double) * MRK * MRC * (size_t)mrhs_tiles(g, mrhs_chunk(m), &nby_, &nbz_)));
         HIP_TRY(hipMalloc(&b->mrst, sizeof(MrState)));
         HIP_TRY(hipHostMalloc(&b->hmrst, sizeof(MrState), hipHostMallocDefault));
     }
     const bool detect = m->cfg.detect != 0;
     const double* w = m->vec[HEAT_VEC_WA];
     int nby_mr, nbz_mr;
-    const int ntiles_mr = (int)mrhs_tiles(g, &nby_mr, &nbz_mr);
+    const int ntiles_mr = (int)mrhs_tiles(g, mrhs_chunk(m), &nby_mr, &nbz_mr);
     const int nblk_nodes = (b->node_off[count] + NT - 1) / NT;
     auto read_state = [&]() -> int {
         HIP_TRY(hipMemcpyAsync(b->hmrst, b->mrst, sizeof(MrState), hipMemcpyDeviceToHost, s));

@@ -14,7 +14,8 @@
 // b = (1, y, z), so each lane holds both terms of its two outputs and no
 // lane exchange is needed.
 //
-// Blocks own a TY x TZ tile of node columns and walk CX node planes in x.
+// Blocks own a TY x TZ tile of node columns and walk cx node planes in x
+// (a launch argument: the host picks it per mesh to fill the last wave).
 // Element plane ex is contracted in two phases of element-row parity: rows
 // of one parity share no node row, and one wave takes a row, adds each pair
 // of z-neighbouring elements' contributions in registers (a cross-lane swap)
@@ -30,7 +31,8 @@
 namespace heat {
 
 constexpr int MR = 16;                                   // right-hand sides
-constexpr int MTY = 15, MTZ = 16, MCX = 16;              // node tile and planes per block
+constexpr int MTY = 15, MTZ = 16;                        // node tile per block
+constexpr int MCX_MIN = 8, MCX_MAX = 64;                 // node planes per block (host-chosen)
 constexpr int MNT = 512;                                 // threads: 8 waves, one element row per wave and phase
 
 typedef float mr_v4f __attribute__((ext_vector_type(4)));
@@ -55,10 +57,10 @@ __device__ __forceinline__ float mrhs_coef(int row, int j) {
 template <bool MASS, int SUMS = 0>
 __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X, float* __restrict__ Y,
                                                     const float* __restrict__ cr, const float* __restrict__ ck,
-                                                    Grid g, int nby, int nbz, int ntiles,
+                                                    Grid g, int nby, int nbz, int ntiles, int cx,
                                                     double* __restrict__ part = nullptr,
                                                     const double* __restrict__ w = nullptr) {
-    constexpr int TY = MTY, TZ = MTZ, CX = MCX;
+    constexpr int TY = MTY, TZ = MTZ;
     constexpr int WY = TY + 2, WZ = TZ + 2, WN = WY * WZ;          // node window with halo
     constexpr int EY = TY + 1, EZ = TZ + 1, EN = EY * EZ;          // elements per plane
     constexpr int XSTRIDE = WN * MR + 16;                          // slot stride: odd multiple of 16 floats
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     const int n = g.n, nn1 = g.nn1, nsq = n * n;
     const size_t S = (size_t)g.sx;
     const int y0 = by * TY, z0 = bz * TZ;
-    const int ix0 = chunk * CX, ix1 = min(ix0 + CX, n + 1);
+    const int ix0 = chunk * cx, ix1 = min(ix0 + cx, n + 1);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, q = lane >> 4;                        // RHS column; k / row group
     const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
