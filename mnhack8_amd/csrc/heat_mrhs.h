@@ -9,7 +9,12 @@
 //   Q_i = sum_j {-4,0,1,1}[d(i,j)] x_j     (kappa K row = k2 Q_i)
 // so y_i = cr_e P_i + ck_e Q_i with cr = rho*c m3 and ck = dt kappa k2, the
 // same coefficients and final fused multiply-add as the single-vector
-// kernels.  The 16 x 8 by 8 x 16 product is two v_mfma_f32_16x16x4f32.
+// kernels.  The 16 x 8 by 8 x 16 product is one v_mfma_f32_16x16x32_bf16:
+// each FP32 value x is split exactly into three bfloat16 terms by truncation
+// (hi = the top 16 bits of x, mid = the top 16 bits of x - hi, lo = x - hi -
+// mid, which has at most 8 significant bits), the contraction matrix's small
+// integers are exact in bfloat16, so every product is exact and the matrix
+// core sums the 24 of them in FP32: K = 32 holds, per node, (hi, mid, lo, 0).
 // Rows are ordered (P_a, P_b, Q_a, Q_b) per pair of nodes a = (0, y, z),
 // b = (1, y, z), so each lane holds both terms of its two outputs and no
 // lane exchange is needed.
@@ -36,6 +41,19 @@ constexpr int MCX_MIN = 8, MCX_MAX = 64;                 // node planes per bloc
 constexpr int MNT = 512;                                 // threads: 8 waves, one element row per wave and phase
 
 typedef float mr_v4f __attribute__((ext_vector_type(4)));
+typedef unsigned mr_v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 mr_v8bf __attribute__((ext_vector_type(8)));
+
+// the exact three-term bfloat16 split of x as two words of the B operand:
+// (hi | mid << 16, lo)
+__device__ __forceinline__ void mrhs_split(float x, unsigned& w0, unsigned& w1) {
+    const unsigned xb = __float_as_uint(x);
+    const float r1 = x - __uint_as_float(xb & 0xffff0000u);          // exact: same sign and exponent
+    const unsigned r1b = __float_as_uint(r1);
+    const float r2 = r1 - __uint_as_float(r1b & 0xffff0000u);        // exact, at most 8 significant bits
+    w0 = __builtin_amdgcn_perm(r1b, xb, 0x07060302u);                // upper halves of x and r1
+    w1 = __float_as_uint(r2) >> 16;
+}
 
 // entry (row, j) of the contraction matrix: row = 4 g + t, g = (y | z << 1) of
 // the node pair, t = 0: P_a, 1: P_b, 2: Q_a, 3: Q_b; column j = local node
@@ -88,7 +106,17 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, q = lane >> 4;                        // RHS column; k / row group
     const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
-    const float a0 = mrhs_coef(col, q), a1 = mrhs_coef(col, 4 + q);  // A: row = lane % 16, k = lane / 16
+    // A: row = lane % 16; lane group q holds node q's three terms and a zero,
+    // then node 4 + q's (the same slots as the B operand's)
+    mr_v4u au;
+    {
+        const unsigned c0 = __float_as_uint(mrhs_coef(col, q)) >> 16, c1 = __float_as_uint(mrhs_coef(col, 4 + q)) >> 16;
+        au[0] = c0 | (c0 << 16);
+        au[1] = c0;
+        au[2] = c1 | (c1 << 16);
+        au[3] = c1;
+    }
+    const mr_v8bf amat = __builtin_bit_cast(mr_v8bf, au);
     constexpr int NK = SUMS == 2 ? 4 : 1;                            // sums per column
     double acc_s[NK][4];                                             // this thread's 4 columns: part = tid & 3
 #pragma unroll
@@ -174,7 +202,7 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
             // read-modify-write per phase: lane group q = 0, 1 the lower node
             // plane and rows gy = q, q = 2, 3 the upper plane and rows q - 2.
             // Node (qx, qy, 1) of element kz is node (qx, qy, 0) of element
-            // kz + 1, so each B operand is read once.  Elements outside the
+            // kz + 1, so each B operand is read and split once.  Elements outside the
             // mesh have zero coefficients (staging) and add zeros; columns -1
             // and TZ are the tile's pads and are not written.  Both planes'
             // sums are always formed: a plane this block does not own is never
@@ -185,18 +213,21 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
                 const float* __restrict__ ek_row = &ec[ex & 1][1][ky * EZ];
                 const float* __restrict__ x_row = xlane + ky * WZ * MR;
                 float* __restrict__ y_row = yq + (ky - 1) * YROW;          // column 0 of its rows
-                float bprev = x_row[0];
+                unsigned bp0, bp1;
+                mrhs_split(x_row[0], bp0, bp1);
                 float carry = 0.f;
                 auto batch = [&](auto k0c, auto nbc) __attribute__((always_inline)) {
                     constexpr int K0 = decltype(k0c)::value, NB = decltype(nbc)::value;
                     mr_v4f c[NB];
 #pragma unroll
                     for (int i = 0; i < NB; ++i) {
-                        const float b1 = x_row[(K0 + i + 1) * MR];
+                        unsigned bn0, bn1;
+                        mrhs_split(x_row[(K0 + i + 1) * MR], bn0, bn1);
+                        const mr_v4u bu = {bp0, bp1, bn0, bn1};
                         const mr_v4f zero = {0.f, 0.f, 0.f, 0.f};
-                        c[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bprev, zero, 0, 0, 0);
-                        c[i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, c[i], 0, 0, 0);
-                        bprev = b1;
+                        c[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(amat, __builtin_bit_cast(mr_v8bf, bu), zero, 0, 0, 0);
+                        bp0 = bn0;
+                        bp1 = bn1;
                     }
 #pragma unroll
                     for (int i = 0; i < NB; ++i) {
