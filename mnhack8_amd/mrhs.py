@@ -88,6 +88,42 @@ def bench_apply(n=320, reps=20, log=None):
     return out
 
 
+def rounding(n=96, seed=7):
+    """Rounding of the 16-RHS apply against the FP64 single-vector apply of
+    each column on the workload fields at mesh n: eight bump columns and
+    eight standard-normal columns; per node, the error relative to the
+    column's largest |y|.  Returns max, rms and mean per column kind."""
+    import torch
+    from .heat import HeatOperator
+    k, rhoc = workload_fields(n)
+    dt = 1e-4
+    op = HeatOperator(n, dt, k.ravel(), rhoc.ravel(), kernel="fast", scatter="colored", source="const",
+                      source_value=1.0)
+    rng = np.random.default_rng(seed)
+    X = np.concatenate([bumps(n)[:, :8], rng.standard_normal((op.nn, 8)).astype(np.float32)], axis=1)
+    Xd = torch.from_numpy(np.ascontiguousarray(X)).cuda()
+    Yd = torch.zeros_like(Xd)
+    op.apply_mrhs(Xd, Yd, dt)
+    torch.cuda.synchronize()
+    Y = Yd.cpu().numpy().astype(np.float64)
+    out = {"n": n}
+    for name, cols in (("bumps", range(8)), ("normal", range(8, 16))):
+        mx, rms, mean = [], [], []
+        for r in cols:
+            x = torch.from_numpy(X[:, r].astype(np.float64)).cuda()
+            y = torch.zeros_like(x)
+            op.apply_operator(x, y, dt)
+            torch.cuda.synchronize()
+            y64 = y.cpu().numpy()
+            d = (Y[:, r] - y64) / np.abs(y64).max()
+            mx.append(float(np.abs(d).max()))
+            rms.append(float(np.sqrt((d ** 2).mean())))
+            mean.append(float(d.mean()))
+        out[name] = {"max": max(mx), "rms": float(np.mean(rms)), "mean": float(np.mean(mean))}
+    op.cleanup()
+    return out
+
+
 def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -96,9 +132,14 @@ def main(argv=None):
     ap.add_argument("--refined-steps", type=int, default=0,
                     help="also time this many steps of the 16-column refined solve (matrix-core operator)")
     ap.add_argument("--max-outer", type=int, default=12)
+    ap.add_argument("--rounding", type=int, default=0, metavar="N",
+                    help="also report the apply's rounding against the FP64 apply at mesh N")
     a = ap.parse_args(argv)
     log = lambda m: print(f"[mrhs] {m}", file=sys.stderr, flush=True)
     out = {"apply": bench_apply(a.n, log=log)}
+    if a.rounding:
+        out["rounding"] = rounding(a.rounding)
+        log(f"rounding at N={a.rounding}: {out['rounding']}")
     if a.solve_steps:
         out["solve"] = bench_solve(a.n, a.solve_steps, log=log)
     if a.refined_steps:
