@@ -25,12 +25,13 @@
 // (cg1_chunks); a block of 256 threads walks its chunk plane by plane,
 // forming u of the incoming plane (tile and halo) in registers, parking it
 // in LDS, and applying the colour-ordered stencil (the same sums as the
-// FP32 apply) to the plane behind it.  Sums are FP64 sums of FP32 products,
-// per thread in plane order (for the vector terms, the thread's two halo-tile
-// slots in slot order), then a 256-wide tree per tile; every block of the
-// next launch folds the tiles' partials in the same fixed order (or, for
-// many tiles, one block does it once).  The CPU oracle restates all of this
-// (ho_pcg_solve_mixed_cg1).
+// FP32 apply) to the plane behind it.  Thread t owns tile node (t / 32,
+// t % 32): it updates that node's vectors, forms its stencil and sums its
+// terms; threads 0..83 also form u at one node of the tile's halo ring.
+// Sums are FP64 sums of FP32 products, per thread in plane order, then a
+// 256-wide tree per tile; every block of the next launch folds the tiles'
+// partials in the same fixed order (or, for many tiles, one block does it
+// once).  The CPU oracle restates all of this (ho_pcg_solve_mixed_cg1).
 #pragma once
 #include "heat_pcg_fused.h"
 
@@ -38,7 +39,8 @@ namespace heat {
 
 constexpr int C1TY = 8, C1TZ = 32, C1NT = C1TY * C1TZ;
 constexpr int C1XW = C1TZ + 2, C1XN = C1XW * (C1TY + 2);     // u tile with its halo: 34 x 10
-constexpr int C1XR = (C1XN + C1NT - 1) / C1NT;               // slots per thread
+constexpr int C1HN = C1XN - C1NT;                            // halo ring: 84 nodes
+static_assert(C1HN <= C1NT, "one halo node per thread at most");
 constexpr int C1EW = C1TZ + 1, C1EN = C1EW * (C1TY + 1);     // element tile: 33 x 9
 constexpr int C1ER = (C1EN + C1NT - 1) / C1NT;
 constexpr int C1ROWS = 8;                                    // partial rows (see cg1_head)
@@ -245,20 +247,31 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
     constexpr int KX = START ? 1 : (FOLD8 ? 8 : (DETECT ? 6 : 3));
     double xpart[KX][4];
     if constexpr (!START) partials_load4<KX>(v.tp[par], v.pre ? 0 : v.ntiles, v.ntiles, xpart);
-    // in-plane node offsets (elements) of this thread's slots of the u tile,
-    // whether each is an interior node and one of this tile's own, and the
-    // element tile's; every array is one buffer resource, and a slot's byte
-    // offset is formed once per plane (C1_OOB off the interior)
-    int xq[C1XR];
-    bool xin[C1XR], xown[C1XR];
-#pragma unroll
-    for (int k = 0; k < C1XR; ++k) {
-        const int i = tid + k * C1NT;
-        const int r = i / C1XW, c = i - (i / C1XW) * C1XW;
+    // this thread's two slots of the u tile: slot 0 its own node (ty, tz),
+    // slot 1 one node of the halo ring (threads below C1HN: rows 0 and 9,
+    // then columns 0 and 33 of rows 1..8); per slot the in-plane node offset
+    // (elements), the LDS index and whether the node is interior.  Every
+    // array is one buffer resource, and a slot's byte offset is formed once
+    // per plane (C1_OOB off the interior)
+    int xq[2], xi[2];
+    bool xin[2];
+    xq[0] = iy * nn1 + iz;
+    xi[0] = (ty + 1) * C1XW + (tz + 1);
+    xin[0] = node_in;
+    {
+        int r, c;
+        if (tid < 2 * C1XW) {
+            r = tid < C1XW ? 0 : C1TY + 1;
+            c = tid < C1XW ? tid : tid - C1XW;
+        } else {
+            const int k = tid - 2 * C1XW;
+            r = 1 + (k >> 1);
+            c = (k & 1) ? C1XW - 1 : 0;
+        }
         const int yy = y0 - 1 + r, zz = z0 - 1 + c;
-        xin[k] = i < C1XN && yy >= 1 && yy <= m && zz >= 1 && zz <= m;
-        xown[k] = xin[k] && r >= 1 && r <= C1TY && c >= 1 && c <= C1TZ;
-        xq[k] = yy * nn1 + zz;
+        xq[1] = yy * nn1 + zz;
+        xi[1] = r * C1XW + c;
+        xin[1] = tid < C1HN && yy >= 1 && yy <= m && zz >= 1 && zz <= m;
     }
     int eq[C1ER];
     bool eok[C1ER];
@@ -270,7 +283,7 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
         eok[k] = i < C1EN && ey < n && ez < n;
         eq[k] = ey * n + ez;
     }
-    const int wq = iy * nn1 + iz;
+    const int wq = xq[0];
     int eidx[2][2], eoff[2][2];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
@@ -285,8 +298,8 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
 #pragma unroll
         for (int c = 0; c < 3; ++c) widx[b2][c] = (ty + 1 + (b2 - 1) * sy) * C1XW + (tz + 1 + (c - 1) * sz);
     struct Stage {
-        float r[C1XR], w[C1XR], s[C1XR], d[C1XR], e[C1XR], p[C1XR];
-        double bb[C1XR], ax[C1XR];
+        float r[2], w[2], s[2], d[2], e, p;             // by slot; e and p at the own node only
+        double bb[2], ax[2];
         float cr[C1ER], ck[C1ER];
         double wa;                                      // D1 weight at the own node of the plane behind
     };
@@ -307,7 +320,7 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
         const bool pown = P >= ix0 && P < ix1;
         const int pq = P * S;
 #pragma unroll
-        for (int k = 0; k < C1XR; ++k) {
+        for (int k = 0; k < 2; ++k) {
             const unsigned o4 = (pin && xin[k]) ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
             St.d[k] = c1_ldf(R_d, o4);
             if (START) {
@@ -315,13 +328,15 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
                 St.bb[k] = c1_ldd(R_b, o8);
                 St.ax[k] = c1_ldd(R_ax, o8);
             } else {
-                const unsigned ow = (pown && xown[k]) ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
                 St.r[k] = c1_ldf(R_rin, o4);
                 St.w[k] = c1_ldf(R_win, o4);
                 St.s[k] = c1_ldf(R_sin, o4);
-                St.e[k] = c1_ldf(R_e, ow);
-                St.p[k] = c1_ldf(R_p, ow);
             }
+        }
+        if (!START) {
+            const unsigned ow = (pown && node_in) ? (unsigned)(pq + xq[0]) * 4u : C1_OOB;
+            St.e = c1_ldf(R_e, ow);
+            St.p = c1_ldf(R_p, ow);
         }
         const int ep = P - 1;
         const bool epin = ep >= 0 && ep < n && P <= ix1;
@@ -350,41 +365,42 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
     double a_g = 0.0, a_rr = 0.0, a_bb = 0.0, a_r64 = 0.0;
     double a_dl = 0.0, a_1w = 0.0, a_c = 0.0, a_uu = 0.0;
     const float af = (float)alpha, bf = (float)beta;
-    // u of plane P (zero off the interior), own nodes updated and stored
+    // u of plane P (zero off the interior) at both slots; the own node
+    // updated, stored and summed
     auto form = [&](int P, int slot, const Stage& St) __attribute__((always_inline)) {
         const bool own = P >= ix0 && P < ix1;           // uniform over the block
-        const int pq = P * S;
-#pragma unroll
-        for (int k = 0; k < C1XR; ++k) {
-            const int i = tid + k * C1NT;
-            const bool mine = own && xown[k];
-            const unsigned ow = mine ? (unsigned)(pq + xq[k]) * 4u : C1_OOB;
-            float un;
-            if (START) {
-                const double r64 = fma(-1.0, St.ax[k], St.bb[k]);
-                const float rf = (float)r64;
-                un = rf * St.d[k];
-                c1_stf(rf, R_rout, ow);
-                c1_stf(0.0f, R_e, ow);
-                a_g += mine ? (double)rf * (double)un : 0.0;
-                a_rr += mine ? (double)rf * (double)rf : 0.0;
-                a_bb += mine ? St.bb[k] * St.bb[k] : 0.0;
-                a_r64 += mine ? r64 * r64 : 0.0;
-            } else {
-                const float sn = first ? St.w[k] : fmaf(bf, St.s[k], St.w[k]);
-                const float rn = fmaf(-af, sn, St.r[k]);
-                un = rn * St.d[k];
-                const float uo = St.r[k] * St.d[k];
-                const float pn = first ? uo : fmaf(bf, St.p[k], uo);
-                c1_stf(fmaf(af, pn, St.e[k]), R_e, ow);
-                c1_stf(pn, R_p, ow);
-                c1_stf(sn, R_sout, ow);
-                c1_stf(rn, R_rout, ow);
-                a_g += mine ? (double)rn * (double)un : 0.0;
-                a_rr += mine ? (double)rn * (double)rn : 0.0;
-            }
-            if (i < C1XN) us[slot][i] = un;
+        const bool mine = own && node_in;
+        const unsigned ow = mine ? (unsigned)(P * S + xq[0]) * 4u : C1_OOB;
+        float u0, u1;
+        if (START) {
+            const double r64 = fma(-1.0, St.ax[0], St.bb[0]);
+            const float rf = (float)r64;
+            u0 = rf * St.d[0];
+            c1_stf(rf, R_rout, ow);
+            c1_stf(0.0f, R_e, ow);
+            a_g += mine ? (double)rf * (double)u0 : 0.0;
+            a_rr += mine ? (double)rf * (double)rf : 0.0;
+            a_bb += mine ? St.bb[0] * St.bb[0] : 0.0;
+            a_r64 += mine ? r64 * r64 : 0.0;
+            u1 = (float)fma(-1.0, St.ax[1], St.bb[1]) * St.d[1];
+        } else {
+            const float sn = first ? St.w[0] : fmaf(bf, St.s[0], St.w[0]);
+            const float rn = fmaf(-af, sn, St.r[0]);
+            u0 = rn * St.d[0];
+            const float uo = St.r[0] * St.d[0];
+            const float pn = first ? uo : fmaf(bf, St.p, uo);
+            c1_stf(fmaf(af, pn, St.e), R_e, ow);
+            c1_stf(pn, R_p, ow);
+            c1_stf(sn, R_sout, ow);
+            c1_stf(rn, R_rout, ow);
+            a_g += mine ? (double)rn * (double)u0 : 0.0;
+            a_rr += mine ? (double)rn * (double)rn : 0.0;
+            // the halo node: the same u its own tile forms
+            const float sh = first ? St.w[1] : fmaf(bf, St.s[1], St.w[1]);
+            u1 = fmaf(-af, sh, St.r[1]) * St.d[1];
         }
+        us[slot][xi[0]] = u0;
+        if (tid < C1HN) us[slot][xi[1]] = u1;
 #pragma unroll
         for (int k = 0; k < C1ER; ++k) {
             const int i = tid + k * C1NT;

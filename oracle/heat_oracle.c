@@ -1112,8 +1112,7 @@ int ho_pcg_solve_mixed_fused(ho_op* op, const double* b, double* x, int max_oute
 /* heat_cg1.h): preconditioned CG in the Chronopoulos-Gear arrangement, its   */
 /* tiles and its summation order.                                            */
 /* ------------------------------------------------------------------------- */
-enum { C1_TY = 8, C1_TZ = 32, C1_XW = C1_TZ + 2, C1_XN = (C1_TZ + 2) * (C1_TY + 2),
-       C1_XR = (C1_XN + F_NT - 1) / F_NT };
+enum { C1_TY = 8, C1_TZ = 32 };
 
 typedef struct { int m, nby, nbz, nch; long ntiles; } c1_geom;
 
@@ -1129,12 +1128,10 @@ static c1_geom cg1_geom(const ho_grid* g) {
 }
 
 /* A row of tile sums: the term A[i] * B[i] (or A[i] when B is NULL) over the
- * tile's own interior nodes, per thread either in slot order (ORDER 0: the
- * thread's nodes of the 34 x 10 halo tile, plane by plane, its two slots in
- * order) or in stencil order (ORDER 1: the thread's node (y0 + tid / 32,
- * z0 + tid % 32), plane by plane), then the 256-wide tree, then the fold of
+ * tile's own interior nodes, per thread over its node (y0 + tid / 32,
+ * z0 + tid % 32) plane by plane, then the 256-wide tree, then the fold of
  * the tiles' partials. */
-typedef struct { int order; const double* a; const double* b; } c1_row;
+typedef struct { const double* a; const double* b; } c1_row;
 
 static void cg1_sums(const ho_grid* g, const c1_geom* q, const c1_row* rows, int nrows, double* part,
                      double* out) {
@@ -1148,20 +1145,9 @@ static void cg1_sums(const ho_grid* g, const c1_geom* q, const c1_row* rows, int
             double s[F_NT];
             for (int tid = 0; tid < F_NT; ++tid) {
                 double acc = 0.0;
-                for (int xp = ix0; xp < ix1; ++xp) {
-                    if (rows[k].order == 0) {
-                        for (int j = 0; j < C1_XR; ++j) {
-                            const int i = tid + j * F_NT;
-                            if (i >= C1_XN) continue;
-                            const int r = i / C1_XW, c = i % C1_XW;
-                            const int yy = y0 - 1 + r, zz = z0 - 1 + c;
-                            if (r < 1 || r > C1_TY || c < 1 || c > C1_TZ || yy > q->m || zz > q->m) continue;
-                            const long id = lnid(g, xp, yy, zz);
-                            acc += rows[k].b ? rows[k].a[id] * rows[k].b[id] : rows[k].a[id];
-                        }
-                    } else {
-                        const int yy = y0 + tid / C1_TZ, zz = z0 + tid % C1_TZ;
-                        if (yy > q->m || zz > q->m) continue;
+                const int yy = y0 + tid / C1_TZ, zz = z0 + tid % C1_TZ;
+                if (yy <= q->m && zz <= q->m) {
+                    for (int xp = ix0; xp < ix1; ++xp) {
                         const long id = lnid(g, xp, yy, zz);
                         acc += rows[k].b ? rows[k].a[id] * rows[k].b[id] : rows[k].a[id];
                     }
@@ -1219,8 +1205,8 @@ int ho_pcg_solve_mixed_cg1(ho_op* op, const double* b, double* x, int max_outer,
         ho_apply_operator_f32(op, u, w, op->dt, 1);
         for (long i = 0; i < n; ++i) dw[i] = w[i];
         double t[8];
-        const c1_row start_rows[8] = {{0, dr, du}, {1, dw, du}, {0, dr, dr}, {1, dw, NULL},
-                                      {1, op->detect ? op->wA : du, du}, {1, du, du}, {0, bcp, bcp}, {0, r64, r64}};
+        const c1_row start_rows[8] = {{dr, du}, {dw, du}, {dr, dr}, {dw, NULL},
+                                      {op->detect ? op->wA : du, du}, {du, du}, {bcp, bcp}, {r64, r64}};
         cg1_sums(g, &q, start_rows, 8, part, t);
         double bn = 1.0, gamma_prev = 0.0, alpha_prev = 0.0;
         int outer_done = 0;
@@ -1270,8 +1256,8 @@ int ho_pcg_solve_mixed_cg1(ho_op* op, const double* b, double* x, int max_outer,
             }
             ho_apply_operator_f32(op, u, w, op->dt, 1);
             for (long i = 0; i < n; ++i) dw[i] = w[i];
-            const c1_row step_rows[6] = {{0, dr, du}, {1, dw, du}, {0, dr, dr}, {1, dw, NULL},
-                                         {1, op->detect ? op->wA : du, du}, {1, du, du}};
+            const c1_row step_rows[6] = {{dr, du}, {dw, du}, {dr, dr}, {dw, NULL},
+                                         {op->detect ? op->wA : du, du}, {du, du}};
             cg1_sums(g, &q, step_rows, op->detect ? 6 : 3, part, t);
             ++total;
         }
