@@ -106,17 +106,19 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, q = lane >> 4;                        // RHS column; k / row group
     const int qx = q & 1, qy = q >> 1;                               // B operand node (x, y) of group q
-    // A: row = lane % 16; lane group q holds node q's three terms and a zero,
-    // then node 4 + q's (the same slots as the B operand's)
-    mr_v4u au;
+    // A: row = lane % 16; lane group q holds in one half of its K slots node
+    // q's three terms and a zero, in the other node 4 + q's (the same slots
+    // as the B operand's).  Element kz of a row keeps its lower-z nodes in
+    // the half its upper-z nodes had in element kz - 1, so the B operand is
+    // updated in place: even kz use amat[0] (lower z first), odd kz amat[1].
+    mr_v8bf amat[2];
     {
         const unsigned c0 = __float_as_uint(mrhs_coef(col, q)) >> 16, c1 = __float_as_uint(mrhs_coef(col, 4 + q)) >> 16;
-        au[0] = c0 | (c0 << 16);
-        au[1] = c0;
-        au[2] = c1 | (c1 << 16);
-        au[3] = c1;
+        const mr_v4u ae = {c0 | (c0 << 16), c0, c1 | (c1 << 16), c1};
+        const mr_v4u ao = {c1 | (c1 << 16), c1, c0 | (c0 << 16), c0};
+        amat[0] = __builtin_bit_cast(mr_v8bf, ae);
+        amat[1] = __builtin_bit_cast(mr_v8bf, ao);
     }
-    const mr_v8bf amat = __builtin_bit_cast(mr_v8bf, au);
     constexpr int NK = SUMS == 2 ? 4 : 1;                            // sums per column
     double acc_s[NK][4];                                             // this thread's 4 columns: part = tid & 3
 #pragma unroll
@@ -213,21 +215,34 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
                 const float* __restrict__ ek_row = &ec[ex & 1][1][ky * EZ];
                 const float* __restrict__ x_row = xlane + ky * WZ * MR;
                 float* __restrict__ y_row = yq + (ky - 1) * YROW;          // column 0 of its rows
-                unsigned bp0, bp1;
-                mrhs_split(x_row[0], bp0, bp1);
+                mr_v4u bu;                              // the B operand: two halves of split words
+                {
+                    unsigned w0, w1;
+                    mrhs_split(x_row[0], w0, w1);
+                    bu[0] = w0;
+                    bu[1] = w1;
+                    bu[2] = 0u;
+                    bu[3] = 0u;
+                }
                 float carry = 0.f;
                 auto batch = [&](auto k0c, auto nbc) __attribute__((always_inline)) {
                     constexpr int K0 = decltype(k0c)::value, NB = decltype(nbc)::value;
                     mr_v4f c[NB];
 #pragma unroll
                     for (int i = 0; i < NB; ++i) {
-                        unsigned bn0, bn1;
-                        mrhs_split(x_row[(K0 + i + 1) * MR], bn0, bn1);
-                        const mr_v4u bu = {bp0, bp1, bn0, bn1};
+                        const int kz = K0 + i;             // a constant once the loop is unrolled
+                        unsigned w0, w1;
+                        mrhs_split(x_row[(kz + 1) * MR], w0, w1);
+                        if (kz & 1) {
+                            bu[0] = w0;
+                            bu[1] = w1;
+                        } else {
+                            bu[2] = w0;
+                            bu[3] = w1;
+                        }
                         const mr_v4f zero = {0.f, 0.f, 0.f, 0.f};
-                        c[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(amat, __builtin_bit_cast(mr_v8bf, bu), zero, 0, 0, 0);
-                        bp0 = bn0;
-                        bp1 = bn1;
+                        c[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(amat[kz & 1], __builtin_bit_cast(mr_v8bf, bu),
+                                                                       zero, 0, 0, 0);
                     }
 #pragma unroll
                     for (int i = 0; i < NB; ++i) {
