@@ -1056,8 +1056,9 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
 // is k_apply_elem<.., true> above and stays as the comparison form).
 //
 // A block owns a tile of STY x STZ elements in (y, z) and walks a chunk of
-// element planes in x.  The x values of the tile's node footprint (9 x 33
-// nodes per plane) are staged in LDS, three planes in a ring.  Per element
+// element planes in x.  The x values of the tile's node footprint
+// ((STY + 1) x (STZ + 1) nodes per plane) are staged in LDS, three planes in
+// a ring.  Per element
 // plane, each thread forms its element's eight rows (the closed-form rows of
 // k_apply_elem) and parks them in LDS, one array per local node; then each
 // thread gathers, for its footprint nodes, the four contributions of the
@@ -1070,18 +1071,28 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
 // order of the adds at nodes where tiles or chunks meet is not fixed: the
 // result differs from the colour order by rounding only, as the reference's
 // atomic scatter does.
+//
+// Tiles are 512 elements: 16 x 32, 8 x 64 or 4 x 128 in (y, z), the
+// narrowest whose z extent covers the mesh's (4 x 128 beyond).  Atomic adds
+// at the z faces of a tile (one node per footprint row, each in its own
+// cache line) cost far more than those at its y faces (whole rows), so a
+// tile spanning the mesh in z leaves only y faces and chunk planes.
 // ---------------------------------------------------------------------------
-constexpr int STY = 8, STZ = 32, SNT = STY * STZ;             // element tile (y, z)
-constexpr int SFY = STY + 1, SFZ = STZ + 1, SFN = SFY * SFZ;  // node footprint of a plane: 9 x 33
-constexpr int SFR = (SFN + SNT - 1) / SNT;                    // footprint nodes per thread
+constexpr int SNT = 512;                                      // elements (threads) per tile
 constexpr int SCX = 8;                                        // target element planes per chunk
 
-template <bool MASS, bool PROJECT>
+// the tile's z extent for a mesh of n elements per edge (STY = SNT / STZ)
+inline int scatter_tile_z(int n) { return n <= 32 ? 32 : (n <= 64 ? 64 : 128); }
+
+template <bool MASS, bool PROJECT, int STZ>
 __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict__ x, double* __restrict__ y,
                                                       const double* __restrict__ cr,
                                                       const double* __restrict__ ck, Grid g, int nby,
                                                       int nbz, int nch, int nblocks, const DevState* st,
                                                       int check_done) {
+    constexpr int STY = SNT / STZ;                                // element tile (y, z)
+    constexpr int SFY = STY + 1, SFZ = STZ + 1, SFN = SFY * SFZ;  // node footprint of a plane
+    constexpr int SFR = (SFN + SNT - 1) / SNT;                    // footprint nodes per thread
     // element rows by local node, on the tile's elements with a zero border
     // of one element all round, so the gather below needs no bounds check
     constexpr int CBW = STZ + 2, CBN = (STY + 2) * CBW;

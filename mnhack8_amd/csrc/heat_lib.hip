@@ -358,19 +358,26 @@ int element_apply(heat_op* op, const double* x, double* y, double dt_eff, FiArm 
     if (tiled_atomic_path(op) && !f.armed) {
         // the tiled scatter (an armed level-R fault goes through the
         // element kernel, which carries the flip)
-        const int nby = (g.n + STY - 1) / STY, nbz = (g.n + STZ - 1) / STZ, nch = scatter_chunks(g.n);
+        const int tz = scatter_tile_z(g.n), ty = SNT / tz;
+        const int nby = (g.n + ty - 1) / ty, nbz = (g.n + tz - 1) / tz, nch = scatter_chunks(g.n);
         const int nblocks = nby * nbz * nch;
         const unsigned grid = (unsigned)(((nblocks + NXCD - 1) / NXCD) * NXCD);
         const bool mass = dt_eff == 0.0;
         const DevState* st = op->st;
-        if (mass && project)
-            k_apply_scatter<true, true><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
-        else if (mass)
-            k_apply_scatter<true, false><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
-        else if (project)
-            k_apply_scatter<false, true><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
-        else
-            k_apply_scatter<false, false><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        auto launch = [&](auto tzc) {
+            constexpr int TZ = decltype(tzc)::value;
+            if (mass && project)
+                k_apply_scatter<true, true, TZ><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+            else if (mass)
+                k_apply_scatter<true, false, TZ><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+            else if (project)
+                k_apply_scatter<false, true, TZ><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+            else
+                k_apply_scatter<false, false, TZ><<<grid, SNT, 0, s>>>(x, y, op->cr, op->ck, g, nby, nbz, nch, nblocks, st, check_done);
+        };
+        if (tz == 32) launch(std::integral_constant<int, 32>{});
+        else if (tz == 64) launch(std::integral_constant<int, 64>{});
+        else launch(std::integral_constant<int, 128>{});
         CHECK_LAUNCH();
         if (projected) *projected = project;
         return HEAT_OK;
