@@ -742,27 +742,61 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
     // and finds element plane ix-1 in E[k & 1], loads ix into E[(k & 1) ^ 1]
     T W[3][3][3];
     Coef<T> E[2][2][2];
+    // prologue: x planes ix0-1 and ix0 and element plane ix0-1 reach the
+    // idle half H1 by the same copies (x slots 0 and 1, element slot 0),
+    // together with the first batch into H0; the window is then read from
+    // LDS like every later plane
+    {
+        // copies of x plane xp into slot s of H1 (xp in 0..n) and of element plane ep into slot 0
+        auto issue_x = [&](int xp, int s) __attribute__((always_inline)) {
+            const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<T*>(x + (size_t)xp * S), 0, (int)(S * sizeof(T)), RSRC3);
+#pragma unroll
+            for (int j = 0; j < XJ; ++j)
+                if (xo[j] >= 0)
+                    lds_copy16(rx, (lds_void*)(reinterpret_cast<char*>(&H1.xs[s][0]) + 16 * (j * NTH + lane_base)),
+                               xo[j]);
+        };
+        if (ix0 >= 1) {
+            issue_x(ix0 - 1, 0);
+            const int ep = ix0 - 1;
+            const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<T*>(cr + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
+            const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<T*>(ck + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
+#pragma unroll
+            for (int j = 0; j < EJ; ++j)
+                if (eo[j] >= 0) {
+                    lds_copy16(rr, (lds_void*)(reinterpret_cast<char*>(&H1.er[0][0]) + 16 * (j * NTH + lane_base)),
+                               eo[j]);
+                    if (!MASS)
+                        lds_copy16(rk, (lds_void*)(reinterpret_cast<char*>(&H1.ek[0][0]) + 16 * (j * NTH + lane_base)),
+                                   eo[j]);
+                }
+        }
+        issue_x(ix0, 1);
+    }
+    issue(H0, ix0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
     if (interior) {
-        const size_t q = (size_t)iy * nn1 + iz;
         if (ix0 >= 1) {
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    W[0][bb][c] = x[(size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+                for (int c = 0; c < 3; ++c) W[0][bb][c] = H1.xs[0][xidx[bb][c]];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
                 for (int cz = 0; cz < 2; ++cz) {
-                    const size_t e = (size_t)(ix0 - 1) * nsq + eoff[cy][cz];
-                    E[0][cy][cz].cr = cr[e];
-                    E[0][cy][cz].ck = MASS ? T(0) : ck[e];
+                    E[0][cy][cz].cr = H1.er[0][eidx[cy][cz]];
+                    E[0][cy][cz].ck = MASS ? T(0) : H1.ek[0][eidx[cy][cz]];
                 }
         }
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+            for (int c = 0; c < 3; ++c) W[1][bb][c] = H1.xs[1][xidx[bb][c]];
     }
     const int par_yz = (py << 1) | pz;
     // an armed level-R fault is applied by k_apply_gather (the host routes that
@@ -825,7 +859,6 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
         step(HC ? H1 : H0, kc, ix);
         return true;
     };
-    issue(H0, ix0);
     for (int ixg = ix0; ixg < ix1; ixg += GL)
         if (!tdma_steps([&](auto kc) __attribute__((always_inline)) { return one(ixg, kc); },
                         std::make_integer_sequence<int, GL>{}))
