@@ -284,14 +284,11 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
         eq[k] = ey * n + ez;
     }
     const int wq = xq[0];
-    int eidx[2][2], eoff[2][2];
+    int eidx[2][2];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int cz = 0; cz < 2; ++cz) {
-            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * C1EW + (tz + 1 - (cz ^ pz));
-            eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
-        }
+        for (int cz = 0; cz < 2; ++cz) eidx[cy][cz] = (ty + 1 - (cy ^ py)) * C1EW + (tz + 1 - (cz ^ pz));
     int widx[3][3];
 #pragma unroll
     for (int b2 = 0; b2 < 3; ++b2)
@@ -410,20 +407,21 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
             }
         }
     };
-    float W[3][3][3];
+    PlaneT<float> PT[3];                            // terms of the u window planes X-1, X, X+1
+    float Xc[3];                                    // u at the own node of each window plane
     Coef<float> Eprev[2][2], Enew[2][2];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a) {
+        Xc[a] = 0.0f;
 #pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2)
+        for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[a][b2][c] = 0.0f;
+            for (int cz = 0; cz < 2; ++cz) PT[a].F[cy][cz] = PT[a].DC[cy][cz] = PT[a].C4[cy][cz] = 0.0f;
+    }
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz) Eprev[cy][cz] = Enew[cy][cz] = Coef<float>{0.0f, 0.0f};
-    const int par_yz = (py << 1) | pz;
-    const FiArm nofi{0, 0, 0};
     // one incoming plane P: form it, load plane P + AHEAD into the same
     // stage, then the stencil of plane P - 1 (stored and summed if it is one
     // of the chunk's planes)
@@ -434,10 +432,15 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
         const double wa = DETECT ? St.wa : 0.0;
         fetch(P + AHEAD, St);
         __syncthreads();
+        {
+            float Wn[3][3];
 #pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2)
+            for (int b2 = 0; b2 < 3; ++b2)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[2][b2][c] = us[slot][widx[b2][c]];
+                for (int c = 0; c < 3; ++c) Wn[b2][c] = us[slot][widx[b2][c]];
+            plane_terms(Wn, PT[2]);
+            Xc[2] = Wn[1][1];
+        }
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -447,10 +450,10 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
             }
         const int X = P - 1;                            // the plane whose stencil is now complete
         const bool xact = X >= ix0 && X < ix1;          // uniform over the block
-        const float wn = (X & 1) ? march_node<float, false, 1>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi)
-                                 : march_node<float, false, 0>(W, Eprev, Enew, par_yz, X, eoff, nsq, nofi);
+        const float wn = (X & 1) ? node_rows<float, false, 1>(PT[0], PT[1], PT[2], Eprev, Enew)
+                                 : node_rows<float, false, 0>(PT[0], PT[1], PT[2], Eprev, Enew);
         // u is 0 at a position outside the mesh, so such a lane adds exact zeros
-        const float uc = W[1][1][1];
+        const float uc = Xc[1];
         c1_stf(wn, R_wout, (xact && node_in) ? (unsigned)(X * S + wq) * 4u : C1_OOB);
         const double wd = (double)wn, ud = (double)uc;
         a_dl += xact ? wd * ud : 0.0;
@@ -459,13 +462,10 @@ __device__ __forceinline__ void cg1_body(const Cg1View& v, int b, int par, float
             a_c += xact ? wa * ud : 0.0;
             a_uu += xact ? ud * ud : 0.0;
         }
-#pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                W[0][b2][c] = W[1][b2][c];
-                W[1][b2][c] = W[2][b2][c];
-            }
+        PT[0] = PT[1];
+        PT[1] = PT[2];
+        Xc[0] = Xc[1];
+        Xc[1] = Xc[2];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll

@@ -57,26 +57,118 @@ __device__ __forceinline__ double flip_bit(double v, int bit) {
     return __longlong_as_double((long long)w);
 }
 
-// One row of the cube's element matrix (rhoc M + dt k K) applied to x_e.
-// S_d sums x over the element's nodes at Hamming distance d from the row
-// node.  M_ij = m3*{8,4,2,1}[d] and K_ij = k2*{-4,0,1,1}[d] are the closed
-// forms the reference checks its quadrature against
-// (src/heat_solver_het.cu:677-705); cr = rhoc*m3, ck = (dt*k)*k2.
-__device__ __forceinline__ double struct_row(double S0, double S1, double S2, double S3,
-                                             double cr, double ck) {
-    double t1 = fma(2.0, S2, S3);
-    double t2 = fma(4.0, S1, t1);
-    double P = fma(8.0, S0, t2);
-    double Q = fma(-4.0, S0, S2 + S3);
-    return fma(cr, P, ck * Q);
+// Element rows, plane-factored.
+//
+// Row i of the cube's element matrix (rhoc M + dt k K) applied to x_e: with
+// M_ij = m3*{8,4,2,1}[d] and K_ij = k2*{-4,0,1,1}[d], d the Hamming distance
+// of the local nodes (the closed forms the reference checks its quadrature
+// against, src/heat_solver_het.cu:677-705), the row is cr*Mr + ck*Kr with
+// cr = rhoc*m3, ck = (dt*k)*k2 and, writing x_p(b, c) for the element's
+// value in x-plane p at y-row b and z-column c (own = node i's, o = the
+// other one):
+//   Mr = 2 F(own plane) + F(other plane)
+//   Kr = DC(other plane) + C4(own plane)
+//   F(p)  = 2 G(p, own row) + G(p, o row),   G(p, b) = 2 x_p(b, own) + x_p(b, o)
+//   DC(p) = (x_p(o, own) + x_p(own, o)) + x_p(o, o)
+//   C4(p) = x_p(o, o) - 4 x_p(own, own)
+// (M's weights are the tensor product of {2, 1} per axis; K's row is
+// S2 + S3 - 4 S0).  In colour order the row is added to the node's sum as
+// acc = fma(ck, Kr, fma(cr, Mr, acc)); a mass-only apply (dt_eff = 0) adds
+// fma(cr, Mr, acc).  F, DC and C4 of a plane depend only on the node's (y, z)
+// position and the element's yz quadrant, so a walk along x forms them once
+// per plane for the three nodes that see the plane.  Every fused multiply-add
+// is written out; the CPU restatement repeats each rounding.
+template <typename T>
+struct PlaneT {
+    T F[2][2], DC[2][2], C4[2][2];   // by the colour bits (cy, cz) of the yz quadrant
+};
+
+// The terms of one (y, z) window plane Wp, mirrored so that Wp[1][1] is the
+// node and the quadrant of colour bits (cy, cz) has its other row at
+// cy ? 0 : 2 and its other column at cz ? 0 : 2.
+template <typename T>
+__device__ __forceinline__ void plane_terms(const T (&Wp)[3][3], PlaneT<T>& o) {
+    T G[3][2];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        G[b][0] = fma(T(2), Wp[b][1], Wp[b][2]);    // cz = 0: other column 2
+        G[b][1] = fma(T(2), Wp[b][1], Wp[b][0]);    // cz = 1: other column 0
+    }
+#pragma unroll
+    for (int cy = 0; cy < 2; ++cy)
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            const int oy = cy ? 0 : 2, oz = cz ? 0 : 2;
+            o.F[cy][cz] = fma(T(2), G[1][cz], G[oy][cz]);
+            o.DC[cy][cz] = (Wp[oy][1] + Wp[1][oz]) + Wp[oy][oz];
+            o.C4[cy][cz] = fma(T(-4), Wp[1][1], Wp[oy][oz]);
+        }
 }
-__device__ __forceinline__ float struct_row(float S0, float S1, float S2, float S3,
-                                            float cr, float ck) {
-    float t1 = fmaf(2.0f, S2, S3);
-    float t2 = fmaf(4.0f, S1, t1);
-    float P = fmaf(8.0f, S0, t2);
-    float Q = fmaf(-4.0f, S0, S2 + S3);
-    return fmaf(cr, P, ck * Q);
+
+// Offsets (x, y, z) of local node i = 0..7: (0,0,0), (1,0,0), (1,1,0),
+// (0,1,0), (0,0,1), (1,0,1), (1,1,1), (0,1,1) (the reference's N0..N7,
+// src/heat_solver_het.cu:261-263), as compile-time functions so that a loop
+// over i unrolls into constant register and LDS indices
+__host__ __device__ constexpr int lox(int i) { return ((i & 3) == 1 || (i & 3) == 2) ? 1 : 0; }
+__host__ __device__ constexpr int loy(int i) { return (i & 3) >= 2 ? 1 : 0; }
+__host__ __device__ constexpr int loz(int i) { return i >> 2; }
+// local node of offsets (a, b, c) (the inverse of lox, loy, loz)
+__host__ __device__ constexpr int lnode(int a, int b, int c) { return (c << 2) | (b ? (a ? 2 : 3) : (a ? 1 : 0)); }
+
+// Mr and Kr of local node i from the element's eight values xe (by local node)
+template <typename T>
+__device__ __forceinline__ void elem_row_terms(const T (&xe)[8], int i, T& Mr, T& Kr) {
+    const int ox = lox(i), oy = loy(i), oz = loz(i);
+    T F[2], DC[2], C4[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {                   // s = 0: own plane, 1: the other one
+        const int p = s ? 1 - ox : ox;
+        const T aa = xe[lnode(p, oy, oz)], ab = xe[lnode(p, oy, 1 - oz)];
+        const T ba = xe[lnode(p, 1 - oy, oz)], bb = xe[lnode(p, 1 - oy, 1 - oz)];
+        F[s] = fma(T(2), fma(T(2), aa, ab), fma(T(2), ba, bb));
+        DC[s] = (ba + ab) + bb;
+        C4[s] = fma(T(-4), aa, bb);
+    }
+    Mr = fma(T(2), F[0], F[1]);
+    Kr = DC[1] + C4[0];
+}
+
+// a row added to a colour-ordered sum, and a row on its own (the atomic
+// forms' contributions and a flipped one)
+template <typename T, bool MASS>
+__device__ __forceinline__ T add_row(T acc, T cr, T ck, T Mr, T Kr) {
+    acc = fma(cr, Mr, acc);
+    return MASS ? acc : fma(ck, Kr, acc);
+}
+template <typename T, bool MASS>
+__device__ __forceinline__ T row_value(T cr, T ck, T Mr, T Kr) {
+    return MASS ? cr * Mr : fma(ck, Kr, cr * Mr);
+}
+
+// The colour-ordered sum of a node from its mirrored 3x3x3 window m
+// (m[1][1][1] the node; along each axis the colour-c element's other node at
+// index 0 when c's bit is set, else 2), the coefficients crc / ckc of the
+// colour-c elements and the mask of the colours whose element exists (the
+// others add nothing); fcol: the colour whose row is added as a value with
+// bit fbit flipped (-1: none).
+template <typename T, bool MASS>
+__device__ __forceinline__ T gather_rows(const T (&m)[3][3][3], const T (&crc)[8], const T (&ckc)[8],
+                                         int emask = 0xff, int fcol = -1, int fbit = 0) {
+    PlaneT<T> P[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) plane_terms(m[a], P[a]);
+    T acc = T(0);
+#pragma unroll
+    for (int col = 0; col < 8; ++col) {
+        if (!((emask >> col) & 1)) continue;
+        const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
+        const PlaneT<T>& A = cx ? P[0] : P[2];
+        const T Mr = fma(T(2), P[1].F[cy][cz], A.F[cy][cz]);
+        const T Kr = A.DC[cy][cz] + P[1].C4[cy][cz];
+        if (col == fcol) acc += (T)flip_bit((double)row_value<T, MASS>(crc[col], ckc[col], Mr, Kr), fbit);
+        else acc = add_row<T, MASS>(acc, crc[col], ckc[col], Mr, Kr);
+    }
+    return acc;
 }
 
 __device__ __forceinline__ bool node_is_boundary(const Grid& g, int ix, int iy, int iz) {
@@ -210,23 +302,19 @@ __global__ __launch_bounds__(NT) void k_apply_gather(
                         m[a][b][c] = x[nid + (a - 1) * DX + (b - 1) * DY + (c - 1) * DZ];
             const int par = (px << 2) | (py << 1) | pz;
             const int nsq = g.n * g.n;
+            T crc[8], ckc[8];
+            int fcol = -1;
 #pragma unroll
             for (int col = 0; col < 8; ++col) {
                 const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
-                // mirrored index of the element's other node along each axis
-                const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
                 const int sa = cx ^ px, sb = cy ^ py, sc = cz ^ pz;   // element below n?
                 const int e = (ix - sa) * nsq + (iy - sb) * g.n + (iz - sc);
-                const T S0 = m[1][1][1];
-                const T S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
-                const T S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
-                const T S3 = m[ox][oy][oz];
-                // mass-only apply (dt_eff = 0): the stiffness coefficient is 0
-                T v = struct_row(S0, S1, S2, S3, cr[e], MASS ? T(0) : ck[e]);
-                if (fi.armed && col == par && (long long)e == fi.target)
-                    v = (T)flip_bit((double)v, fi.bit);
-                acc += v;
+                crc[col] = cr[e];
+                ckc[col] = MASS ? T(0) : ck[e];
+                // the target element's row at its local node 0 (src/heat_solver_het.cu:369-374)
+                if (fi.armed && col == par && (long long)e == fi.target) fcol = col;
             }
+            acc = gather_rows<T, MASS>(m, crc, ckc, 0xff, fcol, fi.bit);
         }
         y[nid] = acc;   // boundary rows: 0 (projection)
     }
@@ -274,59 +362,33 @@ struct Coef {
     T cr, ck;
 };
 
-// march_node with the window's planes rotated by ROT (plane q of the step
-// lives in slot (q + ROT) % 3) and the element planes in E[EROT] (below)
-// and E[EROT ^ 1] (above): a walk that renames instead of moving registers.
-template <typename T, bool MASS, int PX, int ROT, int EROT>
-__device__ __forceinline__ T march_node_rot(const T (&W)[3][3][3], const Coef<T> (&E)[2][2][2], int par_yz,
-                                            int ix, const int (&eoff)[2][2], int nsq, const FiArm& fi) {
-    constexpr int Q0 = ROT % 3, Q1 = (ROT + 1) % 3, Q2 = (ROT + 2) % 3;
+// The eight colour-ordered rows of a node from the terms of the window
+// planes below (Lo), at (Ow) and above (Hi) the node; PX = the node's x
+// parity (the colour-cx element lies below iff cx ^ PX), Ebelow / Eabove the
+// coefficients of the element planes below and above.
+template <typename T, bool MASS, int PX>
+__device__ __forceinline__ T node_rows(const PlaneT<T>& Lo, const PlaneT<T>& Ow, const PlaneT<T>& Hi,
+                                       const Coef<T> (&Ebelow)[2][2], const Coef<T> (&Eabove)[2][2]) {
     T acc = T(0);
 #pragma unroll
     for (int col = 0; col < 8; ++col) {
         const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
-        const int sa = cx ^ PX;                 // 1: the element lies below the node in x
-        const int ax = sa ? Q0 : Q2;            // window slot of the element's other x-node
-        const int oy = cy ? 0 : 2, oz = cz ? 0 : 2;
-        const T S0 = W[Q1][1][1];
-        const T S1 = (W[ax][1][1] + W[Q1][oy][1]) + W[Q1][1][oz];
-        const T S2 = (W[ax][oy][1] + W[ax][1][oz]) + W[Q1][oy][oz];
-        const T S3 = W[ax][oy][oz];
-        const Coef<T>& Ee = sa ? E[EROT][cy][cz] : E[EROT ^ 1][cy][cz];
-        T v = struct_row(S0, S1, S2, S3, Ee.cr, MASS ? T(0) : Ee.ck);
-        if (fi.armed && col == ((PX << 2) | par_yz)) {
-            const long long e = (long long)(ix - sa) * nsq + eoff[cy][cz];
-            if (e == fi.target) v = (T)flip_bit((double)v, fi.bit);
-        }
-        acc += v;
+        const bool below = (cx ^ PX) != 0;
+        const PlaneT<T>& A = below ? Lo : Hi;
+        const Coef<T>& E = below ? Ebelow[cy][cz] : Eabove[cy][cz];
+        const T Mr = fma(T(2), Ow.F[cy][cz], A.F[cy][cz]);
+        const T Kr = A.DC[cy][cz] + Ow.C4[cy][cz];
+        acc = add_row<T, MASS>(acc, E.cr, E.ck, Mr, Kr);
     }
     return acc;
 }
 
-template <typename T, bool MASS, int PX>
-__device__ __forceinline__ T march_node(const T (&W)[3][3][3], const Coef<T> (&Eprev)[2][2],
-                                        const Coef<T> (&Enew)[2][2], int par_yz, int ix,
-                                        const int (&eoff)[2][2], int nsq, const FiArm& fi) {
-    T acc = T(0);
-#pragma unroll
-    for (int col = 0; col < 8; ++col) {
-        const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
-        const int sa = cx ^ PX;                 // 1: the element lies below the node in x
-        const int ax = sa ? 0 : 2;              // window plane of the element's other x-node
-        const int oy = cy ? 0 : 2, oz = cz ? 0 : 2;
-        const T S0 = W[1][1][1];
-        const T S1 = (W[ax][1][1] + W[1][oy][1]) + W[1][1][oz];
-        const T S2 = (W[ax][oy][1] + W[ax][1][oz]) + W[1][oy][oz];
-        const T S3 = W[ax][oy][oz];
-        const Coef<T>& E = sa ? Eprev[cy][cz] : Enew[cy][cz];
-        T v = struct_row(S0, S1, S2, S3, E.cr, MASS ? T(0) : E.ck);
-        if (fi.armed && col == ((PX << 2) | par_yz)) {
-            const long long e = (long long)(ix - sa) * nsq + eoff[cy][cz];
-            if (e == fi.target) v = (T)flip_bit((double)v, fi.bit);
-        }
-        acc += v;
-    }
-    return acc;
+// node_rows of a walk that renames instead of moving registers: the window
+// planes of the step in PT slots (q + ROT) % 3 for q = below, at, above, the
+// element planes in E[EROT] (below) and E[EROT ^ 1] (above)
+template <typename T, bool MASS, int PX, int ROT, int EROT>
+__device__ __forceinline__ T node_rows_rot(const PlaneT<T> (&PT)[3], const Coef<T> (&E)[2][2][2]) {
+    return node_rows<T, MASS, PX>(PT[ROT % 3], PT[(ROT + 1) % 3], PT[(ROT + 2) % 3], E[EROT], E[EROT ^ 1]);
 }
 
 template <typename T, bool MASS, int CX>
@@ -358,14 +420,16 @@ __global__ __launch_bounds__(NT) void k_apply_march(const T* __restrict__ x, T* 
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz) eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
-    T W[3][3][3];
+    PlaneT<T> PT[3];                            // terms of window planes ix-1, ix, ix+1
     Coef<T> Eprev[2][2], Enew[2][2];
     auto load_plane = [&](int slot, int ix) {
         const T* xp = x + (size_t)ix * S + q;
+        T Wn[3][3];
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[slot][bb][c] = xp[(bb - 1) * DY + (c - 1) * DZ];
+            for (int c = 0; c < 3; ++c) Wn[bb][c] = xp[(bb - 1) * DY + (c - 1) * DZ];
+        plane_terms(Wn, PT[slot]);
     };
     auto load_elems = [&](Coef<T> (&E)[2][2], int ex) {
 #pragma unroll
@@ -382,26 +446,21 @@ __global__ __launch_bounds__(NT) void k_apply_march(const T* __restrict__ x, T* 
         load_elems(Eprev, ix0 - 1);
     }
     load_plane(1, ix0);
-    const int par_yz = (py << 1) | pz;
+    (void)fi;                                   // an armed fault goes through k_apply_gather
     for (int ix = ix0; ix < ix1; ++ix) {
         T acc = T(0);
         if (ix >= 1 && ix <= n - 1) {
             load_plane(2, ix + 1);
             load_elems(Enew, ix);
-            if (ix & 1) acc = march_node<T, MASS, 1>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
-            else acc = march_node<T, MASS, 0>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+            if (ix & 1) acc = node_rows<T, MASS, 1>(PT[0], PT[1], PT[2], Eprev, Enew);
+            else acc = node_rows<T, MASS, 0>(PT[0], PT[1], PT[2], Eprev, Enew);
         } else if (ix == 0) {
             load_plane(2, 1);
             load_elems(Enew, 0);
         }
         y[(size_t)ix * S + q] = acc;
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                W[0][bb][c] = W[1][bb][c];
-                W[1][bb][c] = W[2][bb][c];
-            }
+        PT[0] = PT[1];
+        PT[1] = PT[2];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -511,17 +570,19 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
         for (int cz = 0; cz < 2; ++cz) eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
-    T W[3][3][3];
+    PlaneT<T> PT[3];                            // terms of window planes ix-1, ix, ix+1
     Coef<T> Eprev[2][2], Enew[2][2];
     // prologue: planes ix0-1 and ix0 and element plane ix0-1 straight from memory
     if (interior) {
         const size_t q = (size_t)iy * nn1 + iz;
+        T Wn[3][3];
         if (ix0 >= 1) {
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    W[0][bb][c] = x[(size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+                    Wn[bb][c] = x[(size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+            plane_terms(Wn, PT[0]);
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -534,20 +595,23 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[1][bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+            for (int c = 0; c < 3; ++c) Wn[bb][c] = x[(size_t)ix0 * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz];
+        plane_terms(Wn, PT[1]);
     }
     fetch(ix0 + 1, ix0);
     park((ix0 + 1) & 1, ix0 & 1);
     __syncthreads();
-    const int par_yz = (py << 1) | pz;
+    (void)fi;                                   // an armed fault goes through k_apply_gather
     for (int ix = ix0; ix < ix1; ++ix) {
         fetch(ix + 2, ix + 1);                     // in flight during this step
         if (interior && ix <= n - 1) {
             const T* xp = xs[(ix + 1) & 1];
+            T Wn[3][3];
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[2][bb][c] = xp[(ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz)];
+                for (int c = 0; c < 3; ++c) Wn[bb][c] = xp[(ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz)];
+            plane_terms(Wn, PT[2]);
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -559,8 +623,8 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
         if (inmesh) {
             T acc = T(0);
             if (interior && ix >= 1 && ix <= n - 1) {
-                if (ix & 1) acc = march_node<T, MASS, 1>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
-                else acc = march_node<T, MASS, 0>(W, Eprev, Enew, par_yz, ix, eoff, nsq, fi);
+                if (ix & 1) acc = node_rows<T, MASS, 1>(PT[0], PT[1], PT[2], Eprev, Enew);
+                else acc = node_rows<T, MASS, 0>(PT[0], PT[1], PT[2], Eprev, Enew);
             }
             const size_t base = (size_t)ix * S;
             y[base + (size_t)iy * nn1 + iz] = acc;
@@ -570,13 +634,8 @@ __global__ __launch_bounds__(TY* TZ) void k_apply_tile(const T* __restrict__ x, 
         }
         park(ix & 1, (ix + 1) & 1);
         __syncthreads();
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                W[0][bb][c] = W[1][bb][c];
-                W[1][bb][c] = W[2][bb][c];
-            }
+        PT[0] = PT[1];
+        PT[1] = PT[2];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -723,24 +782,22 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
             }
         }
     };
-    int eidx[2][2], eoff[2][2];
+    int eidx[2][2];
 #pragma unroll
     for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
-        for (int cz = 0; cz < 2; ++cz) {
-            eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EWP + (tz + 1 - (cz ^ pz));
-            eoff[cy][cz] = (iy - (cy ^ py)) * n + (iz - (cz ^ pz));
-        }
+        for (int cz = 0; cz < 2; ++cz) eidx[cy][cz] = (ty + 1 - (cy ^ py)) * EWP + (tz + 1 - (cz ^ pz));
     int xidx[3][3];
 #pragma unroll
     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
         for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XWP + (tz + 1 + (c - 1) * sz);
-    // window planes rotate through the slots of W (period 3) and element
-    // planes through E (period 2): step k of a group finds plane
-    // ix-1 in W[k % 3], ix in W[(k+1) % 3], loads ix+1 into W[(k+2) % 3],
+    // window planes (their terms) rotate through the slots of PT (period 3)
+    // and element planes through E (period 2): step k of a group finds plane
+    // ix-1 in PT[k % 3], ix in PT[(k+1) % 3], loads ix+1 into PT[(k+2) % 3],
     // and finds element plane ix-1 in E[k & 1], loads ix into E[(k & 1) ^ 1]
-    T W[3][3][3];
+    PlaneT<T> PT[3];
+    T Xc[3];                                // the node's own x value of each window plane (SUMS)
     Coef<T> E[2][2][2];
     // prologue: x planes ix0-1 and ix0 and element plane ix0-1 reach the
     // idle half H1 by the same copies (x slots 0 and 1, element slot 0),
@@ -780,11 +837,14 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (interior) {
+        T Wn[3][3];
         if (ix0 >= 1) {
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[0][bb][c] = H1.xs[0][xidx[bb][c]];
+                for (int c = 0; c < 3; ++c) Wn[bb][c] = H1.xs[0][xidx[bb][c]];
+            plane_terms(Wn, PT[0]);
+            Xc[0] = Wn[1][1];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -796,22 +856,25 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[1][bb][c] = H1.xs[1][xidx[bb][c]];
+            for (int c = 0; c < 3; ++c) Wn[bb][c] = H1.xs[1][xidx[bb][c]];
+        plane_terms(Wn, PT[1]);
+        Xc[1] = Wn[1][1];
     }
-    const int par_yz = (py << 1) | pz;
     // an armed level-R fault is applied by k_apply_gather (the host routes that
     // one apply there); this walk never flips
-    const FiArm nofi{0, 0, 0};
     // one node plane, step k of the group: k & 1 = ix parity (chunks start
     // on even planes), k % BP = the plane's slot in its batch
     auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
         constexpr int p = k & 1, q = k % TDMA_BP;
         if (interior && ix <= n - 1) {
+            T Wn[3][3];
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[(k + 2) % 3][bb][c] = Hh.xs[q][xidx[bb][c]];
+                for (int c = 0; c < 3; ++c) Wn[bb][c] = Hh.xs[q][xidx[bb][c]];
+            plane_terms(Wn, PT[(k + 2) % 3]);
+            Xc[(k + 2) % 3] = Wn[1][1];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -823,9 +886,9 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
         if (inmesh) {
             T acc = T(0);
             if (interior && ix >= 1 && ix <= n - 1) {
-                acc = march_node_rot<T, MASS, p, k % 3, p>(W, E, par_yz, ix, eoff, nsq, nofi);
+                acc = node_rows_rot<T, MASS, p, k % 3, p>(PT, E);
                 if (SUMS) {
-                    s_pap += (double)W[(k + 1) % 3][1][1] * (double)acc;
+                    s_pap += (double)Xc[(k + 1) % 3] * (double)acc;
                     s_ap += (double)acc;
                 }
             }
@@ -905,13 +968,6 @@ __device__ __forceinline__ bool resolve_element(const Grid& g, int tid, int colo
     return true;
 }
 
-// Offsets (x, y, z) of local node i = 0..7: (0,0,0), (1,0,0), (1,1,0),
-// (0,1,0), (0,0,1), (1,0,1), (1,1,1), (0,1,1) (the reference's N0..N7,
-// src/heat_solver_het.cu:261-263), as compile-time functions so that a loop
-// over i unrolls into constant register and LDS indices
-__host__ __device__ constexpr int lox(int i) { return ((i & 3) == 1 || (i & 3) == 2) ? 1 : 0; }
-__host__ __device__ constexpr int loy(int i) { return (i & 3) >= 2 ? 1 : 0; }
-__host__ __device__ constexpr int loz(int i) { return i >> 2; }
 
 // The two-point Gauss rule on the unit interval: abscissae t0 = (1 - 1/sqrt(3))/2
 // and t1 = 1 - t0, weights 1/2.  At t_p the two linear hats phi_0 = 1 - t and
@@ -1063,17 +1119,30 @@ __global__ __launch_bounds__(ET) void k_apply_elem(const T* __restrict__ x, T* _
             crv = (T)((float)rhoc[e] * (float)m3);
             ckv = (T)(((float)dt_eff * (float)kcell[e]) * (float)k2);
         }
+        const bool mass = dt_eff == 0.0;
+        const bool flip = fi.armed && (long long)e == fi.target;
+        T Mr[8], Kr[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int ox = lox(i), oy = loy(i), oz = loz(i);
-            // local index of the node with the given offsets (inverse of lox, loy, loz)
-            auto L = [](int a, int b, int c) { return (c << 2) | (b ? (a ? 2 : 3) : (a ? 1 : 0)); };
-            const T S0 = xe[i];
-            const T S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
-            const T S2 = (xe[L(1 - ox, 1 - oy, oz)] + xe[L(1 - ox, oy, 1 - oz)]) + xe[L(ox, 1 - oy, 1 - oz)];
-            const T S3 = xe[L(1 - ox, 1 - oy, 1 - oz)];
-            ye[i] = struct_row(S0, S1, S2, S3, crv, ckv);
+        for (int i = 0; i < 8; ++i) elem_row_terms(xe, i, Mr[i], Kr[i]);
+        if (!ATOMIC) {
+            // colour sweep: the rows go into y in colour order (a flipped row,
+            // the target's local node 0, is added as a value)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i == 0 && flip) {
+                    const T v = mass ? row_value<T, true>(crv, ckv, Mr[0], Kr[0])
+                                     : row_value<T, false>(crv, ckv, Mr[0], Kr[0]);
+                    y[gid[0]] += (T)flip_bit((double)v, fi.bit);
+                } else {
+                    y[gid[i]] = mass ? add_row<T, true>(y[gid[i]], crv, ckv, Mr[i], Kr[i])
+                                     : add_row<T, false>(y[gid[i]], crv, ckv, Mr[i], Kr[i]);
+                }
+            }
+            return;
         }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            ye[i] = mass ? row_value<T, true>(crv, ckv, Mr[i], Kr[i]) : row_value<T, false>(crv, ckv, Mr[i], Kr[i]);
     }
     if (fi.armed && (long long)e == fi.target) ye[0] = (T)flip_bit((double)ye[0], fi.bit);
 #pragma unroll
@@ -1247,13 +1316,10 @@ __global__ __launch_bounds__(SNT) void k_apply_scatter(const double* __restrict_
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                const int ox = lox(i), oy = loy(i), oz = loz(i);
-                const double S0 = xe[i];
-                const double S1 = (xe[L(1 - ox, oy, oz)] + xe[L(ox, 1 - oy, oz)]) + xe[L(ox, oy, 1 - oz)];
-                const double S2 = (xe[L(1 - ox, 1 - oy, oz)] + xe[L(1 - ox, oy, 1 - oz)]) + xe[L(ox, 1 - oy, 1 - oz)];
-                const double S3 = xe[L(1 - ox, 1 - oy, 1 - oz)];
+                double Mr, Kr;
+                elem_row_terms(xe, i, Mr, Kr);
                 // elements outside the mesh have zero coefficients: zero rows
-                cb[i][(ty + 1) * CBW + tz + 1] = struct_row(S0, S1, S2, S3, c_r, MASS ? 0.0 : c_k);
+                cb[i][(ty + 1) * CBW + tz + 1] = row_value<double, MASS>(c_r, c_k, Mr, Kr);
             }
         }
         __syncthreads();
@@ -1401,17 +1467,15 @@ __global__ __launch_bounds__(NT) void k_mixed_residual(
                     for (int c = 0; c < 3; ++c)
                         m[a][bb][c] = x[nid + (a - 1) * DX + (bb - 1) * DY + (c - 1) * DZ];
             const int nsq = g.n * g.n;
+            double crc[8], ckc[8];
 #pragma unroll
             for (int col = 0; col < 8; ++col) {
                 const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
-                const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
                 const int e = (ix - (cx ^ px)) * nsq + (iy - (cy ^ py)) * g.n + (iz - (cz ^ pz));
-                const double S0 = m[1][1][1];
-                const double S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
-                const double S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
-                const double S3 = m[ox][oy][oz];
-                acc += struct_row(S0, S1, S2, S3, cr[e], ck[e]);
+                crc[col] = cr[e];
+                ckc[col] = ck[e];
             }
+            acc = gather_rows<double, false>(m, crc, ckc);
         }
         const double bi = b[nid];
         const double ri = fma(-1.0, acc, bi);

@@ -504,16 +504,20 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
         for (int c = 0; c < 3; ++c) xidx[bb][c] = (ty + 1 + (bb - 1) * sy) * XW + (tz + 1 + (c - 1) * sz);
-    T W[3][3][3];
+    PlaneT<T> PT[3];                            // terms of window planes ix-1, ix, ix+1
+    T Xc[3];                                    // the node's own value of each window plane
     Coef<T> Eprev[2][2], Enew[2][2];
     // prologue: plane ix0-1 of the window and element plane ix0-1 straight from memory
     if (interior && ix0 >= 1) {
         const size_t q = (size_t)iy * nn1 + iz;
+        T Wn[3][3];
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                W[0][bb][c] = pnew((size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz);
+                Wn[bb][c] = pnew((size_t)(ix0 - 1) * S + q + (bb - 1) * sy * nn1 + (c - 1) * sz);
+        plane_terms(Wn, PT[0]);
+        Xc[0] = Wn[1][1];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -528,25 +532,29 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
     park(0, 0);
     __syncthreads();
     if (interior) {
+        T Wn[3][3];
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) W[1][bb][c] = xs[0][xidx[bb][c]];
+            for (int c = 0; c < 3; ++c) Wn[bb][c] = xs[0][xidx[bb][c]];
+        plane_terms(Wn, PT[1]);
+        Xc[1] = Wn[1][1];
     }
     __syncthreads();
     fetch(ix0 + 1, ix0);
     park((ix0 + 1) & 1, ix0 & 1);
     __syncthreads();
-    const int par_yz = (py << 1) | pz;
-    const FiArm nofi{0, 0, 0};
     for (int ix = ix0; ix < ix1; ++ix) {
         fetch(ix + 2, ix + 1);
         if (interior && ix <= n - 1) {
             const T* xp = xs[(ix + 1) & 1];
+            T Wn[3][3];
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) W[2][bb][c] = xp[xidx[bb][c]];
+                for (int c = 0; c < 3; ++c) Wn[bb][c] = xp[xidx[bb][c]];
+            plane_terms(Wn, PT[2]);
+            Xc[2] = Wn[1][1];
 #pragma unroll
             for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -558,8 +566,8 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
         if (inmesh) {
             T acc = T(0);
             if (interior && ix >= 1 && ix <= n - 1) {
-                if (ix & 1) acc = march_node<T, false, 1>(W, Eprev, Enew, par_yz, ix, eoff, nsq, nofi);
-                else acc = march_node<T, false, 0>(W, Eprev, Enew, par_yz, ix, eoff, nsq, nofi);
+                if (ix & 1) acc = node_rows<T, false, 1>(PT[0], PT[1], PT[2], Eprev, Enew);
+                else acc = node_rows<T, false, 0>(PT[0], PT[1], PT[2], Eprev, Enew);
             }
             const size_t base = (size_t)ix * S;
             const size_t me = base + (size_t)iy * nn1 + iz;
@@ -579,19 +587,16 @@ __device__ __forceinline__ void fused_apply_body(const FusedView<T>& v, int b) {
             // product leaves a sum that starts at +0 unchanged)
             const bool int_node = interior && ix >= 1 && ix <= n - 1;
             if (int_node) {
-                a_pap += (double)W[1][1][1] * (double)acc;
+                a_pap += (double)Xc[1] * (double)acc;
                 if (DETECT) a_s += (double)acc;
             }
         }
         park(ix & 1, (ix + 1) & 1);
         __syncthreads();
-#pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                W[0][bb][c] = W[1][bb][c];
-                W[1][bb][c] = W[2][bb][c];
-            }
+        PT[0] = PT[1];
+        PT[1] = PT[2];
+        Xc[0] = Xc[1];
+        Xc[1] = Xc[2];
 #pragma unroll
         for (int cy = 0; cy < 2; ++cy)
 #pragma unroll
@@ -959,21 +964,20 @@ __device__ __forceinline__ T node_apply_any(const T* __restrict__ x, const T* __
                 const bool ok = X >= 0 && X <= n && Y >= 0 && Y <= n && Z >= 0 && Z <= n;
                 m[a][b][c] = ok ? x[(size_t)X * g.sx + (size_t)Y * g.sy + Z] : T(0);
             }
-    T acc = T(0);
+    T crc[8], ckc[8];
+    int emask = 0;
 #pragma unroll
     for (int col = 0; col < 8; ++col) {
         const int cx = (col >> 2) & 1, cy = (col >> 1) & 1, cz = col & 1;
         const int ex = ix - (cx ^ px), ey = iy - (cy ^ py), ez = iz - (cz ^ pz);
+        crc[col] = ckc[col] = T(0);
         if (ex < 0 || ex >= n || ey < 0 || ey >= n || ez < 0 || ez >= n) continue;
-        const int ox = cx ? 0 : 2, oy = cy ? 0 : 2, oz = cz ? 0 : 2;
         const size_t e = (size_t)ex * nsq + (size_t)ey * n + ez;
-        const T S0 = m[1][1][1];
-        const T S1 = (m[ox][1][1] + m[1][oy][1]) + m[1][1][oz];
-        const T S2 = (m[ox][oy][1] + m[ox][1][oz]) + m[1][oy][oz];
-        const T S3 = m[ox][oy][oz];
-        acc += struct_row(S0, S1, S2, S3, cr[e], MASS ? T(0) : ck[e]);
+        crc[col] = cr[e];
+        ckc[col] = MASS ? T(0) : ck[e];
+        emask |= 1 << col;
     }
-    return acc;
+    return gather_rows<T, MASS>(m, crc, ckc, emask);
 }
 
 // b = P(M T) [+ dt F] for every node of every active solve: build_rhs's

@@ -254,41 +254,48 @@ static void elem_apply_gauss(const ho_op* op, const double xe[8], double ye[8],
     }
 }
 
-/* Closed-form row of the cube element matrix.  With d(i,j) the Hamming
- * distance of the local nodes, M_ij = m3 * {8,4,2,1}[d] and K_ij =
- * {-4 k2, 0, k2, k2}[d] (the values checked at src/heat_solver_het.cu:677-705),
- * so row i of (rhoc M + dt k K) x is
- *   rhoc*m3*(8 S0 + 4 S1 + 2 S2 + S3) + (dt*k)*k2*(S2 + S3 - 4 S0)
- * with S_d the sum of x over the nodes at distance d from i.  The operation
- * order below is the one the HIP kernels use, so results agree bit for bit. */
-static inline double struct_row(double S0, double S1, double S2, double S3,
-                                double cr, double ck) {
-    double t1 = fma(2.0, S2, S3);
-    double t2 = fma(4.0, S1, t1);
-    double P = fma(8.0, S0, t2);
-    double Q = fma(-4.0, S0, S2 + S3);
-    return fma(cr, P, ck * Q);
-}
-static inline float struct_row_f(float S0, float S1, float S2, float S3, float cr, float ck) {
-    float t1 = fmaf(2.0f, S2, S3);
-    float t2 = fmaf(4.0f, S1, t1);
-    float P = fmaf(8.0f, S0, t2);
-    float Q = fmaf(-4.0f, S0, S2 + S3);
-    return fmaf(cr, P, ck * Q);
-}
-
-static void elem_apply_struct(const ho_op* op, const double xe[8], double ye[8],
-                              double k_e, double rhoc_e, double dt) {
-    const double cr = rhoc_e * op->m3;
-    const double ck = (dt * k_e) * op->k2;
-    for (int i = 0; i < 8; ++i) {
-        int ox = OFF[i][0], oy = OFF[i][1], oz = OFF[i][2];
-        double S0 = xe[i];
-        double S1 = (xe[LOC[1 - ox][oy][oz]] + xe[LOC[ox][1 - oy][oz]]) + xe[LOC[ox][oy][1 - oz]];
-        double S2 = (xe[LOC[1 - ox][1 - oy][oz]] + xe[LOC[1 - ox][oy][1 - oz]]) + xe[LOC[ox][1 - oy][1 - oz]];
-        double S3 = xe[LOC[1 - ox][1 - oy][1 - oz]];
-        ye[i] = struct_row(S0, S1, S2, S3, cr, ck);
+/* Closed-form rows of the cube element matrix, plane-factored.  With d(i,j)
+ * the Hamming distance of the local nodes, M_ij = m3 * {8,4,2,1}[d] and
+ * K_ij = {-4 k2, 0, k2, k2}[d] (the values checked at
+ * src/heat_solver_het.cu:677-705), so row i of (rhoc M + dt k K) x is
+ * cr*Mr + ck*Kr with cr = rhoc*m3, ck = (dt*k)*k2 and, x_p(b,c) the element's
+ * value in x-plane p at y-row b and z-column c (own = node i's, o = the other):
+ *   Mr = 2 F(own plane) + F(other plane),  F(p) = 2 G(p, own row) + G(p, o row),
+ *        G(p, b) = 2 x_p(b, own) + x_p(b, o)          (= 8 S0 + 4 S1 + 2 S2 + S3)
+ *   Kr = DC(other plane) + C4(own plane),  DC(p) = (x_p(o, own) + x_p(own, o)) + x_p(o, o),
+ *        C4(p) = x_p(o, o) - 4 x_p(own, own)           (= S2 + S3 - 4 S0)
+ * with S_d the sum of x over the nodes at distance d from i.  In colour order
+ * a row is added as y = fma(ck, Kr, fma(cr, Mr, y)) (mass-only, dt_eff = 0:
+ * y = fma(cr, Mr, y)); a row taken on its own (the atomic order, a flipped row)
+ * is fma(ck, Kr, cr*Mr) (mass-only: cr*Mr).  The operation order is the one
+ * the HIP kernels use, so results agree bit for bit. */
+static void row_terms(const double xe[8], int i, double* Mr, double* Kr) {
+    const int ox = OFF[i][0], oy = OFF[i][1], oz = OFF[i][2];
+    double F[2], DC[2], C4[2];
+    for (int s = 0; s < 2; ++s) {               /* s = 0: the own plane, 1: the other */
+        const int p = s ? 1 - ox : ox;
+        const double aa = xe[LOC[p][oy][oz]], ab = xe[LOC[p][oy][1 - oz]];
+        const double ba = xe[LOC[p][1 - oy][oz]], bb = xe[LOC[p][1 - oy][1 - oz]];
+        F[s] = fma(2.0, fma(2.0, aa, ab), fma(2.0, ba, bb));
+        DC[s] = (ba + ab) + bb;
+        C4[s] = fma(-4.0, aa, bb);
     }
+    *Mr = fma(2.0, F[0], F[1]);
+    *Kr = DC[1] + C4[0];
+}
+static void row_terms_f(const float xe[8], int i, float* Mr, float* Kr) {
+    const int ox = OFF[i][0], oy = OFF[i][1], oz = OFF[i][2];
+    float F[2], DC[2], C4[2];
+    for (int s = 0; s < 2; ++s) {
+        const int p = s ? 1 - ox : ox;
+        const float aa = xe[LOC[p][oy][oz]], ab = xe[LOC[p][oy][1 - oz]];
+        const float ba = xe[LOC[p][1 - oy][oz]], bb = xe[LOC[p][1 - oy][1 - oz]];
+        F[s] = fmaf(2.0f, fmaf(2.0f, aa, ab), fmaf(2.0f, ba, bb));
+        DC[s] = (ba + ab) + bb;
+        C4[s] = fmaf(-4.0f, aa, bb);
+    }
+    *Mr = fmaf(2.0f, F[0], F[1]);
+    *Kr = DC[1] + C4[0];
 }
 
 static inline uint64_t d2u(double v) { uint64_t u; memcpy(&u, &v, 8); return u; }
@@ -302,11 +309,31 @@ static void operator_element(ho_op* op, const double* x, double* y, double dt_ef
     double xe[8], ye[8];
     for (int i = 0; i < 8; ++i) xe[i] = x[gid[i]];
     const double k_e = op->k[e], rhoc_e = op->rhoc[e];
+    const int flip = fi->armed && fi->level == 2 && fi->rank == 0 && e == fi->target;
+    if (op->kernel == HO_KERNEL_STRUCT) {
+        /* the closed-form rows: in colour order each row goes into y as
+         * fma(ck, Kr, fma(cr, Mr, y)); the atomic order and a flipped row
+         * (local node 0 of the target) add the row as a value */
+        const double cr = rhoc_e * op->m3;
+        const double ck = (dt_eff * k_e) * op->k2;
+        const int mass = dt_eff == 0.0;
+        for (int i = 0; i < 8; ++i) {
+            double Mr, Kr;
+            row_terms(xe, i, &Mr, &Kr);
+            double* yi = &y[gid[i]];
+            if (op->colored && !(i == 0 && flip)) {
+                *yi = mass ? fma(cr, Mr, *yi) : fma(ck, Kr, fma(cr, Mr, *yi));
+            } else {
+                double v = mass ? cr * Mr : fma(ck, Kr, cr * Mr);
+                if (i == 0 && flip) v = u2d(d2u(v) ^ (1ULL << fi->bit));
+                *yi += v;
+            }
+        }
+        return;
+    }
     if (op->kernel == HO_KERNEL_FAST) elem_apply_fast(op, xe, ye, k_e, rhoc_e, dt_eff);
-    else if (op->kernel == HO_KERNEL_GAUSS) elem_apply_gauss(op, xe, ye, k_e, rhoc_e, dt_eff, ix, iy, iz);
-    else elem_apply_struct(op, xe, ye, k_e, rhoc_e, dt_eff);
-    if (fi->armed && fi->level == 2 && fi->rank == 0 && e == fi->target)
-        ye[0] = u2d(d2u(ye[0]) ^ (1ULL << fi->bit));
+    else elem_apply_gauss(op, xe, ye, k_e, rhoc_e, dt_eff, ix, iy, iz);
+    if (flip) ye[0] = u2d(d2u(ye[0]) ^ (1ULL << fi->bit));
     for (int i = 0; i < 8; ++i) y[gid[i]] += ye[i];
 }
 
@@ -424,6 +451,7 @@ void ho_apply_operator(ho_op* op, const double* x, double* y, double dt_eff,
 void ho_apply_operator_f32(ho_op* op, const float* x, float* y, double dt_eff, int project) {
     const ho_grid* g = &op->g;
     const float m3 = (float)op->m3, k2 = (float)op->k2;
+    const int mass = dt_eff == 0.0;
     memset(y, 0, sizeof(float) * g->nn);
     for (int c = 0; c < 8; ++c) {
         long cnt = color_count(g, c);
@@ -437,12 +465,10 @@ void ho_apply_operator_f32(ho_op* op, const float* x, float* y, double dt_eff, i
             const float cr = (float)op->rhoc[e] * m3;
             const float ck = ((float)dt_eff * (float)op->k[e]) * k2;
             for (int i = 0; i < 8; ++i) {
-                int ox = OFF[i][0], oy = OFF[i][1], oz = OFF[i][2];
-                float S0 = xe[i];
-                float S1 = (xe[LOC[1 - ox][oy][oz]] + xe[LOC[ox][1 - oy][oz]]) + xe[LOC[ox][oy][1 - oz]];
-                float S2 = (xe[LOC[1 - ox][1 - oy][oz]] + xe[LOC[1 - ox][oy][1 - oz]]) + xe[LOC[ox][1 - oy][1 - oz]];
-                float S3 = xe[LOC[1 - ox][1 - oy][1 - oz]];
-                y[gid[i]] += struct_row_f(S0, S1, S2, S3, cr, ck);
+                float Mr, Kr;
+                row_terms_f(xe, i, &Mr, &Kr);
+                float* yi = &y[gid[i]];
+                *yi = mass ? fmaf(cr, Mr, *yi) : fmaf(ck, Kr, fmaf(cr, Mr, *yi));
             }
         }
     }
