@@ -1902,7 +1902,9 @@ int heat_autotune_pcg(heat_op* op, int which, int iters, int* planes, int* wave,
     op->pcg_mode = HEAT_PCG_FUSED;
     op->inner_mode = HEAT_INNER_ONE_LAUNCH;
     const int cand_planes[] = {4, 8, 12, 16, 24, 32, 64};
-    const int cand_wave[] = {0, FWAVE};
+    // whole waves of 3 resident blocks per CU (the fused kernels) or, for the
+    // one-launch step (its LDS ring and registers allow 4), of 4
+    const int cand_wave[] = {0, FWAVE, which == 1 ? C1WAVE4 : FWAVE};
     int pick_p = keep_planes, pick_w = keep_wave;
     double best = 1e300;
     std::vector<int> seen;
