@@ -18,8 +18,9 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 # (planes, wave): one plane pair per chunk, short and long chunks with and
-# without the whole-wave fill, and one chunk for the whole column
-GEOMETRIES = [(2, 0), (4, 0), (6, 768), (32, 0), (64, 768), (1 << 20, 0)]
+# without the whole-wave fill (waves of 768 tiles, and of 1024 for the
+# one-launch step's four blocks per CU), and one chunk for the whole column
+GEOMETRIES = [(2, 0), (4, 0), (6, 768), (8, 1024), (32, 0), (64, 768), (1 << 20, 0)]
 
 
 def _solve_both(op, orc, n, precision):
