@@ -700,6 +700,22 @@ __device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t r, lds_void* d
 #endif
 }
 
+// Raw buffer store of one value: an offset past the resource's size drops
+// the store, so a lane that has nothing to store takes such an offset
+// instead of a branch.
+typedef unsigned tdma_v2u __attribute__((ext_vector_type(2)));
+constexpr unsigned TDMA_OOB = 0x7ffffff0u;
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tdma_v2u, v), r, (int)off, 0, 0);
+#endif
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+#endif
+}
+
 // The walk of tile b (of nby x nbz columns x nch chunks) through the LDS
 // ring H0 / H1.  With SUMS, each thread also sums p.Ap (x at the node times
 // the node's y) and 1.Ap over its interior nodes, in plane order.
@@ -752,35 +768,42 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
     // 32-bit per-lane byte offsets, so no 64-bit address arithmetic per copy.
     // Word 3 of the resource is the gfx9-family raw-buffer configuration.
     constexpr int RSRC3 = 0x00020000;
+    // (the lanes without a unit are masked once per tile kind and batch)
     auto issue = [&](TdmaHalf<T, MASS, TY, TZ>& Hh, int ixb) __attribute__((always_inline)) {
 #pragma unroll
-        for (int p = 0; p < TDMA_BP; ++p) {
-            const int xp = ixb + 1 + p, ep = ixb + p;
-            if (xp <= n) {
-                const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<T*>(x + (size_t)xp * S), 0, (int)(S * sizeof(T)), RSRC3);
+        for (int j = 0; j < XJ; ++j)
+            if (xo[j] >= 0) {
 #pragma unroll
-                for (int j = 0; j < XJ; ++j)
-                    if (xo[j] >= 0)
+                for (int p = 0; p < TDMA_BP; ++p) {
+                    const int xp = ixb + 1 + p;
+                    if (xp <= n) {
+                        const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+                            const_cast<T*>(x + (size_t)xp * S), 0, (int)(S * sizeof(T)), RSRC3);
                         lds_copy16(rx, (lds_void*)(reinterpret_cast<char*>(&Hh.xs[p][0]) + 16 * (j * NTH + lane_base)),
                                    xo[j]);
+                    }
+                }
             }
-            if (ep < n) {
-                const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<T*>(cr + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
-                const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(
-                    const_cast<T*>(ck + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
 #pragma unroll
-                for (int j = 0; j < EJ; ++j)
-                    if (eo[j] >= 0) {
+        for (int j = 0; j < EJ; ++j)
+            if (eo[j] >= 0) {
+#pragma unroll
+                for (int p = 0; p < TDMA_BP; ++p) {
+                    const int ep = ixb + p;
+                    if (ep < n) {
+                        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+                            const_cast<T*>(cr + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
                         lds_copy16(rr, (lds_void*)(reinterpret_cast<char*>(&Hh.er[p][0]) + 16 * (j * NTH + lane_base)),
                                    eo[j]);
-                        if (!MASS)
+                        if (!MASS) {
+                            const __amdgpu_buffer_rsrc_t rk = __builtin_amdgcn_make_buffer_rsrc(
+                                const_cast<T*>(ck + (size_t)ep * nsq), 0, (int)(nsq * sizeof(T)), RSRC3);
                             lds_copy16(rk, (lds_void*)(reinterpret_cast<char*>(&Hh.ek[p][0]) + 16 * (j * NTH + lane_base)),
                                        eo[j]);
+                        }
                     }
+                }
             }
-        }
     };
     int eidx[2][2];
 #pragma unroll
@@ -864,10 +887,19 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
     // one apply there); this walk never flips
     // one node plane, step k of the group: k & 1 = ix parity (chunks start
     // on even planes), k % BP = the plane's slot in its batch
+    // Every lane reads its window and coefficients and forms its rows, with
+    // no branch: a lane off the interior (or a plane past the mesh's
+    // interior) reads positions inside the LDS tiles whose values no kept
+    // result uses, and its result is replaced by the projection's 0.  y goes
+    // out by buffer stores with per-lane offsets formed once (a lane with
+    // nothing to store takes an out-of-range offset); the boundary rows
+    // iy = 0 and iz = 0, which no tile owns, are zeroed by the first tile
+    // row and column.
+    const unsigned yo_node = inmesh ? (unsigned)((iy * nn1 + iz) * (int)sizeof(T)) : TDMA_OOB;
     auto step = [&](const TdmaHalf<T, MASS, TY, TZ>& Hh, auto kc, int ix) __attribute__((always_inline)) {
         constexpr int k = decltype(kc)::value;
         constexpr int p = k & 1, q = k % TDMA_BP;
-        if (interior && ix <= n - 1) {
+        if (interior) {
             T Wn[3][3];
 #pragma unroll
             for (int bb = 0; bb < 3; ++bb)
@@ -883,21 +915,19 @@ __device__ __forceinline__ void tdma_body(const T* __restrict__ x, T* __restrict
                     E[p ^ 1][cy][cz].ck = MASS ? T(0) : Hh.ek[q][eidx[cy][cz]];
                 }
         }
-        if (inmesh) {
-            T acc = T(0);
-            if (interior && ix >= 1 && ix <= n - 1) {
-                acc = node_rows_rot<T, MASS, p, k % 3, p>(PT, E);
-                if (SUMS) {
-                    s_pap += (double)Xc[(k + 1) % 3] * (double)acc;
-                    s_ap += (double)acc;
-                }
-            }
-            T* yp = y + (size_t)ix * S;                            // plane base; 32-bit offsets in it
-            yp[iy * nn1 + iz] = acc;
-            if (by == 0 && ty == 0) yp[iz] = T(0);
-            if (bz == 0 && tz == 0) yp[iy * nn1] = T(0);
-            if (by == 0 && bz == 0 && tid == 0) yp[0] = T(0);
+        const bool live = interior && ix >= 1 && ix <= n - 1;
+        const T rows = node_rows_rot<T, MASS, p, k % 3, p>(PT, E);
+        const T acc = live ? rows : T(0);
+        if (SUMS) {
+            s_pap += live ? (double)Xc[(k + 1) % 3] * (double)acc : 0.0;
+            s_ap += live ? (double)acc : 0.0;
         }
+        const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(
+            y + (size_t)ix * S, 0, (int)(S * sizeof(T)), RSRC3);
+        buf_store(acc, ry, yo_node);
+        if (by == 0) buf_store(T(0), ry, (inmesh && ty == 0) ? (unsigned)(iz * (int)sizeof(T)) : TDMA_OOB);
+        if (bz == 0) buf_store(T(0), ry, (inmesh && tz == 0) ? (unsigned)(iy * nn1 * (int)sizeof(T)) : TDMA_OOB);
+        if (by == 0 && bz == 0) buf_store(T(0), ry, tid == 0 ? 0u : TDMA_OOB);
         // keep the next step's loads below this step: hoisting them would
         // keep two windows live at once
         __builtin_amdgcn_sched_barrier(0);
