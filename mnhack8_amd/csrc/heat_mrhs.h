@@ -127,34 +127,47 @@ __global__ __launch_bounds__(MNT) void k_mrhs_apply(const float* __restrict__ X,
         for (int j = 0; j < 4; ++j) acc_s[k][j] = 0.0;
 
     // ---- staging: node window of plane p and element coefficients of plane e
+    // This thread's window units and elements are the same on every plane:
+    // their in-plane offsets (floats of X, elements of cr / ck) and whether
+    // they lie on the mesh are formed once; a plane adds its base.
+    int xoff[XR], eoff[ER];
+    bool xok[XR], eok[ER];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int v = tid + k * MNT;
+        const int w = v >> 2, part = v & 3;
+        const int wy = w / WZ, wz = w - (w / WZ) * WZ;
+        const int yy = y0 - 1 + wy, zz = z0 - 1 + wz;
+        xok[k] = v < V4 && yy >= 0 && yy <= n && zz >= 0 && zz <= n;
+        xoff[k] = xok[k] ? (yy * nn1 + zz) * MR + part * 4 : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < ER; ++k) {
+        const int v = tid + k * MNT;
+        const int ky = v / EZ, kz = v - (v / EZ) * EZ;
+        const int ey = y0 - 1 + ky, ez = z0 - 1 + kz;
+        eok[k] = v < EN && ey >= 0 && ey < n && ez >= 0 && ez < n;
+        eoff[k] = eok[k] ? ey * n + ez : 0;
+    }
     float4 xf[XR];
     float ef[2][ER];
     auto fetch = [&](int p, int e) {
+        const bool pin = p >= 0 && p <= n, ein = e >= 0 && e < n;          // uniform over the block
+        const float* Xp = X + (size_t)(pin ? p : 0) * S * MR;
+        const float* crp = cr + (size_t)(ein ? e : 0) * nsq;
+        const float* ckp = ck + (size_t)(ein ? e : 0) * nsq;
 #pragma unroll
         for (int k = 0; k < XR; ++k) {
-            const int v = tid + k * MNT;
             float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (v < V4) {
-                const int w = v >> 2, part = v & 3;
-                const int wy = w / WZ, wz = w - (w / WZ) * WZ;
-                const int yy = y0 - 1 + wy, zz = z0 - 1 + wz;
-                if (p >= 0 && p <= n && yy >= 0 && yy <= n && zz >= 0 && zz <= n)
-                    val = *reinterpret_cast<const float4*>(X + ((size_t)p * S + (size_t)yy * nn1 + zz) * MR + part * 4);
-            }
+            if (pin && xok[k]) val = *reinterpret_cast<const float4*>(Xp + xoff[k]);
             xf[k] = val;
         }
 #pragma unroll
         for (int k = 0; k < ER; ++k) {
-            const int v = tid + k * MNT;
             float c0 = 0.f, c1 = 0.f;
-            if (v < EN) {
-                const int ky = v / EZ, kz = v - (v / EZ) * EZ;
-                const int ey = y0 - 1 + ky, ez = z0 - 1 + kz;
-                if (e >= 0 && e < n && ey >= 0 && ey < n && ez >= 0 && ez < n) {
-                    const size_t ei = (size_t)e * nsq + (size_t)ey * n + ez;
-                    c0 = cr[ei];
-                    c1 = MASS ? 0.f : ck[ei];
-                }
+            if (ein && eok[k]) {
+                c0 = crp[eoff[k]];
+                c1 = MASS ? 0.f : ckp[eoff[k]];
             }
             ef[0][k] = c0;
             ef[1][k] = c1;
