@@ -1562,11 +1562,28 @@ __global__ __launch_bounds__(NT) void k_mixed_correct(double* __restrict__ x,
 // ---------------------------------------------------------------------------
 // Setup and utility kernels
 // ---------------------------------------------------------------------------
-// y = 0 before a scatter inside a captured PCG (returns once the solve is done)
-__global__ __launch_bounds__(NT) void k_zero_unless_done(double* __restrict__ v, int n, const DevState* st) {
-    if (st->done) return;
-    const int i = blockIdx.x * NT + threadIdx.x;
-    if (i < n) v[i] = 0.0;
+// v[0, n) = 0 with 16-byte stores, ZU pairs of doubles per thread (the zero y
+// of the atomic scatter).  A vector of a batch may start on an odd double, so
+// a v that is not 16-byte aligned gets its first element from a scalar store,
+// and an odd remainder its last.  CHECK: nothing once the solve is done.
+constexpr int ZU = 4;
+inline int zero_blocks(int n) { return n / 2 > 0 ? (n / 2 + NT * ZU - 1) / (NT * ZU) : 1; }
+template <bool CHECK>
+__global__ __launch_bounds__(NT) void k_zero(double* __restrict__ v, int n, const DevState* st) {
+    if (CHECK && st->done) return;
+    const int head = (reinterpret_cast<uintptr_t>(v) & 15) ? 1 : 0;
+    const int pairs = (n - head) / 2;
+    double2* vp = reinterpret_cast<double2*>(v + head);
+    const int t = blockIdx.x * NT + threadIdx.x, stride = gridDim.x * NT;
+#pragma unroll
+    for (int u = 0; u < ZU; ++u) {
+        const int i = t + u * stride;
+        if (i < pairs) vp[i] = make_double2(0.0, 0.0);
+    }
+    if (t == 0 && n > 0) {
+        if (head) v[0] = 0.0;
+        if ((n - head) & 1) v[n - 1] = 0.0;
+    }
 }
 
 template <typename T>

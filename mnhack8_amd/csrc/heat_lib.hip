@@ -351,8 +351,8 @@ int element_apply(heat_op* op, const double* x, double* y, double dt_eff, FiArm 
     const Grid& g = op->g;
     hipStream_t s = op->stream;
     if (projected) *projected = false;
-    if (check_done) k_zero_unless_done<<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, op->st);
-    else k_fill<double><<<blocks(g.nn, NT), NT, 0, s>>>(y, g.nn, 0.0);
+    if (check_done) k_zero<true><<<zero_blocks(g.nn), NT, 0, s>>>(y, g.nn, op->st);
+    else k_zero<false><<<zero_blocks(g.nn), NT, 0, s>>>(y, g.nn, op->st);
     CHECK_LAUNCH();
     const bool gauss = op->cfg.kernel == HEAT_KERNEL_GAUSS;
     if (tiled_atomic_path(op) && !f.armed) {
